@@ -1,0 +1,185 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X LM bundle adjuster.
+
+Default workload (one GPU): the "mid" scene — 200 cameras with 9 parameters
+(angle-axis, translation, f, k1, k2), 50k points, ~1.0M keys, each point seen
+by 3..40 cameras (truncated geometric, mean 20) — solved in mixed precision
+(FP32 Jacobian / Hessian / Schur, FP64 Cholesky, cost and gain ratio).
+
+A "step" is one LM iteration (J + N + g build, Schur, FP64 Cholesky, point
+back-substitution, candidate cost, accept / reject).  The timed region runs
+exactly --steps iterations from x0 with every input already resident in HBM,
+bracketed by device synchronisation.  Prints one JSON line.
+
+Other workloads: --workload ring | windows | large | serving.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from hs_sfm_amd import abi, synthetic as syn  # noqa: E402
+from hs_sfm_amd.hsba import Batch, Problem  # noqa: E402
+
+METRIC = "BA LM iters/s (1M obs) + batched solves/s, p99 ms, at FP64-matched RMSE"
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def pair_flops(scene: syn.Scene, E: int) -> float:
+    """Schur products of one trial: every point contributes k(k+1)/2 camera
+    pairs, each an E x E block of 3-term dot products (6 E^2 flops), plus the
+    right-hand side Z^T q per key (6 E flops)."""
+    k = np.bincount(scene.feature_maps[:, 1], minlength=scene.num_points).astype(np.float64)
+    return float(np.sum(k * (k + 1) / 2) * 6 * E * E + scene.num_keys * 6 * E)
+
+
+def linearize_bytes(scene: syn.Scene, E: int, tbytes: int) -> float:
+    """K1+K2 algorithmic HBM bytes: per key the observation (16 B), image id
+    (4 B), point id (4 B) and the stored W block (3 E T), per point its
+    parameters (24 B), CSR offset (4 B) and V / g blocks (9 T)."""
+    N, P = scene.num_keys, scene.num_points
+    return float(N * (16 + 4 + 4 + 3 * E * tbytes) + P * (24 + 4 + 9 * tbytes))
+
+
+def load_traffic(name: str):
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def oracle_reference(name: str):
+    path = os.path.join(ROOT, "tests", "golden", f"{name}_fp64_oracle.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline_mid(scene: syn.Scene, iters: int = 1):
+    """The FP64 oracle (single-threaded C++ restatement of the reference path)
+    on the same scene: time of `iters` LM iterations, from two runs (0 and
+    `iters` iterations) so the initial linearization is not counted."""
+    sys.path.insert(0, os.path.join(ROOT))
+    from oracle.oracle import Oracle  # CPU baseline leg only
+    o = Oracle(scene)
+    t0 = time.perf_counter()
+    o.lm(max_iterations=0)
+    t1 = time.perf_counter()
+    o.lm(max_iterations=iters)
+    t2 = time.perf_counter()
+    per_iter = max(1e-9, (t2 - t1) - (t1 - t0)) / iters
+    return {"value": 1.0 / per_iter, "unit": "iters/s", "cores": 1, "kind": "port",
+            "sample": f"{iters} LM iteration(s) of the same {scene.num_keys}-key scene "
+                      f"(FP64, point-wise Schur + dense LDL^T, single thread; "
+                      f"initial linearization excluded), cpu={cpu_model()}, nproc={os.cpu_count()}"}
+
+
+def run_mid(args):
+    scene = syn.mid_scene()
+    prec = abi.PRECISION_FP64 if args.precision == "fp64" else abi.PRECISION_MIXED
+    E = 6 + scene.camera_params
+    tbytes = 8 if prec == abi.PRECISION_FP64 else 4
+    p = Problem(scene, precision=prec)
+    K, W = args.steps, args.warmup
+    for _ in range(W):
+        abi.check(p.lm_begin(max_iterations=K), "lm_begin")
+        abi.check(p.lm_iterate(K)[0], "lm_iterate")
+        p.lm_end()
+    # timed region: K iterations from x0, inputs resident, synchronised both sides
+    abi.check(p.lm_begin(max_iterations=K), "lm_begin")
+    t0 = time.perf_counter()
+    rc, _ = p.lm_iterate(K)
+    t1 = time.perf_counter()
+    abi.check(rc, "lm_iterate")
+    rc, x, st = p.lm_end()
+    dt = t1 - t0
+    rmse = syn.reprojection_rmse(scene, x)
+    # stage breakdown: the same K iterations replayed with HIP event nodes
+    # captured around each stage of the graph (library stream)
+    abi.check(p.lm_begin(max_iterations=K), "lm_begin")
+    rc, stages = p.lm_iterate(K, timed=True)
+    abi.check(rc, "lm_iterate(timed)")
+    p.lm_end()
+    flops = pair_flops(scene, E)
+    lin_bytes = linearize_bytes(scene, E, tbytes)
+    dominant = max(("pairs_ms", "cholesky_ms", "linearize_ms", "image_reduce_ms", "point_factor_ms",
+                    "assemble_ms", "backsub_ms", "step_cost_ms"), key=lambda k: stages[k])
+    traffic = load_traffic(f"k_pairs_mid_{args.precision}")
+    roof = {"kernel": "k_pairs (K4 camera-pair Schur products)", "bound": "fp32-valu" if tbytes == 4 else "fp64",
+            "achieved": flops / (stages["pairs_ms"] * 1e-3) / 1e12,
+            "peak": PEAK_FP32_TFLOPS if tbytes == 4 else PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "algorithmic_flops": flops}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["bound"] = "mfma" if False else roof["bound"]
+    ref = oracle_reference("mid")
+    out = {
+        "metric": METRIC, "value": K / dt, "unit": "iters/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": 1e3 * dt / K, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+fp64" if prec == abi.PRECISION_MIXED else "fp64", "data": "synthetic",
+        "config": {"workload": "mid: 200 cameras x 9 params (r, t, f, k1, k2), 50k points, "
+                               f"{scene.num_keys} keys, LM from x0",
+                   "cameras": scene.num_images, "points": scene.num_points, "keys": scene.num_keys,
+                   "precision": args.precision, "reduced_size": 9 * scene.num_images},
+        "roofline": roof,
+        "stage_ms": stages, "dominant_stage": dominant,
+        "linearize_gbs": lin_bytes / (stages["linearize_ms"] * 1e-3) / 1e9 if stages["linearize_ms"] > 0 else None,
+        "cholesky_fp64_tflops": (9 * scene.num_images) ** 3 / 3 / (stages["cholesky_ms"] * 1e-3) / 1e12,
+        "lm": {"status": st.status, "iterations": st.iterations, "accepted": st.accepted,
+               "initial_cost": st.initial_cost, "final_cost": st.final_cost, "rmse_px": rmse,
+               "rmse_x0_px": syn.reprojection_rmse(scene, scene.x0)},
+    }
+    if ref and ref.get("iterations") == K:
+        out["lm"]["fp64_oracle_rmse_px"] = ref["rmse_px"]
+        out["lm"]["rmse_delta_px"] = abs(rmse - ref["rmse_px"])
+        out["lm"]["final_cost_rel_err"] = abs(st.final_cost - ref["final_cost"]) / ref["final_cost"]
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_mid(scene, iters=args.cpu_iters)
+    p.close()
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="mid", choices=["mid"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=1)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only"}))
+        sys.exit(2)
+    out = run_mid(args)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,220 @@
+"""ctypes mirror of include/hsba.h.
+
+Builds `hsba_problem_desc` records from `synthetic.Scene` arrays and loads the
+HIP library `hs_sfm_amd/lib/libhsba.so`.  Loading fails loudly when the
+library has not been built: there is no CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional
+
+import numpy as np
+
+from .synthetic import Scene
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("HSBA_LIB", os.path.join(HERE, "lib", "libhsba.so"))
+
+HSBA_OK = 0
+ERRORS = {
+    -1: "HSBA_ERR_GENERIC", -2: "HSBA_ERR_INVALID", -3: "HSBA_ERR_NOT_SPD",
+    -4: "HSBA_ERR_NAN", -5: "HSBA_ERR_OOM", -6: "HSBA_ERR_DEVICE",
+    -7: "HSBA_ERR_ADMISSION", -8: "HSBA_ERR_UNSUPPORTED", -9: "HSBA_ERR_NOT_READY",
+}
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+SOLVER_AUTO, SOLVER_DENSE, SOLVER_PCG = 0, 1, 2
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class hsba_constraint(C.Structure):
+    _fields_ = [("id", C.c_int64), ("mask", C.c_uint32), ("reserved_", C.c_uint32)]
+
+
+class hsba_problem_desc(C.Structure):
+    _fields_ = [
+        ("num_points", C.c_int64), ("num_images", C.c_int64),
+        ("num_cameras", C.c_int64), ("num_keys", C.c_int64),
+        ("feature_maps", _ip), ("image_camera_map", _ip),
+        ("intrinsic_computations_mask", C.c_uint32), ("intrinsic_free_mask", C.c_uint32),
+        ("intrinsic_defaults", _dp),
+        ("fix_mask", C.c_uint32), ("reserved_", C.c_uint32),
+        ("fix_points", _dp), ("fix_images", _dp), ("fix_cameras", _dp),
+        ("num_point_constraints", C.c_int64), ("point_constraints", C.POINTER(hsba_constraint)),
+        ("num_image_constraints", C.c_int64), ("image_constraints", C.POINTER(hsba_constraint)),
+        ("num_camera_constraints", C.c_int64), ("camera_constraints", C.POINTER(hsba_constraint)),
+    ]
+
+
+class hsba_ycov_inverse(C.Structure):
+    _fields_ = [("key_blocks", _dp), ("constraint_weights", _dp)]
+
+
+class hsba_options(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("solver", C.c_int32), ("pcg_rtol", C.c_double),
+                ("pcg_max_iterations", C.c_int32), ("point_group_lanes", C.c_int32)]
+
+
+class hsba_lm_options(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("reserved_", C.c_int32), ("tau", C.c_double),
+                ("eps1", C.c_double), ("eps2", C.c_double)]
+
+
+class hsba_lm_stats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("stop_reason", C.c_int32), ("iterations", C.c_int32),
+                ("accepted", C.c_int32), ("initial_cost", C.c_double), ("final_cost", C.c_double),
+                ("final_mu", C.c_double), ("pcg_iterations", C.c_int64)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class hsba_stage_times(C.Structure):
+    _fields_ = [("linearize_ms", C.c_double), ("image_reduce_ms", C.c_double),
+                ("point_factor_ms", C.c_double), ("pairs_ms", C.c_double),
+                ("assemble_ms", C.c_double), ("cholesky_ms", C.c_double),
+                ("backsub_ms", C.c_double), ("step_cost_ms", C.c_double),
+                ("iteration_ms", C.c_double), ("iterations", C.c_int32), ("reserved_", C.c_int32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved_"}
+
+
+class hsba_server_options(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active_jobs", C.c_int32),
+                ("precision", C.c_int32), ("lm", hsba_lm_options)]
+
+
+class hsba_job_result(C.Structure):
+    _fields_ = [("job_id", C.c_int64), ("status", C.c_int32), ("reserved_", C.c_int32),
+                ("submit_time_s", C.c_double), ("admit_time_s", C.c_double),
+                ("finish_time_s", C.c_double), ("stats", hsba_lm_stats)]
+
+
+def dptr(a: Optional[np.ndarray]):
+    if a is None:
+        return C.cast(None, _dp)
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "float64 C-contiguous required"
+    return a.ctypes.data_as(_dp)
+
+
+class Desc:
+    """Owns the numpy buffers a `hsba_problem_desc` points into."""
+
+    def __init__(self, scene: Scene):
+        self.scene = scene
+        keep = []
+
+        def arr(a, dt):
+            if a is None:
+                return None
+            b = np.ascontiguousarray(a, dtype=dt)
+            keep.append(b)
+            return b
+
+        fm = arr(scene.feature_maps, np.int32)
+        icm = arr(scene.image_camera_map, np.int32)
+        d = hsba_problem_desc()
+        d.num_points, d.num_images = scene.num_points, scene.num_images
+        d.num_cameras, d.num_keys = scene.num_cameras, scene.num_keys
+        d.feature_maps = fm.ctypes.data_as(_ip)
+        d.image_camera_map = icm.ctypes.data_as(_ip)
+        d.intrinsic_computations_mask = scene.comp_mask
+        d.intrinsic_free_mask = scene.free_mask
+        d.intrinsic_defaults = dptr(arr(scene.intrinsic_defaults, np.float64))
+        d.fix_mask = scene.fix_mask
+        d.fix_points = dptr(arr(scene.fix_points, np.float64))
+        d.fix_images = dptr(arr(scene.fix_images, np.float64))
+        d.fix_cameras = dptr(arr(scene.fix_cameras, np.float64))
+
+        def cons(a):
+            if a is None or len(a) == 0:
+                return 0, C.cast(None, C.POINTER(hsba_constraint))
+            buf = (hsba_constraint * len(a))()
+            for j, (cid, m) in enumerate(a):
+                buf[j].id, buf[j].mask = int(cid), int(m)
+            keep.append(buf)
+            return len(a), buf
+
+        d.num_point_constraints, d.point_constraints = cons(scene.point_constraints)
+        d.num_image_constraints, d.image_constraints = cons(scene.image_constraints)
+        d.num_camera_constraints, d.camera_constraints = cons(scene.camera_constraints)
+        self.desc = d
+        self.kb = arr(scene.key_blocks.reshape(-1) if scene.key_blocks is not None else None,
+                      np.float64)
+        self.cw = arr(scene.constraint_weights, np.float64)
+        self.ycov = hsba_ycov_inverse(dptr(self.kb), dptr(self.cw))
+        self._keep = keep
+
+
+_LIB = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libhsba.so; raises if it is missing (no fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"libhsba.so not built at {path}: run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    lib.hsba_version.restype = C.c_char_p
+    lib.hsba_device_count.argtypes = [C.POINTER(C.c_int)]
+    lib.hsba_problem_create.argtypes = [C.POINTER(hsba_problem_desc), C.POINTER(hsba_options),
+                                        C.POINTER(P)]
+    lib.hsba_problem_destroy.argtypes = [P]
+    lib.hsba_problem_sizes.argtypes = [P] + [C.POINTER(C.c_int64)] * 4
+    lib.hsba_evaluate.argtypes = [P, _dp, _dp]
+    lib.hsba_jacobian.argtypes = [P, _dp, _dp, _dp, _dp]
+    lib.hsba_build_normal.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
+    lib.hsba_solve.argtypes = [P, C.c_double, _dp]
+    lib.hsba_reduced_system.argtypes = [P, C.c_double, _dp, _dp]
+    lib.hsba_max_diagonal.argtypes = [P, _dp]
+    lib.hsba_gradient_inf_norm.argtypes = [P, _dp]
+    lib.hsba_gradient_dot.argtypes = [P, _dp, _dp]
+    lib.hsba_mahalanobis.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp]
+    lib.hsba_lm_solve.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse),
+                                  C.POINTER(hsba_lm_options), _dp, C.POINTER(hsba_lm_stats)]
+    lib.hsba_lm_begin.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse),
+                                  C.POINTER(hsba_lm_options)]
+    lib.hsba_lm_iterate.argtypes = [P, C.c_int32, C.POINTER(hsba_stage_times)]
+    lib.hsba_lm_end.argtypes = [P, _dp, C.POINTER(hsba_lm_stats)]
+    lib.hsba_batch_create.argtypes = [C.POINTER(hsba_problem_desc), C.c_int64,
+                                      C.POINTER(hsba_options), C.POINTER(P)]
+    lib.hsba_batch_destroy.argtypes = [P]
+    lib.hsba_batch_lm_solve.argtypes = [P, C.POINTER(_dp), C.POINTER(_dp),
+                                        C.POINTER(hsba_ycov_inverse), C.POINTER(hsba_lm_options),
+                                        C.POINTER(_dp), C.POINTER(hsba_lm_stats)]
+    lib.hsba_server_create.argtypes = [C.POINTER(hsba_server_options), C.POINTER(P)]
+    lib.hsba_server_destroy.argtypes = [P]
+    lib.hsba_server_submit.argtypes = [P, C.POINTER(hsba_problem_desc), _dp, _dp,
+                                       C.POINTER(hsba_ycov_inverse), _dp, C.POINTER(C.c_int64)]
+    lib.hsba_server_poll.argtypes = [P, C.POINTER(hsba_job_result), C.c_int32,
+                                     C.POINTER(C.c_int32)]
+    lib.hsba_server_drain.argtypes = [P]
+    lib.hsba_server_clock.argtypes = [P]
+    lib.hsba_server_clock.restype = C.c_double
+    _LIB = lib
+    return lib
+
+
+EXPORTED_SYMBOLS = [
+    "hsba_version", "hsba_device_count", "hsba_problem_create", "hsba_problem_destroy",
+    "hsba_problem_sizes", "hsba_evaluate", "hsba_jacobian", "hsba_build_normal",
+    "hsba_solve", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
+    "hsba_gradient_dot", "hsba_mahalanobis", "hsba_lm_solve", "hsba_lm_begin",
+    "hsba_lm_iterate", "hsba_lm_end", "hsba_batch_create",
+    "hsba_batch_destroy", "hsba_batch_lm_solve", "hsba_server_create",
+    "hsba_server_destroy", "hsba_server_submit", "hsba_server_poll", "hsba_server_drain",
+    "hsba_server_clock",
+]
+
+
+def check(rc: int, what: str = ""):
+    if rc != HSBA_OK:
+        raise RuntimeError(f"{what} failed: {rc} ({ERRORS.get(rc, '?')})")

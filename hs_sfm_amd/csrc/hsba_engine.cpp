@@ -1,0 +1,664 @@
+// hsba_engine.cpp — device-resident state of a problem / batch and the LM
+// driver.  One LM iteration is a fixed sequence of launches whose work is
+// gated by per-window flags on the device (WinState), so the sequence is
+// captured once into a hipGraph and replayed max_iterations times with no
+// host round trip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "hsba_host.h"
+
+namespace hsba {
+
+namespace {
+
+#define HCHK(expr)                                                                     \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      std::fprintf(stderr, "hsba: %s failed: %s\n", #expr, hipGetErrorString(e_));    \
+      return e_ == hipErrorOutOfMemory ? HSBA_ERR_OOM : HSBA_ERR_DEVICE;               \
+    }                                                                                  \
+  } while (0)
+
+template <typename X>
+int dalloc(X *&p, int64_t n, std::vector<void *> &owned) {
+  p = nullptr;
+  if (n <= 0) n = 1;
+  HCHK(hipMalloc((void **)&p, sizeof(X) * (size_t)n));
+  owned.push_back((void *)p);
+  return HSBA_OK;
+}
+
+template <typename X>
+int upload(X *dst, const std::vector<X> &src, hipStream_t st) {
+  if (src.empty()) return HSBA_OK;
+  HCHK(hipMemcpyAsync(dst, src.data(), sizeof(X) * src.size(), hipMemcpyHostToDevice, st));
+  return HSBA_OK;
+}
+
+}  // namespace
+
+template <typename T>
+struct Engine final : EngineBase {
+  KernelSet<T> *ks = nullptr;
+  Dev<T> d;
+  std::vector<void *> owned;
+  hipStream_t st = nullptr;
+  bool own_stream = false;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int *small_list_dev = nullptr;
+  CholPlan plan;
+  std::vector<WinState> ws_host;
+  bool built = false;  // a linearization is resident
+
+  ~Engine() override {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    drop_stage_graphs();
+    for (int i = 0; i <= kStages; ++i)
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+    if (st) (void)hipStreamSynchronize(st);
+    for (void *p : owned) (void)hipFree(p);
+    if (own_stream && st) (void)hipStreamDestroy(st);
+    delete ks;
+  }
+
+  hipStream_t stream() const override { return st; }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    drop_stage_graphs();
+  }
+  void drop_stage_graphs() {
+    for (int i = 0; i < kStages; ++i) {
+      if (sgexec[i]) (void)hipGraphExecDestroy(sgexec[i]);
+      if (sgraph[i]) (void)hipGraphDestroy(sgraph[i]);
+      sgexec[i] = nullptr;
+      sgraph[i] = nullptr;
+    }
+  }
+
+  int init(hipStream_t stream_in) override {
+    ks = make_kernel_set<T>(L.IMG, L.D);
+    if (!ks) return HSBA_ERR_UNSUPPORTED;
+    if (stream_in) st = stream_in;
+    else { HCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); own_stream = true; }
+    std::memset(&d, 0, sizeof(d));
+    const int E = L.E, D = L.D;
+    d.P = (int32_t)L.P; d.I = (int32_t)L.I; d.C = (int32_t)L.C; d.N = (int32_t)L.N;
+    d.nwin = L.nwin; d.npairs = (int32_t)L.npairs; d.ncons = (int32_t)L.cons.size();
+    d.nchunks = (int32_t)L.chunk_kind.size();
+    d.pts_free = L.pts_free; d.imgs_free = L.imgs_free; d.cams_free = L.cams_free; d.has_cov = 0;
+    for (int j = 0; j < kFull; ++j) d.fidx[j] = L.fidx[j];
+    int rc = 0;
+#define A(ptr, n) if ((rc = dalloc(ptr, (n), owned))) return rc
+    int32_t *key_img, *key_pt, *key_orig, *pt_ptr, *pt_win, *img_ptr, *img_keys, *img_win, *img_cam, *img_row;
+    int32_t *cam_ptr, *cam_imgs, *cam_win, *cam_row, *pair_win, *pair_img, *row_blk, *row_sub;
+    int32_t *ck, *cbg, *cen, *cw;
+    int64_t *pair_ptr, *blk_ptr;
+    uint32_t *inc;
+    Contrib *contrib;
+    ConstraintRow *cons;
+    WinInfo *wi;
+    A(key_img, L.N); A(key_pt, L.N); A(key_orig, L.N);
+    A(pt_ptr, L.P + 1); A(pt_win, L.P);
+    A(img_ptr, L.I + 1); A(img_keys, L.N); A(img_win, L.I); A(img_cam, L.I); A(img_row, L.I);
+    A(cam_ptr, L.C + 1); A(cam_imgs, L.I); A(cam_win, L.C); A(cam_row, L.C);
+    A(pair_ptr, L.npairs + 1); A(inc, L.incidences); A(pair_win, L.npairs); A(pair_img, L.npairs);
+    A(blk_ptr, (int64_t)L.blk_ptr.size()); A(contrib, (int64_t)L.contrib.size());
+    A(row_blk, (int64_t)L.row_blk.size()); A(row_sub, (int64_t)L.row_sub.size());
+    A(cons, (int64_t)L.cons.size());
+    A(ck, d.nchunks); A(cbg, d.nchunks); A(cen, d.nchunks); A(cw, d.nchunks);
+    A(wi, L.nwin);
+    A(d.ws, L.nwin);
+    double *key_uv; A(key_uv, 2 * L.N);
+    A(d.pts, 3 * L.P); A(d.pts_new, 3 * L.P); A(d.dpt, 3 * L.P);
+    A(d.V, 6 * L.P); A(d.gp, 3 * L.P); A(d.Linv, 6 * L.P); A(d.q, 3 * L.P); A(d.tpt, 3 * L.P);
+    A(d.imgs, 6 * L.I); A(d.imgs_new, 6 * L.I);
+    A(d.Dimg, (int64_t)E * E * L.I); A(d.gr, (int64_t)E * L.I); A(d.rp, (int64_t)E * L.I);
+    A(d.cam_free, (int64_t)std::max(D, 1) * L.C); A(d.cam_free_new, (int64_t)std::max(D, 1) * L.C);
+    A(d.cam_full, kFull * L.C); A(d.cam_full_new, kFull * L.C);
+    A(d.Uc, (int64_t)D * D * L.C); A(d.gc, (int64_t)D * L.C);
+    A(d.W, 3LL * E * L.N); A(d.Z, 3LL * E * L.N);
+    A(d.pair_out, (int64_t)E * E * L.npairs);
+    A(d.S, L.S_total); A(d.rhs, L.rows_total); A(d.dred, L.rows_total); A(d.Dinv, L.dinv_total);
+    A(d.partial, 4LL * d.nchunks);
+    A(d.pv, L.rows_total); A(d.pr, L.rows_total); A(d.pz, L.rows_total); A(d.pp, L.rows_total);
+    A(d.pAp, L.rows_total); A(d.Minv, (int64_t)E * E * L.I);
+    A(small_list_dev, (int64_t)L.small_list.size());
+#undef A
+    if ((rc = upload(key_img, L.key_img, st)) || (rc = upload(key_pt, L.key_pt, st)) ||
+        (rc = upload(key_orig, L.key_orig, st)) || (rc = upload(pt_ptr, L.pt_ptr, st)) ||
+        (rc = upload(pt_win, L.pt_win, st)) || (rc = upload(img_ptr, L.img_ptr, st)) ||
+        (rc = upload(img_keys, L.img_keys, st)) || (rc = upload(img_win, L.img_win, st)) ||
+        (rc = upload(img_cam, L.img_cam, st)) || (rc = upload(img_row, L.img_row, st)) ||
+        (rc = upload(cam_ptr, L.cam_ptr, st)) || (rc = upload(cam_imgs, L.cam_imgs, st)) ||
+        (rc = upload(cam_win, L.cam_win, st)) || (rc = upload(cam_row, L.cam_row, st)) ||
+        (rc = upload(pair_ptr, L.pair_ptr, st)) || (rc = upload(inc, L.inc, st)) ||
+        (rc = upload(pair_win, L.pair_win, st)) || (rc = upload(pair_img, L.pair_img, st)) ||
+        (rc = upload(blk_ptr, L.blk_ptr, st)) || (rc = upload(contrib, L.contrib, st)) ||
+        (rc = upload(row_blk, L.row_blk, st)) || (rc = upload(row_sub, L.row_sub, st)) ||
+        (rc = upload(cons, L.cons, st)) || (rc = upload(ck, L.chunk_kind, st)) ||
+        (rc = upload(cbg, L.chunk_begin, st)) || (rc = upload(cen, L.chunk_end, st)) ||
+        (rc = upload(cw, L.chunk_win, st)) || (rc = upload(wi, L.wi, st)) ||
+        (rc = upload(small_list_dev, L.small_list, st)))
+      return rc;
+    d.key_img = key_img; d.key_pt = key_pt; d.key_orig = key_orig; d.key_uv = key_uv; d.key_cov = nullptr;
+    d.pt_ptr = pt_ptr; d.pt_win = pt_win;
+    d.img_ptr = img_ptr; d.img_keys = img_keys; d.img_win = img_win; d.img_cam = img_cam; d.img_row = img_row;
+    d.cam_ptr = cam_ptr; d.cam_imgs = cam_imgs; d.cam_win = cam_win; d.cam_row = cam_row;
+    d.pair_ptr = pair_ptr; d.inc = inc; d.pair_win = pair_win; d.pair_img = pair_img;
+    d.wi = wi; d.blk_ptr = blk_ptr; d.contrib = contrib; d.row_blk = row_blk; d.row_sub = row_sub;
+    d.cons = cons; d.chunk_kind = ck; d.chunk_begin = cbg; d.chunk_end = cen; d.chunk_win = cw;
+    d.win_of_chunk = cw;
+    // fixed blocks and camera base vectors
+    if (!L.pts_free && (rc = upload(d.pts, L.fix_pts, st))) return rc;
+    if (!L.pts_free && (rc = upload(d.pts_new, L.fix_pts, st))) return rc;
+    if (!L.imgs_free && (rc = upload(d.imgs, L.fix_imgs, st))) return rc;
+    if (!L.imgs_free && (rc = upload(d.imgs_new, L.fix_imgs, st))) return rc;
+    if (!L.cams_free) {
+      std::vector<double> full = L.cam_base;
+      const int dd = __builtin_popcount(L.free_bits);
+      for (int64_t c = 0; c < L.C; ++c)
+        for (int q = 0; q < dd; ++q) full[c * kFull + L.fidx[q]] = L.fix_cam_free[c * dd + q];
+      if ((rc = upload(d.cam_full, full, st)) || (rc = upload(d.cam_full_new, full, st))) return rc;
+    }
+    plan.nwin = L.nwin; plan.wi = wi; plan.ws = d.ws; plan.S = d.S; plan.rhs = d.rhs; plan.dred = d.dred;
+    plan.Dinv = d.Dinv; plan.wi_host = L.wi.data();
+    HCHK(hipMemsetAsync(d.S, 0, sizeof(double) * (size_t)std::max<int64_t>(1, L.S_total), st));
+    chol_init_padding(plan, st);
+    HCHK(hipGetLastError());
+    ws_host.assign(L.nwin, WinState());
+    HCHK(hipStreamSynchronize(st));
+    return HSBA_OK;
+  }
+
+  // ---- host <-> device parameter transfer in the reference x layout
+  int upload_x(const double *const *xs) override {
+    const int D = L.D;
+    std::vector<double> pts, imgs, camf, full;
+    if (L.pts_free) pts.resize(3 * L.P);
+    if (L.imgs_free) imgs.resize(6 * L.I);
+    if (L.cams_free) { camf.resize((int64_t)std::max(D, 1) * L.C); full = L.cam_base; }
+    for (int w = 0; w < L.nwin; ++w) {
+      const HostWindow &h = L.hw[w];
+      const WinInfo &wi = L.wi[w];
+      const double *x = xs[w];
+      if (!x && h.x_size) return HSBA_ERR_INVALID;
+      if (L.pts_free) std::copy(x + h.x_pt, x + h.x_pt + 3 * h.P, pts.begin() + 3 * (int64_t)wi.pt_off);
+      if (L.imgs_free) std::copy(x + h.x_img, x + h.x_img + 6 * h.I, imgs.begin() + 6 * (int64_t)wi.img_off);
+      if (L.cams_free)
+        for (int64_t c = 0; c < h.C; ++c)
+          for (int q = 0; q < D; ++q) {
+            const double v = x[h.x_cam + D * c + q];
+            camf[(wi.cam_off + c) * D + q] = v;
+            full[(wi.cam_off + c) * kFull + L.fidx[q]] = v;
+          }
+    }
+    int rc = 0;
+    if (L.pts_free && ((rc = upload(d.pts, pts, st)) || (rc = upload(d.pts_new, pts, st)))) return rc;
+    if (L.imgs_free && ((rc = upload(d.imgs, imgs, st)) || (rc = upload(d.imgs_new, imgs, st)))) return rc;
+    if (L.cams_free && ((rc = upload(d.cam_free, camf, st)) || (rc = upload(d.cam_free_new, camf, st)) ||
+                        (rc = upload(d.cam_full, full, st)) || (rc = upload(d.cam_full_new, full, st))))
+      return rc;
+    built = false;
+    return HSBA_OK;
+  }
+
+  int download_x(double *const *xs) override {
+    const int D = L.D;
+    std::vector<double> pts(L.pts_free ? 3 * L.P : 0), imgs(L.imgs_free ? 6 * L.I : 0),
+        camf(L.cams_free ? (int64_t)D * L.C : 0);
+    if (!pts.empty()) HCHK(hipMemcpyAsync(pts.data(), d.pts, 8 * pts.size(), hipMemcpyDeviceToHost, st));
+    if (!imgs.empty()) HCHK(hipMemcpyAsync(imgs.data(), d.imgs, 8 * imgs.size(), hipMemcpyDeviceToHost, st));
+    if (!camf.empty()) HCHK(hipMemcpyAsync(camf.data(), d.cam_free, 8 * camf.size(), hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    for (int w = 0; w < L.nwin; ++w) {
+      const HostWindow &h = L.hw[w];
+      const WinInfo &wi = L.wi[w];
+      double *x = xs[w];
+      if (!x) continue;
+      if (L.pts_free) std::copy(pts.begin() + 3 * (int64_t)wi.pt_off, pts.begin() + 3 * ((int64_t)wi.pt_off + h.P), x + h.x_pt);
+      if (L.imgs_free) std::copy(imgs.begin() + 6 * (int64_t)wi.img_off, imgs.begin() + 6 * ((int64_t)wi.img_off + h.I), x + h.x_img);
+      if (L.cams_free) std::copy(camf.begin() + (int64_t)D * wi.cam_off, camf.begin() + (int64_t)D * (wi.cam_off + h.C), x + h.x_cam);
+    }
+    return HSBA_OK;
+  }
+
+  int upload_obs(const double *const *ys, const hsba_ycov_inverse *ycovs) override {
+    std::vector<double> uv(2 * L.N);
+    bool any_cov = false;
+    for (int w = 0; w < L.nwin; ++w)
+      if (ycovs && ycovs[w].key_blocks) any_cov = true;
+    std::vector<double> cov(any_cov ? 3 * L.N : 0);
+    for (int w = 0; w < L.nwin; ++w) {
+      const WinInfo &wi = L.wi[w];
+      const HostWindow &h = L.hw[w];
+      const double *y = ys[w];
+      if (!y && h.N) return HSBA_ERR_INVALID;
+      const double *kb = (ycovs && ycovs[w].key_blocks) ? ycovs[w].key_blocks : nullptr;
+      for (int64_t k = wi.key_off; k < wi.key_off + h.N; ++k) {
+        const int64_t o = L.key_orig[k] - wi.key_off;
+        uv[2 * k] = y[2 * o];
+        uv[2 * k + 1] = y[2 * o + 1];
+        if (any_cov) {
+          double l00 = 1.0, l10 = 0.0, l11 = 1.0;
+          if (kb) {
+            // Lambda = L L^T; whitening uses L^T = [[l00, l10], [0, l11]]
+            const double a = kb[4 * o], b = 0.5 * (kb[4 * o + 1] + kb[4 * o + 2]), c = kb[4 * o + 3];
+            if (!(a > 0.0)) return HSBA_ERR_INVALID;
+            l00 = std::sqrt(a);
+            l10 = b / l00;
+            const double t = c - l10 * l10;
+            if (!(t > 0.0)) return HSBA_ERR_INVALID;
+            l11 = std::sqrt(t);
+          }
+          cov[3 * k] = l00; cov[3 * k + 1] = l10; cov[3 * k + 2] = l11;
+        }
+      }
+      for (int q = wi.con_begin; q < wi.con_end; ++q) {
+        const int local = q - wi.con_begin;
+        L.cons[q].y = y[2 * h.N + local];
+        L.cons[q].weight = (ycovs && ycovs[w].constraint_weights) ? ycovs[w].constraint_weights[local] : 1.0;
+      }
+    }
+    int rc = upload((double *)d.key_uv, uv, st);
+    if (rc) return rc;
+    if (any_cov) {
+      if (!d.key_cov) {
+        double *p;
+        if ((rc = dalloc(p, 3 * L.N, owned))) return rc;
+        d.key_cov = p;
+        drop_graph();
+      }
+      if ((rc = upload((double *)d.key_cov, cov, st))) return rc;
+    }
+    if (d.has_cov != (any_cov ? 1 : 0) && gexec) {
+      drop_graph();
+    }
+    d.has_cov = any_cov ? 1 : 0;
+    if (!L.cons.empty() && (rc = upload((ConstraintRow *)d.cons, L.cons, st))) return rc;
+    built = false;
+    return HSBA_OK;
+  }
+
+  // ---- window state helpers
+  int set_state(const hsba_lm_options *lm, double mu, int active, int need_lin, int trial) {
+    for (int w = 0; w < L.nwin; ++w) {
+      WinState &s = ws_host[w];
+      std::memset(&s, 0, sizeof(s));
+      s.mu = mu; s.nu = 2.0;
+      s.tau = lm ? lm->tau : 0.0; s.eps1 = lm ? lm->eps1 : 0.0; s.eps2 = lm ? lm->eps2 : 0.0;
+      s.max_iter = lm ? lm->max_iterations : 0;
+      s.active = active; s.need_lin = need_lin; s.trial = trial;
+    }
+    HCHK(hipMemcpyAsync(d.ws, ws_host.data(), sizeof(WinState) * L.nwin, hipMemcpyHostToDevice, st));
+    return HSBA_OK;
+  }
+  int get_state() {
+    HCHK(hipMemcpyAsync(ws_host.data(), d.ws, sizeof(WinState) * L.nwin, hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    return HSBA_OK;
+  }
+
+  void enqueue_linearize() {
+    ks->linearize(d, st, glog2);
+    ks->image_reduce(d, st);
+    ks->camera_reduce(d, st);
+    ks->constraints(d, st);
+  }
+  void enqueue_trial() {
+    ks->point_factor(d, st);
+    ks->key_z(d, st);
+    ks->pairs(d, st);
+    ks->assemble(d, st, L.lower_total, (int)L.rows_total);
+    chol_factor_solve_impl(plan, small_list_dev, (int)L.small_list.size(), L.large_list.data(),
+                           (int)L.large_list.size(), st);
+    ks->backsub(d, st, glog2);
+  }
+  void enqueue_iteration() {
+    ks->lm_control(d, st, 1);
+    enqueue_trial();
+    ks->step(d, st);
+    ks->lm_control(d, st, 2);
+    ks->cost(d, st, 1);
+    ks->lm_control(d, st, 3);
+    ks->commit(d, st);
+    enqueue_linearize();
+    ks->grad_stats(d, st);
+    ks->lm_control(d, st, 4);
+  }
+
+  int lm(const hsba_lm_options *lm, hsba_lm_stats *stats) override {
+    int rc = lm_begin(lm);
+    if (rc == HSBA_OK && lm->max_iterations > 0) rc = lm_iterate(lm->max_iterations, nullptr);
+    const int rc2 = lm_end(stats);
+    return rc ? rc : rc2;
+  }
+
+  bool lm_failed_early = false;
+
+  int lm_begin(const hsba_lm_options *lm) override {
+    lm_failed_early = false;
+    for (int w = 0; w < L.nwin; ++w)
+      if (L.hw[w].unobserved) {
+        // the reference's reduced system misses that block: SimplicialLDLT fails
+        lm_failed_early = true;
+        return HSBA_ERR_GENERIC;
+      }
+    int rc = set_state(lm, 0.0, 1, 1, 0);
+    if (rc) return rc;
+    // initial cost, linearization, mu0 and the gradient test
+    ks->cost(d, st, 0);
+    enqueue_linearize();
+    ks->grad_stats(d, st);
+    ks->lm_control(d, st, 0);
+    HCHK(hipGetLastError());
+    HCHK(hipStreamSynchronize(st));
+    return HSBA_OK;
+  }
+
+  // Timed mode: each stage of an iteration is its own captured graph and the
+  // HIP events are recorded on the stream between stage launches (timing
+  // queries on events recorded inside a captured graph are not supported).
+  static constexpr int kStages = 8;
+  hipEvent_t ev[kStages + 1] = {};
+  hipGraphExec_t sgexec[kStages] = {};
+  hipGraph_t sgraph[kStages] = {};
+
+  void enqueue_stage(int i) {
+    switch (i) {
+      case 0:  // K3 point factor + Z
+        ks->lm_control(d, st, 1);
+        ks->point_factor(d, st);
+        ks->key_z(d, st);
+        break;
+      case 1: ks->pairs(d, st); break;  // K4
+      case 2: ks->assemble(d, st, L.lower_total, (int)L.rows_total); break;
+      case 3:  // K5
+        chol_factor_solve_impl(plan, small_list_dev, (int)L.small_list.size(), L.large_list.data(),
+                               (int)L.large_list.size(), st);
+        break;
+      case 4: ks->backsub(d, st, glog2); break;  // K6
+      case 5:  // K7 + K8
+        ks->step(d, st);
+        ks->lm_control(d, st, 2);
+        ks->cost(d, st, 1);
+        ks->lm_control(d, st, 3);
+        ks->commit(d, st);
+        break;
+      case 6: ks->linearize(d, st, glog2); break;  // K1 + K2
+      default:
+        ks->image_reduce(d, st);
+        ks->camera_reduce(d, st);
+        ks->constraints(d, st);
+        ks->grad_stats(d, st);
+        ks->lm_control(d, st, 4);
+        break;
+    }
+  }
+
+  int lm_iterate(int iterations, hsba_stage_times *times) override {
+    if (lm_failed_early) return HSBA_ERR_GENERIC;
+    if (iterations <= 0) return HSBA_OK;
+    if (!times) {
+      if (!gexec) {
+        HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue_iteration();
+        hipError_t e1 = hipStreamEndCapture(st, &graph);
+        if (e1 != hipSuccess) {
+          std::fprintf(stderr, "hsba: capture failed: %s\n", hipGetErrorString(e1));
+          return HSBA_ERR_DEVICE;
+        }
+        HCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+      }
+      for (int it = 0; it < iterations; ++it) HCHK(hipGraphLaunch(gexec, st));
+      HCHK(hipStreamSynchronize(st));
+      return HSBA_OK;
+    }
+    if (!sgexec[0]) {
+      for (int i = 0; i <= kStages; ++i) HCHK(hipEventCreate(&ev[i]));
+      for (int i = 0; i < kStages; ++i) {
+        HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        enqueue_stage(i);
+        hipError_t e1 = hipStreamEndCapture(st, &sgraph[i]);
+        if (e1 != hipSuccess) {
+          std::fprintf(stderr, "hsba: capture failed: %s\n", hipGetErrorString(e1));
+          return HSBA_ERR_DEVICE;
+        }
+        HCHK(hipGraphInstantiate(&sgexec[i], sgraph[i], nullptr, nullptr, 0));
+      }
+    }
+    double acc[kStages + 1] = {0};
+    for (int it = 0; it < iterations; ++it) {
+      HCHK(hipEventRecord(ev[0], st));
+      for (int i = 0; i < kStages; ++i) {
+        HCHK(hipGraphLaunch(sgexec[i], st));
+        HCHK(hipEventRecord(ev[i + 1], st));
+      }
+      HCHK(hipEventSynchronize(ev[kStages]));
+      for (int i = 0; i < kStages; ++i) {
+        float ms = 0.f;
+        HCHK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+        acc[i] += ms;
+      }
+      float tot = 0.f;
+      HCHK(hipEventElapsedTime(&tot, ev[0], ev[kStages]));
+      acc[kStages] += tot;
+    }
+    const double inv = 1.0 / iterations;
+    std::memset(times, 0, sizeof(*times));
+    times->point_factor_ms = acc[0] * inv;
+    times->pairs_ms = acc[1] * inv;
+    times->assemble_ms = acc[2] * inv;
+    times->cholesky_ms = acc[3] * inv;
+    times->backsub_ms = acc[4] * inv;
+    times->step_cost_ms = acc[5] * inv;
+    times->linearize_ms = acc[6] * inv;
+    times->image_reduce_ms = acc[7] * inv;
+    times->iteration_ms = acc[kStages] * inv;
+    times->iterations = iterations;
+    return HSBA_OK;
+  }
+
+  int lm_end(hsba_lm_stats *stats) override {
+    if (lm_failed_early) {
+      for (int q = 0; q < L.nwin; ++q) {
+        std::memset(&stats[q], 0, sizeof(hsba_lm_stats));
+        stats[q].status = L.hw[q].unobserved ? HSBA_ERR_GENERIC : HSBA_ERR_UNSUPPORTED;
+        stats[q].stop_reason = HSBA_STOP_FAILED;
+      }
+      return HSBA_ERR_GENERIC;
+    }
+    int rc = get_state();
+    if (rc) return rc;
+    built = true;
+    int worst = HSBA_OK;
+    for (int w = 0; w < L.nwin; ++w) {
+      const WinState &s = ws_host[w];
+      hsba_lm_stats &o = stats[w];
+      std::memset(&o, 0, sizeof(o));
+      o.status = s.status;
+      o.stop_reason = s.status ? HSBA_STOP_FAILED : s.stop_reason;
+      o.iterations = s.iter;
+      o.accepted = s.accepted;
+      o.initial_cost = s.cost0;
+      o.final_cost = s.cost;
+      o.final_mu = s.mu;
+      o.pcg_iterations = s.pcg_iters;
+      if (s.status) worst = s.status;
+    }
+    return worst;
+  }
+
+  int build_normal(double *V, double *Ui, double *Uc, double *g) override {
+    int rc = set_state(nullptr, 0.0, 1, 1, 0);
+    if (rc) return rc;
+    enqueue_linearize();
+    HCHK(hipGetLastError());
+    built = true;
+    return download_normal(V, Ui, Uc, g);
+  }
+
+  int download_normal(double *V, double *Ui, double *Uc, double *g) override {
+    if (!built) return HSBA_ERR_GENERIC;
+    const int E = L.E, D = L.D, IMG = L.IMG;
+    std::vector<T> hV(6 * L.P), hg(3 * L.P), hD((int64_t)E * E * L.I), hgr((int64_t)E * L.I),
+        hU((int64_t)D * D * L.C), hgc((int64_t)D * L.C);
+    if (L.pts_free && L.P) {
+      HCHK(hipMemcpyAsync(hV.data(), d.V, sizeof(T) * hV.size(), hipMemcpyDeviceToHost, st));
+      HCHK(hipMemcpyAsync(hg.data(), d.gp, sizeof(T) * hg.size(), hipMemcpyDeviceToHost, st));
+    }
+    if (E && L.I) {
+      HCHK(hipMemcpyAsync(hD.data(), d.Dimg, sizeof(T) * hD.size(), hipMemcpyDeviceToHost, st));
+      HCHK(hipMemcpyAsync(hgr.data(), d.gr, sizeof(T) * hgr.size(), hipMemcpyDeviceToHost, st));
+    }
+    if (D && L.C) {
+      HCHK(hipMemcpyAsync(hU.data(), d.Uc, sizeof(T) * hU.size(), hipMemcpyDeviceToHost, st));
+      HCHK(hipMemcpyAsync(hgc.data(), d.gc, sizeof(T) * hgc.size(), hipMemcpyDeviceToHost, st));
+    }
+    HCHK(hipStreamSynchronize(st));
+    const HostWindow &h = L.hw[0];
+    static const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    if (V && L.pts_free)
+      for (int64_t p = 0; p < L.P; ++p)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) V[9 * p + 3 * a + b] = (double)hV[6 * p + sym[a][b]];
+    if (Ui && IMG)
+      for (int64_t i = 0; i < L.I; ++i)
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < 6; ++b) Ui[36 * i + 6 * a + b] = (double)hD[(int64_t)i * E * E + a * E + b];
+    if (Uc && D)
+      for (int64_t j = 0; j < (int64_t)D * D * L.C; ++j) Uc[j] = (double)hU[j];
+    if (g) {
+      if (L.pts_free) for (int64_t j = 0; j < 3 * L.P; ++j) g[h.x_pt + j] = (double)hg[j];
+      if (IMG)
+        for (int64_t i = 0; i < L.I; ++i)
+          for (int a = 0; a < 6; ++a) g[h.x_img + 6 * i + a] = (double)hgr[(int64_t)i * E + a];
+      if (D)
+        for (int64_t j = 0; j < (int64_t)D * L.C; ++j) g[h.x_cam + j] = (double)hgc[j];
+    }
+    return HSBA_OK;
+  }
+
+  // One damped Newton step on the resident normal equations (window 0):
+  // CameraSharedNormalEquationSolver::operator().  S / rhs, when given,
+  // receive the reduced system instead of (or besides) the step.
+  int solve(double mu, double *delta, double *S, double *rhs) override {
+    if (!built) return HSBA_ERR_GENERIC;
+    if (L.hw[0].unobserved) return HSBA_ERR_GENERIC;
+    int rc = set_state(nullptr, mu, 1, 0, 1);
+    if (rc) return rc;
+    ks->point_factor(d, st);
+    ks->key_z(d, st);
+    ks->pairs(d, st);
+    ks->assemble(d, st, L.lower_total, (int)L.rows_total);
+    const WinInfo &wi = L.wi[0];
+    if (S || rhs) {
+      HCHK(hipStreamSynchronize(st));
+      std::vector<double> hS((size_t)wi.ld * wi.ld), hr(wi.ld);
+      if (wi.n) {
+        HCHK(hipMemcpy(hS.data(), d.S + wi.s_off, 8 * hS.size(), hipMemcpyDeviceToHost));
+        HCHK(hipMemcpy(hr.data(), d.rhs + wi.r_off, 8 * (size_t)wi.n, hipMemcpyDeviceToHost));
+      }
+      for (int i = 0; i < wi.n; ++i)
+        for (int j = 0; j <= i; ++j) {
+          if (S) { S[(int64_t)i * wi.n + j] = hS[(int64_t)i * wi.ld + j]; S[(int64_t)j * wi.n + i] = hS[(int64_t)i * wi.ld + j]; }
+        }
+      if (rhs) for (int i = 0; i < wi.n; ++i) rhs[i] = hr[i];
+      if (!delta) return HSBA_OK;
+    }
+    chol_factor_solve_impl(plan, small_list_dev, (int)L.small_list.size(), L.large_list.data(),
+                           (int)L.large_list.size(), st);
+    ks->backsub(d, st, glog2);
+    HCHK(hipGetLastError());
+    if ((rc = get_state())) return rc;
+    if (ws_host[0].chol_fail) return HSBA_ERR_GENERIC;
+    const HostWindow &h = L.hw[0];
+    std::vector<double> dp(L.pts_free ? 3 * L.P : 0), dr(std::max<int64_t>(1, L.rows_total));
+    if (!dp.empty()) HCHK(hipMemcpy(dp.data(), d.dpt, 8 * dp.size(), hipMemcpyDeviceToHost));
+    HCHK(hipMemcpy(dr.data(), d.dred, 8 * dr.size(), hipMemcpyDeviceToHost));
+    if (L.pts_free) std::copy(dp.begin(), dp.end(), delta + h.x_pt);
+    const int64_t r0 = L.pts_free ? 3 * L.P : 0;
+    for (int j = 0; j < wi.n; ++j) delta[r0 + j] = dr[wi.r_off + j];
+    return HSBA_OK;
+  }
+
+  int evaluate(double *y) override {
+    double *dy;
+    HCHK(hipMalloc((void **)&dy, 16 * (size_t)std::max<int64_t>(1, L.N)));
+    ks->export_keys(d, st, dy, nullptr, nullptr, nullptr, 0);
+    HCHK(hipGetLastError());
+    HCHK(hipMemcpyAsync(y, dy, 16 * (size_t)L.N, hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    HCHK(hipFree(dy));
+    return HSBA_OK;
+  }
+
+  int jacobian(double *jp, double *ji, double *jc) override {
+    const int D = std::max(L.D, 1);
+    double *dp = nullptr, *di = nullptr, *dc = nullptr;
+    const size_t n = (size_t)std::max<int64_t>(1, L.N);
+    HCHK(hipMalloc((void **)&dp, 8 * 6 * n));
+    HCHK(hipMalloc((void **)&di, 8 * 12 * n));
+    HCHK(hipMalloc((void **)&dc, 8 * 2 * D * n));
+    ks->export_keys(d, st, nullptr, dp, di, dc, 1);
+    HCHK(hipGetLastError());
+    if (jp) HCHK(hipMemcpyAsync(jp, dp, 8 * 6 * (size_t)L.N, hipMemcpyDeviceToHost, st));
+    if (ji) HCHK(hipMemcpyAsync(ji, di, 8 * 12 * (size_t)L.N, hipMemcpyDeviceToHost, st));
+    if (jc && L.D) HCHK(hipMemcpyAsync(jc, dc, 8 * 2 * (size_t)L.D * L.N, hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    HCHK(hipFree(dp));
+    HCHK(hipFree(di));
+    HCHK(hipFree(dc));
+    return HSBA_OK;
+  }
+
+  int scalars(double *maxdiag, double *ginf) override {
+    if (!built) return HSBA_ERR_GENERIC;
+    int rc = set_state(nullptr, 0.0, 1, 1, 0);
+    if (rc) return rc;
+    ks->grad_stats(d, st);
+    std::vector<double> part(4 * L.chunk_kind.size());
+    HCHK(hipMemcpyAsync(part.data(), d.partial, 8 * part.size(), hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    double gm = 0.0, dm = -1e308;
+    for (size_t c = 0; c < L.chunk_kind.size(); ++c)
+      if (L.chunk_kind[c] < 3) { gm = std::max(gm, part[4 * c]); dm = std::max(dm, part[4 * c + 1]); }
+    if (maxdiag) *maxdiag = dm;
+    if (ginf) *ginf = gm;
+    return HSBA_OK;
+  }
+
+  int cost(double *value) override {
+    int rc = set_state(nullptr, 0.0, 1, 0, 0);
+    if (rc) return rc;
+    ks->cost(d, st, 0);
+    HCHK(hipGetLastError());
+    std::vector<double> part(4 * L.chunk_kind.size());
+    HCHK(hipMemcpyAsync(part.data(), d.partial, 8 * part.size(), hipMemcpyDeviceToHost, st));
+    if ((rc = get_state())) return rc;
+    double F = 0.0;
+    for (size_t c = 0; c < L.chunk_kind.size(); ++c)
+      if (L.chunk_kind[c] == 3 && L.chunk_win[c] == 0) F += part[4 * c + 3];
+    *value = F + ws_host[0].cost_new;
+    return HSBA_OK;
+  }
+};
+
+EngineBase *make_engine(int precision) {
+  if (precision == HSBA_PRECISION_FP64) return new Engine<double>();
+  return new Engine<float>();
+}
+
+}  // namespace hsba

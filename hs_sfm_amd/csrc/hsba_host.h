@@ -1,0 +1,80 @@
+// hsba_host.h — host-side problem layout and the device engine.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hsba.h"
+#include "hsba_kernels.h"
+
+namespace hsba {
+
+struct HostWindow {
+  int64_t P = 0, I = 0, C = 0, N = 0;
+  int64_t x_pt = 0, x_img = 0, x_cam = 0, x_size = 0, y_size = 0;
+  std::vector<int64_t> cons_xidx;  // window-local x index of each constraint row
+  bool unobserved = false;         // a free image / camera with no key
+};
+
+// Host copy of everything the kernels index; built once per problem / batch.
+struct Layout {
+  int IMG = 0, D = 0, E = 0;
+  uint32_t comp = 0, free_bits = 0, fix = 0;
+  int fidx[kFull] = {0};
+  bool pts_free = true, imgs_free = true, cams_free = true, has_cov = false;
+  int nwin = 0;
+  std::vector<HostWindow> hw;
+  std::vector<WinInfo> wi;
+  std::vector<int32_t> key_img, key_pt, key_orig;
+  std::vector<int32_t> pt_ptr, pt_win;
+  std::vector<int32_t> img_ptr, img_keys, img_win, img_cam, img_row;
+  std::vector<int32_t> cam_ptr, cam_imgs, cam_win, cam_row;
+  std::vector<int64_t> pair_ptr;
+  std::vector<uint32_t> inc;
+  std::vector<int32_t> pair_win, pair_img;
+  std::vector<int64_t> blk_ptr;
+  std::vector<Contrib> contrib;
+  std::vector<int32_t> row_blk, row_sub;
+  std::vector<ConstraintRow> cons;
+  std::vector<int32_t> chunk_kind, chunk_begin, chunk_end, chunk_win;
+  std::vector<double> cam_base;    // [C][10] held / identity values
+  std::vector<double> fix_pts, fix_imgs, fix_cam_free;
+  std::vector<int> small_list, large_list;
+  int64_t P = 0, I = 0, C = 0, N = 0, npairs = 0;
+  int64_t S_total = 0, rows_total = 0, dinv_total = 0, lower_total = 0;
+  int64_t incidences = 0;
+};
+
+int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::string *why);
+int64_t layout_device_bytes(const Layout &L, int precision);
+
+struct EngineBase {
+  virtual ~EngineBase() {}
+  Layout L;
+  int precision = HSBA_PRECISION_MIXED;
+  int solver = HSBA_SOLVER_DENSE;
+  double pcg_rtol = 1e-6;
+  int pcg_max_iterations = 500;
+  int glog2 = 3;
+  virtual int init(hipStream_t stream) = 0;
+  virtual int upload_x(const double *const *xs) = 0;
+  virtual int upload_obs(const double *const *ys, const hsba_ycov_inverse *ycovs) = 0;
+  virtual int download_x(double *const *xs) = 0;
+  virtual int lm(const hsba_lm_options *lm, hsba_lm_stats *stats) = 0;
+  virtual int lm_begin(const hsba_lm_options *lm) = 0;
+  virtual int lm_iterate(int iterations, hsba_stage_times *times) = 0;
+  virtual int lm_end(hsba_lm_stats *stats) = 0;
+  virtual int build_normal(double *V, double *Ui, double *Uc, double *g) = 0;
+  virtual int download_normal(double *V, double *Ui, double *Uc, double *g) = 0;
+  virtual int solve(double mu, double *delta, double *S, double *rhs) = 0;
+  virtual int evaluate(double *y) = 0;
+  virtual int jacobian(double *jp, double *ji, double *jc) = 0;
+  virtual int scalars(double *maxdiag, double *ginf) = 0;
+  virtual int cost(double *value) = 0;
+  virtual hipStream_t stream() const = 0;
+};
+
+EngineBase *make_engine(int precision);
+
+}  // namespace hsba

@@ -1,0 +1,50 @@
+// hsba_kernels.h — kernel-set interface between the host driver and the
+// templated HIP kernels (one instantiation per precision and camera block).
+#pragma once
+
+#include "hsba_device.h"
+
+namespace hsba {
+
+template <typename T>
+struct KernelSet {
+  virtual ~KernelSet() {}
+  virtual int E_() const = 0;
+  virtual void linearize(const Dev<T> &d, hipStream_t st, int glog2) = 0;
+  virtual void image_reduce(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void camera_reduce(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void constraints(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void point_factor(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void key_z(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void pairs(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) = 0;
+  virtual void backsub(const Dev<T> &d, hipStream_t st, int glog2) = 0;
+  virtual void step(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void cost(const Dev<T> &d, hipStream_t st, int use_new) = 0;
+  virtual void grad_stats(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void lm_control(const Dev<T> &d, hipStream_t st, int phase) = 0;
+  virtual void commit(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void export_keys(const Dev<T> &d, hipStream_t st, double *y, double *jp, double *ji,
+                           double *jc, int want_j) = 0;
+  // implicit-Schur PCG pieces (hsba_pcg.hip)
+  virtual void pcg_precond(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void pcg_matvec(const Dev<T> &d, hipStream_t st, const double *v, double *out,
+                          int glog2) = 0;
+};
+
+template <typename T>
+KernelSet<T> *make_kernel_set(int img, int dcam);
+
+// Dense FP64 Cholesky of every window's reduced system (hsba_cholesky.hip).
+struct CholPlan {
+  int nwin;
+  const WinInfo *wi;         // device
+  WinState *ws;              // device
+  double *S, *rhs, *dred, *Dinv;
+  const WinInfo *wi_host;    // host copy (large-window launch loop)
+};
+void chol_init_padding(const CholPlan &plan, hipStream_t st);
+void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int num_small,
+                            const int *large_host, int num_large, hipStream_t st);
+
+}  // namespace hsba

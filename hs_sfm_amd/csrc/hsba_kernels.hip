@@ -1,0 +1,886 @@
+// hsba_kernels.hip — per-key, per-point, per-image and per-pair kernels of
+// the MI355X LM bundle adjuster (gfx950, wave64).
+//
+// Storage / accumulation precision T is float (mixed) or double (FP64 path);
+// projections and Jacobians are always evaluated in FP64 registers.  Every
+// reduction runs in a fixed order (segmented by point, image, camera or pair,
+// then a fixed butterfly / tree), so results are bit-identical run to run and
+// no kernel uses global atomics.
+#include "hsba_kernels.h"
+
+namespace hsba {
+
+// ------------------------------------------------------------ small helpers
+template <typename T>
+__device__ __forceinline__ T shfl_xor(T v, int m) { return __shfl_xor(v, m, 64); }
+
+__device__ __forceinline__ bool win_trial(const WinState &s) { return s.active && s.trial; }
+
+template <typename T>
+__device__ __forceinline__ void load_cam(const Dev<T> &d, int c, bool new_x, double cam[kFull]) {
+  const double *src = (new_x ? d.cam_full_new : d.cam_full) + (int64_t)c * kFull;
+#pragma unroll
+  for (int j = 0; j < kFull; ++j) cam[j] = src[j];
+}
+
+// ------------------------------------------------------------ K1+K2 linearize
+// One group of G lanes per point (points are contiguous per window; keys of a
+// point contiguous).  Each lane linearizes keys lane, lane+G, ... of its point:
+// residual and Jacobian in FP64, then W_k = Jp^T L Jr (3 x E) is stored and
+// V_p = sum Jp^T L Jp, g_p = sum Jp^T L r are reduced across the group with a
+// butterfly.  Replaces the per-key loops of
+// camera_shared_analytical_jacobian_matrix_calculator.hpp:94-321 and
+// camera_shared_normal_equation_builder.hpp:207-220, 249-287, 377-391.
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(256) k_linearize(Dev<T> d, int glog2) {
+  constexpr int E = IMG + D;
+  const int G = 1 << glog2;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = gid >> glog2;
+  const int lane = gid & (G - 1);
+  bool act = false;
+  if (p < d.P) {
+    const WinState &s = d.ws[d.pt_win[p]];
+    act = s.active && s.need_lin;
+  }
+  T v[6] = {0, 0, 0, 0, 0, 0}, g[3] = {0, 0, 0};
+  if (act) {
+    const double X[3] = {d.pts[3 * p], d.pts[3 * p + 1], d.pts[3 * p + 2]};
+    const int kb = d.pt_ptr[p], ke = d.pt_ptr[p + 1];
+    for (int k = kb + lane; k < ke; k += G) {
+      const int im = d.key_img[k];
+      double img[6], cam[kFull];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) img[j] = d.imgs[6 * im + j];
+      load_cam(d, d.img_cam[im], false, cam);
+      double res[2], Jp[2][3], Ji[2][6], Jc[2][D > 0 ? D : 1];
+      key_linearize<D, true>(X, img, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
+      if (d.has_cov) {
+        const double *lt = d.key_cov + 3 * k;
+        whiten2(lt, res[0], res[1]);
+#pragma unroll
+        for (int m = 0; m < 3; ++m) whiten2(lt, Jp[0][m], Jp[1][m]);
+#pragma unroll
+        for (int m = 0; m < 6; ++m) whiten2(lt, Ji[0][m], Ji[1][m]);
+#pragma unroll
+        for (int m = 0; m < (D > 0 ? D : 1); ++m) whiten2(lt, Jc[0][m], Jc[1][m]);
+      }
+      v[0] += T(Jp[0][0] * Jp[0][0] + Jp[1][0] * Jp[1][0]);
+      v[1] += T(Jp[0][0] * Jp[0][1] + Jp[1][0] * Jp[1][1]);
+      v[2] += T(Jp[0][0] * Jp[0][2] + Jp[1][0] * Jp[1][2]);
+      v[3] += T(Jp[0][1] * Jp[0][1] + Jp[1][1] * Jp[1][1]);
+      v[4] += T(Jp[0][1] * Jp[0][2] + Jp[1][1] * Jp[1][2]);
+      v[5] += T(Jp[0][2] * Jp[0][2] + Jp[1][2] * Jp[1][2]);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) g[a] += T(Jp[0][a] * res[0] + Jp[1][a] * res[1]);
+      if (E > 0) {
+        T *w = d.W + (int64_t)k * 3 * E;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+#pragma unroll
+          for (int j = 0; j < E; ++j) {
+            double jr0, jr1;
+            if (j < IMG) { jr0 = Ji[0][j]; jr1 = Ji[1][j]; }
+            else { jr0 = Jc[0][j - IMG]; jr1 = Jc[1][j - IMG]; }
+            w[a * E + j] = T(Jp[0][a] * jr0 + Jp[1][a] * jr1);
+          }
+        }
+      }
+    }
+  }
+  for (int m = 1; m < G; m <<= 1) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) v[j] += shfl_xor(v[j], m);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g[j] += shfl_xor(g[j], m);
+  }
+  if (act && lane == 0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) d.V[6 * p + j] = v[j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) d.gp[3 * p + j] = g[j];
+  }
+}
+
+// ------------------------------------------------------------ image blocks
+// One workgroup per image: D_i = sum Jr^T L Jr (E x E, upper triangle) and
+// gr_i = sum Jr^T L r over the image's keys (image-major list), recomputing the
+// Jacobian instead of reading a stored one.  Replaces
+// camera_shared_normal_equation_builder.hpp:221-248, 288-307, 392-415.
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
+  constexpr int E = IMG + D;
+  constexpr int NU = E * (E + 1) / 2;
+  constexpr int NV = NU + E;
+  const int im = blockIdx.x;
+  const WinState &s = d.ws[d.img_win[im]];
+  if (!(s.active && s.need_lin)) return;
+  T acc[NV > 0 ? NV : 1];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) acc[j] = 0;
+  double img[6], cam[kFull];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) img[j] = d.imgs[6 * im + j];
+  load_cam(d, d.img_cam[im], false, cam);
+  const int kb = d.img_ptr[im], ke = d.img_ptr[im + 1];
+  for (int t = kb + threadIdx.x; t < ke; t += blockDim.x) {
+    const int k = d.img_keys[t];
+    const int p = d.key_pt[k];
+    const double X[3] = {d.pts[3 * p], d.pts[3 * p + 1], d.pts[3 * p + 2]};
+    double res[2], Jp[2][3], Ji[2][6], Jc[2][D > 0 ? D : 1];
+    key_linearize<D, true>(X, img, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
+    if (d.has_cov) {
+      const double *lt = d.key_cov + 3 * k;
+      whiten2(lt, res[0], res[1]);
+#pragma unroll
+      for (int m = 0; m < 6; ++m) whiten2(lt, Ji[0][m], Ji[1][m]);
+#pragma unroll
+      for (int m = 0; m < (D > 0 ? D : 1); ++m) whiten2(lt, Jc[0][m], Jc[1][m]);
+    }
+    double jr[2][E > 0 ? E : 1];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+      jr[0][j] = j < IMG ? Ji[0][j] : Jc[0][j - IMG];
+      jr[1][j] = j < IMG ? Ji[1][j] : Jc[1][j - IMG];
+    }
+    int u = 0;
+#pragma unroll
+    for (int a = 0; a < E; ++a) {
+#pragma unroll
+      for (int b = a; b < E; ++b) acc[u++] += T(jr[0][a] * jr[0][b] + jr[1][a] * jr[1][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < E; ++a) acc[NU + a] += T(jr[0][a] * res[0] + jr[1][a] * res[1]);
+  }
+  // wave butterfly then fixed-order sum of the two waves
+  for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] += shfl_xor(acc[j], m);
+  }
+  __shared__ T red[2][NV > 0 ? NV : 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < NV; ++j) red[wave][j] = acc[j];
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < NV; j += blockDim.x) {
+    const T v = red[0][j] + red[1][j];
+    if (j < NU) {
+      // unpack upper-triangle index j -> (a, b) and write both halves
+      int a = 0, rem = j;
+      while (rem >= E - a) { rem -= E - a; ++a; }
+      const int b = a + rem;
+      d.Dimg[(int64_t)im * E * E + a * E + b] = v;
+      d.Dimg[(int64_t)im * E * E + b * E + a] = v;
+    } else {
+      d.gr[(int64_t)im * E + (j - NU)] = v;
+    }
+  }
+}
+
+// ------------------------------------------------------------ camera blocks
+// U_c = sum over the camera's images of the camera part of D_i, g_c likewise,
+// in image order (the reference accumulates U_c per key in key order,
+// camera_shared_normal_equation_builder.hpp:235-248, 403-415).
+template <typename T, int IMG, int D>
+__global__ void k_camera_reduce(Dev<T> d) {
+  constexpr int E = IMG + D;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (D == 0 || c >= d.C) return;
+  const WinState &s = d.ws[d.cam_win[c]];
+  if (!(s.active && s.need_lin)) return;
+  T u[D > 0 ? D * D : 1], g[D > 0 ? D : 1];
+#pragma unroll
+  for (int j = 0; j < (D > 0 ? D * D : 1); ++j) u[j] = 0;
+#pragma unroll
+  for (int j = 0; j < (D > 0 ? D : 1); ++j) g[j] = 0;
+  for (int t = d.cam_ptr[c]; t < d.cam_ptr[c + 1]; ++t) {
+    const int im = d.cam_imgs[t];
+    const T *Di = d.Dimg + (int64_t)im * E * E;
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+#pragma unroll
+      for (int b = 0; b < D; ++b) u[a * D + b] += Di[(IMG + a) * E + IMG + b];
+      g[a] += d.gr[(int64_t)im * E + IMG + a];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < D * D; ++j) d.Uc[(int64_t)c * D * D + j] = u[j];
+#pragma unroll
+  for (int j = 0; j < D; ++j) d.gc[(int64_t)c * D + j] = g[j];
+}
+
+// Constraint rows (identity rows of the vector function): weight on the
+// diagonal and w * r in the gradient, in y order by a single thread so the
+// order of additions matches the reference
+// (camera_shared_normal_equation_builder.hpp:309-349, 418-459).
+template <typename T, int IMG, int D>
+__global__ void k_constraints(Dev<T> d) {
+  constexpr int E = IMG + D;
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int q = 0; q < d.ncons; ++q) {
+    const ConstraintRow cr = d.cons[q];
+    const WinState &s = d.ws[cr.win];
+    if (!(s.active && s.need_lin)) continue;
+    if (cr.kind == 0) {
+      const double r = d.pts[3 * cr.entity + cr.param] - cr.y;
+      const int vi = cr.param == 0 ? 0 : (cr.param == 1 ? 3 : 5);
+      d.V[6 * cr.entity + vi] += T(cr.weight);
+      d.gp[3 * cr.entity + cr.param] += T(cr.weight * r);
+    } else if (cr.kind == 1) {
+      const double r = d.imgs[6 * cr.entity + cr.param] - cr.y;
+      d.Dimg[(int64_t)cr.entity * E * E + cr.param * E + cr.param] += T(cr.weight);
+      d.gr[(int64_t)cr.entity * E + cr.param] += T(cr.weight * r);
+    } else {
+      const double r = d.cam_free[(int64_t)cr.entity * (D > 0 ? D : 1) + cr.param] - cr.y;
+      d.Uc[(int64_t)cr.entity * D * D + cr.param * D + cr.param] += T(cr.weight);
+      d.gc[(int64_t)cr.entity * D + cr.param] += T(cr.weight * r);
+    }
+  }
+}
+
+// ------------------------------------------------------------ K3 point factor
+// V* = V + mu I, V* = L L^T (FP64), store L^-1 (lower, 6 values) and
+// q = L^-1 g_p.  Then W^T V*^-1 W' = (L^-1 W)^T (L^-1 W') — the inverse of
+// camera_shared_normal_equation_solver.hpp:141-148 in factored form.
+template <typename T>
+__global__ void k_point_factor(Dev<T> d) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= d.P) return;
+  WinState &s = d.ws[d.pt_win[p]];
+  if (!win_trial(s)) return;
+  const double mu = s.mu;
+  const double a00 = double(d.V[6 * p]) + mu, a01 = d.V[6 * p + 1], a02 = d.V[6 * p + 2];
+  const double a11 = double(d.V[6 * p + 3]) + mu, a12 = d.V[6 * p + 4], a22 = double(d.V[6 * p + 5]) + mu;
+  const double l00 = sqrt(a00);
+  const double l10 = a01 / l00, l20 = a02 / l00;
+  const double t11 = a11 - l10 * l10;
+  const double l11 = sqrt(t11);
+  const double l21 = (a12 - l20 * l10) / l11;
+  const double t22 = a22 - l20 * l20 - l21 * l21;
+  const double l22 = sqrt(t22);
+  if (!(a00 > 0.0) || !(t11 > 0.0) || !(t22 > 0.0)) {
+    s.chol_fail = 1;  // benign race: every writer stores 1
+    return;
+  }
+  // inverse of lower-triangular L
+  const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
+  const double i10 = -l10 * i00 * i11;
+  const double i21 = -l21 * i11 * i22;
+  const double i20 = -(l20 * i00 + l21 * i10) * i22;
+  T *L = d.Linv + 6 * p;
+  L[0] = T(i00); L[1] = T(i10); L[2] = T(i11); L[3] = T(i20); L[4] = T(i21); L[5] = T(i22);
+  const double g0 = d.gp[3 * p], g1 = d.gp[3 * p + 1], g2 = d.gp[3 * p + 2];
+  d.q[3 * p] = T(i00 * g0);
+  d.q[3 * p + 1] = T(i10 * g0 + i11 * g1);
+  d.q[3 * p + 2] = T(i20 * g0 + i21 * g1 + i22 * g2);
+}
+
+// Z_k = L_p^-1 W_k, one thread per key.
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.N) return;
+  const int p = d.key_pt[k];
+  if (!win_trial(d.ws[d.pt_win[p]])) return;
+  const T *L = d.Linv + 6 * p;
+  const T l00 = L[0], l10 = L[1], l11 = L[2], l20 = L[3], l21 = L[4], l22 = L[5];
+  const T *w = d.W + (int64_t)k * 3 * E;
+  T *z = d.Z + (int64_t)k * 3 * E;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const T w0 = w[j], w1 = w[E + j], w2 = w[2 * E + j];
+    z[j] = l00 * w0;
+    z[E + j] = l10 * w0 + l11 * w1;
+    z[2 * E + j] = l20 * w0 + l21 * w1 + l22 * w2;
+  }
+}
+
+// ------------------------------------------------------------ K4 camera pairs
+// One wavefront per image pair (ia <= ib): P = sum over points seeing both of
+// Z_a^T Z_b (E x E), plus for diagonal pairs rp_i = sum Z_a^T q_p.  Lanes
+// stride the pair's incidence list; the 64 partial blocks are combined by a
+// fixed butterfly.  Replaces the O(I^2 P) loops of
+// camera_shared_normal_equation_solver.hpp:198-443.
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
+  const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (pair >= d.npairs) return;
+  if (!win_trial(d.ws[d.pair_win[pair]])) return;
+  const bool diag = d.pair_img[pair] >= 0;
+  T acc[E * E];
+  T racc[E];
+#pragma unroll
+  for (int j = 0; j < E * E; ++j) acc[j] = 0;
+#pragma unroll
+  for (int j = 0; j < E; ++j) racc[j] = 0;
+  const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
+  for (int64_t j = jb + lane; j < je; j += 64) {
+    const uint32_t code = d.inc[j];
+    const int a = int(code >> 7);
+    const int b = a + int(code & 127u);
+    const T *za = d.Z + (int64_t)a * 3 * E;
+    const T *zb = d.Z + (int64_t)b * 3 * E;
+    T A[3][E], B[3][E];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int c = 0; c < E; ++c) { A[r][c] = za[r * E + c]; B[r][c] = zb[r * E + c]; }
+    }
+#pragma unroll
+    for (int r = 0; r < E; ++r) {
+#pragma unroll
+      for (int c = 0; c < E; ++c) acc[r * E + c] += A[0][r] * B[0][c] + A[1][r] * B[1][c] + A[2][r] * B[2][c];
+    }
+    if (diag) {
+      const int p = d.key_pt[a];
+      const T q0 = d.q[3 * p], q1 = d.q[3 * p + 1], q2 = d.q[3 * p + 2];
+#pragma unroll
+      for (int r = 0; r < E; ++r) racc[r] += A[0][r] * q0 + A[1][r] * q1 + A[2][r] * q2;
+    }
+  }
+  for (int m = 1; m < 64; m <<= 1) {
+#pragma unroll
+    for (int j = 0; j < E * E; ++j) acc[j] += shfl_xor(acc[j], m);
+    if (diag) {
+#pragma unroll
+      for (int j = 0; j < E; ++j) racc[j] += shfl_xor(racc[j], m);
+    }
+  }
+  T *out = d.pair_out + (int64_t)pair * E * E;
+#pragma unroll
+  for (int j = 0; j < E * E; ++j)
+    if ((j & 63) == lane) out[j] = acc[j];
+  if (diag) {
+    T *rp = d.rp + (int64_t)d.pair_img[pair] * E;
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+      if (j == lane) rp[j] = racc[j];
+  }
+}
+
+// ------------------------------------------------------------ assembly
+// One thread per lower-triangle entry of a window's reduced matrix (FP64):
+// S = U + mu I (+ image-camera coupling) - sum of pair blocks, by the
+// per-block contribution lists built at problem creation.
+template <typename T, int IMG, int D>
+__global__ void k_assemble(Dev<T> d, int64_t total) {
+  constexpr int E = IMG + D;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  // window by binary search over lower-entry offsets
+  int lo = 0, hi = d.nwin - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.wi[mid].lo_off <= t) lo = mid; else hi = mid - 1;
+  }
+  const WinInfo &wi = d.wi[lo];
+  const WinState &s = d.ws[lo];
+  if (!win_trial(s) || s.chol_fail) return;
+  const int64_t u = t - wi.lo_off;
+  int64_t R = (int64_t)((sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
+  while (R * (R + 1) / 2 > u) --R;
+  while ((R + 1) * (R + 2) / 2 <= u) ++R;
+  const int64_t Cc = u - R * (R + 1) / 2;
+  const int bR = d.row_blk[wi.r_off + R], bC = d.row_blk[wi.r_off + Cc];
+  const int oR = d.row_sub[wi.r_off + R], oC = d.row_sub[wi.r_off + Cc];
+  const int64_t blk = (int64_t)bR * (bR + 1) / 2 + bC;
+  double v = (R == Cc) ? s.mu : 0.0;
+  for (int64_t j = d.blk_ptr[wi.blk_off + blk]; j < d.blk_ptr[wi.blk_off + blk + 1]; ++j) {
+    const Contrib cb = d.contrib[j];
+    if (cb.kind == 0) {
+      v += (double)d.Dimg[(int64_t)cb.id * E * E + oR * E + oC];
+    } else if (cb.kind == 1) {
+      v += (double)d.Uc[(int64_t)cb.id * D * D + oR * D + oC];
+    } else if (cb.kind == 2) {
+      // camera row block x image column block: W_ic^T = D_i[IMG + oR][oC]
+      v += (double)d.Dimg[(int64_t)cb.id * E * E + (IMG + oR) * E + oC];
+    } else {
+      const T *P = d.pair_out + (int64_t)cb.id * E * E;
+      const int r = cb.ro + oR, c = cb.so + oC;
+      v -= cb.trans ? (double)P[c * E + r] : (double)P[r * E + c];
+    }
+  }
+  d.S[wi.s_off + R * wi.ld + Cc] = v;
+}
+
+// Reduced right-hand side: g_r - sum of rp over the row's images.
+template <typename T, int IMG, int D>
+__global__ void k_assemble_rhs(Dev<T> d, int rows) {
+  constexpr int E = IMG + D;
+  const int R = blockIdx.x * blockDim.x + threadIdx.x;
+  if (R >= rows) return;
+  // row -> window via row_blk sign: windows store row ranges contiguously
+  int lo = 0, hi = d.nwin - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.wi[mid].r_off <= R) lo = mid; else hi = mid - 1;
+  }
+  const WinInfo &wi = d.wi[lo];
+  const WinState &s = d.ws[lo];
+  if (!win_trial(s) || s.chol_fail) return;
+  const int r = R - wi.r_off;
+  if (r >= wi.n) return;  // padding rows: the solves read zeros there
+  const int blk = d.row_blk[R], sub = d.row_sub[R];
+  double v;
+  if (IMG > 0 && r < wi.cb) {
+    const int im = wi.img_off + blk;
+    v = (double)d.gr[(int64_t)im * E + sub];
+    if (d.pts_free) v -= (double)d.rp[(int64_t)im * E + sub];
+  } else {
+    const int c = wi.cam_off + (blk - (IMG > 0 ? wi.num_imgs : 0));
+    v = (double)d.gc[(int64_t)c * D + sub];
+    if (d.pts_free)
+      for (int t = d.cam_ptr[c]; t < d.cam_ptr[c + 1]; ++t)
+        v -= (double)d.rp[(int64_t)d.cam_imgs[t] * E + IMG + sub];
+  }
+  d.rhs[R] = v;
+}
+
+// ------------------------------------------------------------ K6 back-substitution
+// delta_p = L^-T (q_p - sum_a Z_a delta_r(a)), one lane group per point
+// (camera_shared_normal_equation_solver.hpp:468-538 in factored form).
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(256) k_backsub(Dev<T> d, int glog2) {
+  constexpr int E = IMG + D;
+  const int G = 1 << glog2;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = gid >> glog2;
+  const int lane = gid & (G - 1);
+  bool act = false;
+  if (p < d.P) {
+    const WinState &s = d.ws[d.pt_win[p]];
+    act = win_trial(s) && !s.chol_fail;
+  }
+  double t[3] = {0.0, 0.0, 0.0};
+  if (act && E > 0) {
+    for (int k = d.pt_ptr[p] + lane; k < d.pt_ptr[p + 1]; k += G) {
+      const int im = d.key_img[k];
+      const T *z = d.Z + (int64_t)k * 3 * E;
+      double dr[E > 0 ? E : 1];
+#pragma unroll
+      for (int j = 0; j < E; ++j)
+        dr[j] = j < IMG ? d.dred[d.img_row[im] + j] : d.dred[d.cam_row[d.img_cam[im]] + (j - IMG)];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < E; ++j) acc += (double)z[a * E + j] * dr[j];
+        t[a] += acc;
+      }
+    }
+  }
+  for (int m = 1; m < G; m <<= 1) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) t[a] += shfl_xor(t[a], m);
+  }
+  if (act && lane == 0) {
+    const T *L = d.Linv + 6 * p;
+    const double u0 = (double)d.q[3 * p] - t[0], u1 = (double)d.q[3 * p + 1] - t[1],
+                 u2 = (double)d.q[3 * p + 2] - t[2];
+    // delta = L^-T u with L^-1 = [[i00,0,0],[i10,i11,0],[i20,i21,i22]]
+    const double i00 = L[0], i10 = L[1], i11 = L[2], i20 = L[3], i21 = L[4], i22 = L[5];
+    d.dpt[3 * p] = i00 * u0 + i10 * u1 + i20 * u2;
+    d.dpt[3 * p + 1] = i11 * u1 + i21 * u2;
+    d.dpt[3 * p + 2] = i22 * u2;
+  }
+}
+
+// ------------------------------------------------------------ step statistics
+// Chunked reductions over parameters, one workgroup per chunk (a chunk never
+// straddles windows): x_new = x - delta and partial sums of |delta|^2, |x|^2
+// and delta . (mu delta + g) in FP64.  kind 0 points, 1 images, 2 cameras.
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(256) k_step(Dev<T> d) {
+  constexpr int E = IMG + D;
+  const int ch = blockIdx.x;
+  const int kind = d.chunk_kind[ch];
+  const int w = d.chunk_win[ch];
+  const WinState &s = d.ws[w];
+  __shared__ double red[3][256];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  const bool act = win_trial(s) && !s.chol_fail;
+  if (act && kind < 3) {
+    const double mu = s.mu;
+    for (int e = d.chunk_begin[ch] + threadIdx.x; e < d.chunk_end[ch]; e += blockDim.x) {
+      if (kind == 0) {
+        const int p = e / 3, j = e - 3 * p;
+        const double dl = d.dpt[e], x = d.pts[e];
+        const double g = d.gp[3 * p + j];
+        d.pts_new[e] = x - dl;
+        a0 += dl * dl; a1 += x * x; a2 += dl * (mu * dl + g);
+      } else if (kind == 1) {
+        const int im = e / 6, j = e - 6 * im;
+        const double dl = d.dred[d.img_row[im] + j], x = d.imgs[e];
+        const double g = d.gr[(int64_t)im * E + j];
+        d.imgs_new[e] = x - dl;
+        a0 += dl * dl; a1 += x * x; a2 += dl * (mu * dl + g);
+      } else {
+        const int c = e / (D > 0 ? D : 1), j = e - c * (D > 0 ? D : 1);
+        const double dl = d.dred[d.cam_row[c] + j], x = d.cam_free[e];
+        const double g = d.gc[(int64_t)c * D + j];
+        d.cam_free_new[e] = x - dl;
+        d.cam_full_new[(int64_t)c * kFull + d.fidx[j]] = x - dl;
+        a0 += dl * dl; a1 += x * x; a2 += dl * (mu * dl + g);
+      }
+    }
+  }
+  red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1; red[2][threadIdx.x] = a2;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + h];
+      red[1][threadIdx.x] += red[1][threadIdx.x + h];
+      red[2][threadIdx.x] += red[2][threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && kind < 3) {
+    d.partial[4 * ch] = red[0][0];
+    d.partial[4 * ch + 1] = red[1][0];
+    d.partial[4 * ch + 2] = red[2][0];
+  }
+}
+
+// K7 candidate cost: chunks of keys (kind 3), FP64 residuals at x_new,
+// partial sums of r^T Lambda r; camera_shared_mahalanobis_distance_calculator.hpp:28-61.
+// `use_new` = 0 evaluates at the current x (initial cost).
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(256) k_cost(Dev<T> d, int use_new) {
+  const int ch = blockIdx.x;
+  const int kind = d.chunk_kind[ch];
+  const WinState &s = d.ws[d.chunk_win[ch]];
+  __shared__ double red[256];
+  double acc = 0.0;
+  const bool act = use_new ? (win_trial(s) && s.take_step) : (s.active != 0);
+  if (act && kind == 3) {
+    const double *pts = use_new && d.pts_free ? d.pts_new : d.pts;
+    const double *imgs = use_new && d.imgs_free ? d.imgs_new : d.imgs;
+    const double *camf = use_new && d.cams_free ? d.cam_full_new : d.cam_full;
+    for (int k = d.chunk_begin[ch] + threadIdx.x; k < d.chunk_end[ch]; k += blockDim.x) {
+      const int p = d.key_pt[k], im = d.key_img[k];
+      const double X[3] = {pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]};
+      double img[6], cam[kFull];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) img[j] = imgs[6 * im + j];
+      const int c = d.img_cam[im];
+#pragma unroll
+      for (int j = 0; j < kFull; ++j) cam[j] = camf[(int64_t)c * kFull + j];
+      double res[2], Jp[2][3], Ji[2][6], Jc[2][1];
+      key_linearize<0, false>(X, img, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
+      if (d.has_cov) whiten2(d.key_cov + 3 * k, res[0], res[1]);
+      acc += res[0] * res[0] + res[1] * res[1];
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && kind == 3) d.partial[4 * ch + 3] = red[0];
+}
+
+// Gradient / diagonal statistics after a linearization: chunk partials of
+// max |g| and max diag(N) (kinds 0..2).
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(256) k_grad_stats(Dev<T> d) {
+  constexpr int E = IMG + D;
+  const int ch = blockIdx.x;
+  const int kind = d.chunk_kind[ch];
+  const WinState &s = d.ws[d.chunk_win[ch]];
+  __shared__ double red[2][256];
+  double gm = 0.0, dm = -1e308;
+  if (s.active && s.need_lin && kind < 3) {
+    for (int e = d.chunk_begin[ch] + threadIdx.x; e < d.chunk_end[ch]; e += blockDim.x) {
+      if (kind == 0) {
+        const int p = e / 3, j = e - 3 * p;
+        gm = fmax(gm, fabs((double)d.gp[e]));
+        dm = fmax(dm, (double)d.V[6 * p + (j == 0 ? 0 : (j == 1 ? 3 : 5))]);
+      } else if (kind == 1) {
+        const int im = e / 6, j = e - 6 * im;
+        gm = fmax(gm, fabs((double)d.gr[(int64_t)im * E + j]));
+        dm = fmax(dm, (double)d.Dimg[(int64_t)im * E * E + j * E + j]);
+      } else {
+        const int c = e / (D > 0 ? D : 1), j = e - c * (D > 0 ? D : 1);
+        gm = fmax(gm, fabs((double)d.gc[(int64_t)c * D + j]));
+        dm = fmax(dm, (double)d.Uc[(int64_t)c * D * D + j * D + j]);
+      }
+    }
+  }
+  red[0][threadIdx.x] = gm; red[1][threadIdx.x] = dm;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      red[0][threadIdx.x] = fmax(red[0][threadIdx.x], red[0][threadIdx.x + h]);
+      red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && kind < 3) {
+    d.partial[4 * ch] = red[0][0];
+    d.partial[4 * ch + 1] = red[1][0];
+  }
+}
+
+// ------------------------------------------------------------ K8 LM control
+// One thread per window.  Madsen–Nielsen–Tingleff Algorithm 3.16 with the
+// reference's conventions: F = r^T W r, (N + mu I) delta = g, x_new = x - delta,
+// rho = (F - F_new) / (delta^T (mu delta + g)).
+//   phase 0: after the initial cost + linearization: F0, mu0 = tau max diag N,
+//            found = ||g||_inf <= eps1
+//   phase 1: start of an iteration: stop at max_iter, mark the trial
+//   phase 2: after the solve: ||delta|| <= eps2 (||x|| + eps2) -> stop
+//   phase 3: after the candidate cost: gain ratio, accept / reject, mu update
+//   phase 4: after a relinearization: ||g||_inf <= eps1 -> stop
+template <typename T>
+__global__ void k_lm_control(Dev<T> d, int phase) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= d.nwin) return;
+  WinState &s = d.ws[w];
+  // chunk range of this window: chunks are sorted by window
+  int cb = 0, ce = d.nchunks;
+  {
+    int lo = 0, hi = d.nchunks;
+    while (lo < hi) { const int m = (lo + hi) >> 1; if (d.chunk_win[m] < w) lo = m + 1; else hi = m; }
+    cb = lo; hi = d.nchunks;
+    while (lo < hi) { const int m = (lo + hi) >> 1; if (d.chunk_win[m] <= w) lo = m + 1; else hi = m; }
+    ce = lo;
+  }
+  if (phase == 0) {
+    if (!s.active) return;
+    double F = 0.0, gm = 0.0, dm = -1e308;
+    for (int c = cb; c < ce; ++c) {
+      const int k = d.chunk_kind[c];
+      if (k == 3) F += d.partial[4 * c + 3];
+      else { gm = fmax(gm, d.partial[4 * c]); dm = fmax(dm, d.partial[4 * c + 1]); }
+    }
+    s.cost = F + s.cost_new;  // cost_new holds the constraint part
+    s.cost0 = s.cost;
+    s.ginf = gm; s.maxdiag = dm;
+    s.mu = s.tau * dm;
+    s.nu = 2.0;
+    s.need_lin = 0;
+    if (!(F == F) || !(gm == gm)) { s.status = -4; s.stop_reason = 3; s.active = 0; return; }
+    if (gm <= s.eps1) { s.stop_reason = 1; s.active = 0; }
+    if (s.max_iter <= 0) { s.active = 0; }
+  } else if (phase == 1) {
+    s.trial = 0;
+    s.take_step = 0;
+    s.step_accepted = 0;
+    s.chol_fail = 0;
+    if (!s.active) return;
+    if (s.iter >= s.max_iter) { s.active = 0; s.stop_reason = 0; return; }
+    s.iter += 1;
+    s.trial = 1;
+  } else if (phase == 2) {
+    if (!(s.active && s.trial)) return;
+    if (s.chol_fail) {
+      s.status = -3; s.stop_reason = 3; s.active = 0; s.trial = 0; return;
+    }
+    double dn = 0.0, xn = 0.0, gd = 0.0;
+    for (int c = cb; c < ce; ++c) {
+      if (d.chunk_kind[c] < 3) { dn += d.partial[4 * c]; xn += d.partial[4 * c + 1]; gd += d.partial[4 * c + 2]; }
+    }
+    s.dnorm2 = dn; s.xnorm2 = xn; s.gain_den = gd;
+    if (!(dn == dn)) { s.status = -4; s.stop_reason = 3; s.active = 0; s.trial = 0; return; }
+    if (sqrt(dn) <= s.eps2 * (sqrt(xn) + s.eps2)) {
+      s.stop_reason = 2; s.active = 0; s.trial = 0; return;
+    }
+    s.take_step = 1;
+  } else if (phase == 3) {
+    if (!(s.active && s.trial && s.take_step)) return;
+    double F = 0.0;
+    for (int c = cb; c < ce; ++c)
+      if (d.chunk_kind[c] == 3) F += d.partial[4 * c + 3];
+    F += s.cost_new;  // constraint part, written by k_constraint_cost
+    s.cost_new = F;
+    const double rho = (s.cost - F) / s.gain_den;
+    s.rho = rho;
+    if (rho > 0.0 && F == F) {
+      s.cost = F;
+      const double t = 2.0 * rho - 1.0;
+      s.mu *= fmax(1.0 / 3.0, 1.0 - t * t * t);
+      s.nu = 2.0;
+      s.accepted += 1;
+      s.step_accepted = 1;
+      s.need_lin = 1;
+    } else {
+      s.mu *= s.nu;
+      s.nu *= 2.0;
+      s.need_lin = 0;
+    }
+  } else if (phase == 4) {
+    if (!(s.active && s.need_lin)) return;
+    double gm = 0.0;
+    for (int c = cb; c < ce; ++c)
+      if (d.chunk_kind[c] < 3) gm = fmax(gm, d.partial[4 * c]);
+    s.ginf = gm;
+    s.need_lin = 0;
+    if (gm <= s.eps1) { s.stop_reason = 1; s.active = 0; }
+  }
+}
+
+// Constraint-row part of the cost (single thread, y order) into cost_new.
+template <typename T>
+__global__ void k_constraint_cost(Dev<T> d, int use_new) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int w = 0; w < d.nwin; ++w) d.ws[w].cost_new = 0.0;
+  for (int q = 0; q < d.ncons; ++q) {
+    const ConstraintRow cr = d.cons[q];
+    WinState &s = d.ws[cr.win];
+    const bool act = use_new ? (s.active && s.trial && s.take_step) : (s.active != 0);
+    if (!act) continue;
+    double x;
+    if (cr.kind == 0) x = (use_new ? d.pts_new : d.pts)[3 * cr.entity + cr.param];
+    else if (cr.kind == 1) x = (use_new ? d.imgs_new : d.imgs)[6 * cr.entity + cr.param];
+    else x = (use_new ? d.cam_full_new : d.cam_full)[(int64_t)cr.entity * kFull + d.fidx[cr.param]];
+    const double r = x - cr.y;
+    s.cost_new += r * cr.weight * r;
+  }
+}
+
+// Commit x_new -> x for windows whose step was accepted.
+template <typename T, int IMG, int D>
+__global__ void k_commit(Dev<T> d) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t np = d.pts_free ? 3LL * d.P : 0;
+  const int64_t ni = d.imgs_free ? 6LL * d.I : 0;
+  const int64_t nc = d.cams_free ? (int64_t)kFull * d.C : 0;
+  if (e < np) {
+    if (d.ws[d.pt_win[e / 3]].step_accepted) d.pts[e] = d.pts_new[e];
+  } else if (e < np + ni) {
+    const int64_t j = e - np;
+    if (d.ws[d.img_win[j / 6]].step_accepted) d.imgs[j] = d.imgs_new[j];
+  } else if (e < np + ni + nc) {
+    const int64_t j = e - np - ni;
+    const int c = (int)(j / kFull), q = (int)(j - (int64_t)c * kFull);
+    if (d.ws[d.cam_win[c]].step_accepted) {
+      d.cam_full[j] = d.cam_full_new[j];
+      if (q < D) d.cam_free[(int64_t)c * D + q] = d.cam_free_new[(int64_t)c * D + q];
+    }
+  }
+}
+
+// ------------------------------------------------------------ export kernels
+// Per-key residuals and Jacobian blocks in the caller's key order (FP64),
+// used by hsba_evaluate / hsba_jacobian.
+template <typename T, int IMG, int D>
+__global__ void k_export_keys(Dev<T> d, double *y, double *jp, double *ji, double *jc, int want_j) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.N) return;
+  const int p = d.key_pt[k], im = d.key_img[k];
+  const double X[3] = {d.pts[3 * p], d.pts[3 * p + 1], d.pts[3 * p + 2]};
+  double img[6], cam[kFull];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) img[j] = d.imgs[6 * im + j];
+  load_cam(d, d.img_cam[im], false, cam);
+  double res[2], Jp[2][3], Ji[2][6], Jc[2][D > 0 ? D : 1];
+  // observation 0: res is the projection itself
+  key_linearize<D, true>(X, img, cam, d.fidx, 0.0, 0.0, res, Jp, Ji, Jc);
+  const int64_t o = d.key_orig[k];  // global original key index
+  if (y) { y[2 * o] = res[0]; y[2 * o + 1] = res[1]; }
+  if (!want_j) return;
+  if (jp)
+    for (int a = 0; a < 2; ++a)
+      for (int m = 0; m < 3; ++m) jp[o * 6 + a * 3 + m] = Jp[a][m];
+  if (ji)
+    for (int a = 0; a < 2; ++a)
+      for (int m = 0; m < 6; ++m) ji[o * 12 + a * 6 + m] = Ji[a][m];
+  if (jc && D > 0)
+    for (int a = 0; a < 2; ++a)
+      for (int m = 0; m < D; ++m) jc[o * 2 * D + a * D + m] = Jc[a][m];
+}
+
+// ------------------------------------------------------------ launchers
+template <typename T, int IMG, int D>
+struct KernelSetImpl final : KernelSet<T> {
+  static constexpr int E = IMG + D;
+  int E_() const override { return E; }
+  void linearize(const Dev<T> &d, hipStream_t st, int glog2) override {
+    if (!d.pts_free || d.P == 0) return;
+    const int64_t threads = (int64_t)d.P << glog2;
+    hipLaunchKernelGGL((k_linearize<T, IMG, D>), dim3((threads + 255) / 256), dim3(256), 0, st, d, glog2);
+  }
+  void image_reduce(const Dev<T> &d, hipStream_t st) override {
+    if (E == 0 || d.I == 0) return;
+    hipLaunchKernelGGL((k_image_reduce<T, IMG, D>), dim3(d.I), dim3(128), 0, st, d);
+  }
+  void camera_reduce(const Dev<T> &d, hipStream_t st) override {
+    if (D == 0 || d.C == 0) return;
+    hipLaunchKernelGGL((k_camera_reduce<T, IMG, D>), dim3((d.C + 127) / 128), dim3(128), 0, st, d);
+  }
+  void constraints(const Dev<T> &d, hipStream_t st) override {
+    if (d.ncons == 0) return;
+    hipLaunchKernelGGL((k_constraints<T, IMG, D>), dim3(1), dim3(64), 0, st, d);
+  }
+  void point_factor(const Dev<T> &d, hipStream_t st) override {
+    if (!d.pts_free || d.P == 0) return;
+    hipLaunchKernelGGL((k_point_factor<T>), dim3((d.P + 255) / 256), dim3(256), 0, st, d);
+  }
+  void key_z(const Dev<T> &d, hipStream_t st) override {
+    if (!d.pts_free || E == 0 || d.N == 0) return;
+    hipLaunchKernelGGL((k_key_z<T, (E > 0 ? E : 1)>), dim3((d.N + 255) / 256), dim3(256), 0, st, d);
+  }
+  void pairs(const Dev<T> &d, hipStream_t st) override {
+    if (!d.pts_free || E == 0 || d.npairs == 0) return;
+    hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3((d.npairs + 3) / 4), dim3(256), 0, st, d);
+  }
+  void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) override {
+    if (E == 0 || rows == 0) return;
+    hipLaunchKernelGGL((k_assemble<T, IMG, D>), dim3((lower_total + 255) / 256), dim3(256), 0, st, d,
+                       lower_total);
+    hipLaunchKernelGGL((k_assemble_rhs<T, IMG, D>), dim3((rows + 255) / 256), dim3(256), 0, st, d, rows);
+  }
+  void backsub(const Dev<T> &d, hipStream_t st, int glog2) override {
+    if (!d.pts_free || d.P == 0) return;
+    const int64_t threads = (int64_t)d.P << glog2;
+    hipLaunchKernelGGL((k_backsub<T, IMG, D>), dim3((threads + 255) / 256), dim3(256), 0, st, d, glog2);
+  }
+  void step(const Dev<T> &d, hipStream_t st) override {
+    hipLaunchKernelGGL((k_step<T, IMG, D>), dim3(d.nchunks), dim3(256), 0, st, d);
+  }
+  void cost(const Dev<T> &d, hipStream_t st, int use_new) override {
+    hipLaunchKernelGGL((k_constraint_cost<T>), dim3(1), dim3(64), 0, st, d, use_new);
+    hipLaunchKernelGGL((k_cost<T, IMG, D>), dim3(d.nchunks), dim3(256), 0, st, d, use_new);
+  }
+  void grad_stats(const Dev<T> &d, hipStream_t st) override {
+    hipLaunchKernelGGL((k_grad_stats<T, IMG, D>), dim3(d.nchunks), dim3(256), 0, st, d);
+  }
+  void lm_control(const Dev<T> &d, hipStream_t st, int phase) override {
+    hipLaunchKernelGGL((k_lm_control<T>), dim3((d.nwin + 63) / 64), dim3(64), 0, st, d, phase);
+  }
+  void commit(const Dev<T> &d, hipStream_t st) override {
+    const int64_t n = (d.pts_free ? 3LL * d.P : 0) + (d.imgs_free ? 6LL * d.I : 0) +
+                      (d.cams_free ? (int64_t)kFull * d.C : 0);
+    if (n == 0) return;
+    hipLaunchKernelGGL((k_commit<T, IMG, D>), dim3((n + 255) / 256), dim3(256), 0, st, d);
+  }
+  void pcg_precond(const Dev<T> &d, hipStream_t st) override {}
+  void pcg_matvec(const Dev<T> &d, hipStream_t st, const double *v, double *out, int glog2) override {}
+  void export_keys(const Dev<T> &d, hipStream_t st, double *y, double *jp, double *ji, double *jc,
+                   int want_j) override {
+    if (d.N == 0) return;
+    hipLaunchKernelGGL((k_export_keys<T, IMG, D>), dim3((d.N + 255) / 256), dim3(256), 0, st, d, y, jp, ji,
+                       jc, want_j);
+  }
+};
+
+template <typename T>
+KernelSet<T> *make_kernel_set(int img, int dcam) {
+#define HSBA_CASE(I_, D_) \
+  if (img == I_ && dcam == D_) return new KernelSetImpl<T, I_, D_>();
+  HSBA_CASE(6, 0) HSBA_CASE(6, 1) HSBA_CASE(6, 2) HSBA_CASE(6, 3) HSBA_CASE(6, 4) HSBA_CASE(6, 5)
+  HSBA_CASE(6, 6) HSBA_CASE(6, 7) HSBA_CASE(6, 8) HSBA_CASE(6, 9) HSBA_CASE(6, 10)
+  HSBA_CASE(0, 0) HSBA_CASE(0, 1) HSBA_CASE(0, 2) HSBA_CASE(0, 3) HSBA_CASE(0, 4) HSBA_CASE(0, 5)
+  HSBA_CASE(0, 6) HSBA_CASE(0, 7) HSBA_CASE(0, 8) HSBA_CASE(0, 9) HSBA_CASE(0, 10)
+#undef HSBA_CASE
+  return nullptr;
+}
+
+template KernelSet<float> *make_kernel_set<float>(int, int);
+template KernelSet<double> *make_kernel_set<double>(int, int);
+
+}  // namespace hsba

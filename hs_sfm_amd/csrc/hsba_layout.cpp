@@ -1,0 +1,369 @@
+// hsba_layout.cpp — host-side structure of a problem or batch of windows.
+//
+// Mirrors the sparsity pass of CameraSharedNormalEquationBuilder::
+// ComputeNormalMatrix (include/hs_sfm/bundle_adjustment/
+// camera_shared_normal_equation_builder.hpp:137-205): instead of Eigen sparse
+// maps of (point, image) / (point, camera) / (image, camera) blocks it builds
+//   * keys in point-major order (a point's keys contiguous, sorted by image),
+//   * point CSR, image CSR (image-major key lists), camera -> image lists,
+//   * the camera-pair incidence lists of the Schur complement
+//     (pairs (ia <= ib) of images that share a point, one incidence per point),
+//   * the reduced-system layout [images 6I | cameras dC] of
+//     camera_shared_normal_equation_solver.hpp:131-135 with per-block
+//     contribution lists,
+//   * chunk tables for the deterministic per-window reductions.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "hsba_host.h"
+
+namespace hsba {
+
+namespace {
+
+uint32_t enabled_bits(uint32_t comp) {
+  uint32_t b = 0;
+  if (comp & HSBA_COMPUTE_RADIAL_DISTORTION) b |= 0x7u;
+  if (comp & HSBA_COMPUTE_DECENTERING_DISTORTION) b |= 0x18u;
+  if (comp & HSBA_COMPUTE_INTRINSIC_PARAMS) b |= 0x3E0u;
+  return b;
+}
+
+double identity_value(int j) { return (j == IF || j == IPR) ? 1.0 : 0.0; }
+
+int fail(std::string *why, const std::string &msg, int code = HSBA_ERR_INVALID) {
+  if (why) *why = msg;
+  return code;
+}
+
+constexpr int kChunk = 8192;
+
+void add_chunks(Layout &L, int kind, int64_t begin, int64_t end, int w) {
+  for (int64_t b = begin; b < end; b += kChunk) {
+    L.chunk_kind.push_back(kind);
+    L.chunk_begin.push_back((int32_t)b);
+    L.chunk_end.push_back((int32_t)std::min<int64_t>(end, b + kChunk));
+    L.chunk_win.push_back(w);
+  }
+}
+
+}  // namespace
+
+int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::string *why) {
+  if (!descs || count <= 0) return fail(why, "no problems");
+  L = Layout();
+  L.nwin = (int)count;
+  const hsba_problem_desc &d0 = descs[0];
+  L.comp = d0.intrinsic_computations_mask & 7u;
+  L.fix = d0.fix_mask & 7u;
+  if (L.fix == 7u) return fail(why, "every block fixed");
+  const uint32_t en = enabled_bits(L.comp);
+  L.free_bits = d0.intrinsic_free_mask ? d0.intrinsic_free_mask : en;
+  if (L.free_bits & ~en) return fail(why, "free mask outside the enabled groups");
+  int d = 0;
+  for (int j = 0; j < kFull; ++j)
+    if (L.free_bits & (1u << j)) L.fidx[d++] = j;
+  for (int j = d; j < kFull; ++j) L.fidx[j] = 0;
+  L.pts_free = !(L.fix & HSBA_FIX_POINTS);
+  L.imgs_free = !(L.fix & HSBA_FIX_IMAGES);
+  L.cams_free = !(L.fix & HSBA_FIX_CAMERAS);
+  L.IMG = L.imgs_free ? 6 : 0;
+  L.D = L.cams_free ? d : 0;
+  L.E = L.IMG + L.D;
+  const int D = L.D, IMG = L.IMG;
+
+  // ---- totals and per-window validation
+  L.hw.resize(count);
+  L.wi.resize(count);
+  for (int64_t w = 0; w < count; ++w) {
+    const hsba_problem_desc &ds = descs[w];
+    if ((ds.intrinsic_computations_mask & 7u) != L.comp || (ds.fix_mask & 7u) != L.fix ||
+        (ds.intrinsic_free_mask ? ds.intrinsic_free_mask : enabled_bits(ds.intrinsic_computations_mask)) !=
+            L.free_bits)
+      return fail(why, "windows of one batch must share masks", HSBA_ERR_UNSUPPORTED);
+    if (ds.num_points < 0 || ds.num_images < 0 || ds.num_cameras < 0 || ds.num_keys < 0)
+      return fail(why, "negative size");
+    if (ds.num_keys > 0 && !ds.feature_maps) return fail(why, "feature_maps missing");
+    if (ds.num_images > 0 && !ds.image_camera_map) return fail(why, "image_camera_map missing");
+    HostWindow &h = L.hw[w];
+    h.P = ds.num_points; h.I = ds.num_images; h.C = ds.num_cameras; h.N = ds.num_keys;
+    h.x_pt = 0;
+    h.x_img = L.pts_free ? 3 * h.P : 0;
+    h.x_cam = h.x_img + (L.imgs_free ? 6 * h.I : 0);
+    h.x_size = h.x_cam + (L.cams_free ? (int64_t)d * h.C : 0);
+    if (!L.pts_free && !ds.fix_points && h.P) return fail(why, "fix_points missing");
+    if (!L.imgs_free && !ds.fix_images && h.I) return fail(why, "fix_images missing");
+    if (!L.cams_free && !ds.fix_cameras && h.C && d) return fail(why, "fix_cameras missing");
+    for (int64_t i = 0; i < h.I; ++i)
+      if (ds.image_camera_map[i] < 0 || ds.image_camera_map[i] >= h.C) return fail(why, "image_camera_map out of range");
+    for (int64_t k = 0; k < h.N; ++k) {
+      const int32_t im = ds.feature_maps[2 * k], pt = ds.feature_maps[2 * k + 1];
+      if (im < 0 || im >= h.I || pt < 0 || pt >= h.P) return fail(why, "feature_maps out of range");
+    }
+    L.P += h.P; L.I += h.I; L.C += h.C; L.N += h.N;
+  }
+  if (L.N >= (1LL << 25) && L.pts_free && L.E > 0)
+    return fail(why, "more than 2^25 keys in one dense-Schur problem", HSBA_ERR_UNSUPPORTED);
+  if (L.P >= (1LL << 31) || L.N >= (1LL << 31)) return fail(why, "too large", HSBA_ERR_UNSUPPORTED);
+
+  L.key_img.resize(L.N); L.key_pt.resize(L.N); L.key_orig.resize(L.N);
+  L.pt_ptr.assign(L.P + 1, 0); L.pt_win.resize(L.P);
+  L.img_ptr.assign(L.I + 1, 0); L.img_keys.resize(L.N); L.img_win.resize(L.I); L.img_cam.resize(L.I);
+  L.img_row.assign(L.I, -1);
+  L.cam_ptr.assign(L.C + 1, 0); L.cam_imgs.resize(L.I); L.cam_win.resize(L.C); L.cam_row.assign(L.C, -1);
+  L.cam_base.assign(L.C * kFull, 0.0);
+  if (!L.pts_free) L.fix_pts.resize(3 * L.P);
+  if (!L.imgs_free) L.fix_imgs.resize(6 * L.I);
+  if (!L.cams_free) L.fix_cam_free.resize((int64_t)d * L.C);
+  L.blk_ptr.assign(1, 0);
+
+  int64_t pt_off = 0, img_off = 0, cam_off = 0, key_off = 0;
+  int64_t r_off = 0, s_off = 0, dinv_off = 0, lo_off = 0, blk_off = 0;
+  std::vector<int64_t> pcount;
+  for (int64_t w = 0; w < count; ++w) {
+    const hsba_problem_desc &ds = descs[w];
+    HostWindow &h = L.hw[w];
+    WinInfo &wi = L.wi[w];
+    std::memset(&wi, 0, sizeof(wi));
+    wi.key_off = key_off; wi.pt_off = (int32_t)pt_off; wi.img_off = (int32_t)img_off; wi.cam_off = (int32_t)cam_off;
+    wi.num_pts = (int32_t)h.P; wi.num_imgs = (int32_t)h.I; wi.num_cams = (int32_t)h.C; wi.num_keys = (int32_t)h.N;
+    // ---- parameters held fixed and camera base vectors
+    for (int64_t c = 0; c < h.C; ++c)
+      for (int j = 0; j < kFull; ++j) {
+        double v = identity_value(j);
+        if ((en >> j) & 1u) v = ds.intrinsic_defaults ? ds.intrinsic_defaults[c * kFull + j] : identity_value(j);
+        L.cam_base[(cam_off + c) * kFull + j] = v;
+      }
+    if (!L.pts_free) std::copy(ds.fix_points, ds.fix_points + 3 * h.P, L.fix_pts.begin() + 3 * pt_off);
+    if (!L.imgs_free) std::copy(ds.fix_images, ds.fix_images + 6 * h.I, L.fix_imgs.begin() + 6 * img_off);
+    if (!L.cams_free && d)
+      std::copy(ds.fix_cameras, ds.fix_cameras + (int64_t)d * h.C, L.fix_cam_free.begin() + (int64_t)d * cam_off);
+    // ---- point-major key order: counting sort by point, images ascending inside
+    pcount.assign(h.P + 1, 0);
+    for (int64_t k = 0; k < h.N; ++k) pcount[ds.feature_maps[2 * k + 1] + 1]++;
+    for (int64_t p = 0; p < h.P; ++p) pcount[p + 1] += pcount[p];
+    {
+      std::vector<int64_t> fill(pcount.begin(), pcount.end() - 1);
+      for (int64_t k = 0; k < h.N; ++k) {
+        const int64_t slot = key_off + fill[ds.feature_maps[2 * k + 1]]++;
+        L.key_orig[slot] = (int32_t)(key_off + k);
+        L.key_img[slot] = (int32_t)(img_off + ds.feature_maps[2 * k]);
+        L.key_pt[slot] = (int32_t)(pt_off + ds.feature_maps[2 * k + 1]);
+      }
+    }
+    int max_track = 0;
+    for (int64_t p = 0; p < h.P; ++p) {
+      const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
+      max_track = std::max<int>(max_track, (int)(e - b));
+      // insertion sort of the point's keys by image (stable in key order)
+      for (int64_t a = b + 1; a < e; ++a) {
+        int32_t im = L.key_img[a], ko = L.key_orig[a];
+        int64_t c = a - 1;
+        while (c >= b && L.key_img[c] > im) {
+          L.key_img[c + 1] = L.key_img[c]; L.key_orig[c + 1] = L.key_orig[c]; --c;
+        }
+        L.key_img[c + 1] = im; L.key_orig[c + 1] = ko;
+      }
+      for (int64_t a = b + 1; a < e; ++a)
+        if (L.key_img[a] == L.key_img[a - 1]) return fail(why, "duplicate (image, point) key");
+      L.pt_ptr[pt_off + p + 1] = (int32_t)e;
+      L.pt_win[pt_off + p] = (int32_t)w;
+    }
+    if (h.P == 0) L.pt_ptr[pt_off] = (int32_t)key_off;
+    if (L.pts_free && L.E > 0 && max_track > 127)
+      return fail(why, "a point seen by more than 127 images", HSBA_ERR_UNSUPPORTED);
+    // ---- images and cameras
+    for (int64_t i = 0; i < h.I; ++i) {
+      L.img_win[img_off + i] = (int32_t)w;
+      L.img_cam[img_off + i] = (int32_t)(cam_off + ds.image_camera_map[i]);
+    }
+    for (int64_t c = 0; c < h.C; ++c) L.cam_win[cam_off + c] = (int32_t)w;
+    // ---- reduced layout
+    wi.cb = (int32_t)(IMG ? 6 * h.I : 0);
+    wi.n = (int32_t)(wi.cb + (D ? D * h.C : 0));
+    wi.ld = wi.n > 0 ? ((wi.n + 63) / 64) * 64 : 0;
+    wi.nb = wi.ld / 64;
+    wi.r_off = (int32_t)r_off;
+    wi.s_off = s_off;
+    wi.dinv_off = dinv_off;
+    wi.lo_off = lo_off;
+    wi.blk_off = blk_off;
+    wi.small = wi.nb <= 4 ? 1 : 0;
+    if (wi.n > 0) (wi.small ? L.small_list : L.large_list).push_back((int)w);
+    for (int64_t i = 0; i < h.I && IMG; ++i) L.img_row[img_off + i] = (int32_t)(r_off + 6 * i);
+    for (int64_t c = 0; c < h.C && D; ++c) L.cam_row[cam_off + c] = (int32_t)(r_off + wi.cb + D * c);
+    const int64_t nblk = (IMG ? h.I : 0) + (D ? h.C : 0);
+    auto bimg = [&](int64_t i) { return i; };
+    auto bcam = [&](int64_t c) { return (IMG ? h.I : 0) + c; };
+    L.row_blk.resize(r_off + wi.ld, 0);
+    L.row_sub.resize(r_off + wi.ld, 0);
+    for (int64_t R = 0; R < wi.n; ++R) {
+      if (R < wi.cb) { L.row_blk[r_off + R] = (int32_t)(R / 6); L.row_sub[r_off + R] = (int32_t)(R % 6); }
+      else {
+        L.row_blk[r_off + R] = (int32_t)((IMG ? h.I : 0) + (R - wi.cb) / D);
+        L.row_sub[r_off + R] = (int32_t)((R - wi.cb) % D);
+      }
+    }
+    // observed images / cameras ("exist" rule of the reference Schur loops)
+    std::vector<int64_t> img_n(h.I, 0), cam_n(h.C, 0);
+    for (int64_t k = 0; k < h.N; ++k) {
+      img_n[ds.feature_maps[2 * k]]++;
+      cam_n[ds.image_camera_map[ds.feature_maps[2 * k]]]++;
+    }
+    if (L.pts_free) {
+      for (int64_t i = 0; i < h.I && IMG; ++i) if (!img_n[i]) h.unobserved = true;
+      for (int64_t c = 0; c < h.C && D; ++c) if (!cam_n[c]) h.unobserved = true;
+    }
+    // ---- camera pairs (dense pair table of the window)
+    std::vector<std::vector<Contrib>> blocks((size_t)(nblk * (nblk + 1) / 2));
+    auto add = [&](int64_t bx, int64_t by, Contrib c) {
+      blocks[(size_t)(bx * (bx + 1) / 2 + by)].push_back(c);
+    };
+    for (int64_t i = 0; i < h.I && IMG; ++i) add(bimg(i), bimg(i), Contrib{0, (int32_t)(img_off + i), 0, 0, 0});
+    for (int64_t c = 0; c < h.C && D; ++c) add(bcam(c), bcam(c), Contrib{1, (int32_t)(cam_off + c), 0, 0, 0});
+    if (IMG && D)
+      for (int64_t i = 0; i < h.I; ++i)
+        add(bcam(ds.image_camera_map[i]), bimg(i), Contrib{2, (int32_t)(img_off + i), 0, 0, 0});
+    if (L.pts_free && L.E > 0 && h.I > 0) {
+      std::vector<int32_t> pid((size_t)(h.I * h.I), -1);
+      std::vector<int64_t> cnt((size_t)(h.I * h.I), 0);
+      for (int64_t p = 0; p < h.P; ++p) {
+        const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
+        for (int64_t u = b; u < e; ++u)
+          for (int64_t v = u; v < e; ++v)
+            cnt[(size_t)((L.key_img[u] - img_off) * h.I + (L.key_img[v] - img_off))]++;
+      }
+      const int64_t first_pair = L.npairs;
+      std::vector<int64_t> cursor;
+      for (int64_t ia = 0; ia < h.I; ++ia)
+        for (int64_t ib = ia; ib < h.I; ++ib) {
+          const int64_t c = cnt[(size_t)(ia * h.I + ib)];
+          if (!c) continue;
+          pid[(size_t)(ia * h.I + ib)] = (int32_t)L.npairs++;
+          L.pair_win.push_back((int32_t)w);
+          L.pair_img.push_back(ia == ib ? (int32_t)(img_off + ia) : -1);
+          cursor.push_back(L.incidences);
+          L.pair_ptr.push_back(L.incidences);
+          L.incidences += c;
+          // contributions of this pair block
+          struct Side { int64_t blk; int16_t off; };
+          Side xs[2], ys[2];
+          int nx = 0, ny = 0;
+          if (IMG) { xs[nx++] = {bimg(ia), 0}; ys[ny++] = {bimg(ib), 0}; }
+          if (D) {
+            xs[nx++] = {bcam(ds.image_camera_map[ia]), (int16_t)IMG};
+            ys[ny++] = {bcam(ds.image_camera_map[ib]), (int16_t)IMG};
+          }
+          const int32_t pg = (int32_t)(L.npairs - 1);
+          for (int a = 0; a < nx; ++a)
+            for (int b2 = 0; b2 < ny; ++b2) {
+              const Side X = xs[a], Y = ys[b2];
+              if (X.blk >= Y.blk) add(X.blk, Y.blk, Contrib{3, pg, X.off, Y.off, 0});
+              if (ia != ib && Y.blk >= X.blk) add(Y.blk, X.blk, Contrib{3, pg, Y.off, X.off, 1});
+            }
+        }
+      L.inc.resize(L.incidences);
+      for (int64_t p = 0; p < h.P; ++p) {
+        const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
+        for (int64_t u = b; u < e; ++u)
+          for (int64_t v = u; v < e; ++v) {
+            const int32_t q = pid[(size_t)((L.key_img[u] - img_off) * h.I + (L.key_img[v] - img_off))];
+            L.inc[cursor[q - first_pair]++] = (uint32_t)((u << 7) | (v - u));
+          }
+      }
+    }
+    // flatten contribution lists
+    for (auto &bl : blocks) {
+      for (const Contrib &c : bl) L.contrib.push_back(c);
+      L.blk_ptr.push_back((int64_t)L.contrib.size());
+    }
+    // ---- constraints in y order (camera_shared_vector_function.hpp:485-629)
+    auto push = [&](int kind, int64_t ent_local, int param, int64_t xidx) {
+      ConstraintRow cr;
+      cr.kind = kind; cr.param = param; cr.win = (int32_t)w; cr.weight = 1.0; cr.y = 0.0;
+      cr.entity = (int32_t)(ent_local + (kind == 0 ? pt_off : (kind == 1 ? img_off : cam_off)));
+      L.cons.push_back(cr);
+      h.cons_xidx.push_back(xidx);
+    };
+    wi.con_begin = (int32_t)L.cons.size();
+    if (L.pts_free)
+      for (int64_t q = 0; q < ds.num_point_constraints; ++q) {
+        const hsba_constraint &c = ds.point_constraints[q];
+        if (c.id < 0 || c.id >= h.P) return fail(why, "point constraint id");
+        for (int b = 0; b < 3; ++b) if (c.mask & (1u << b)) push(0, c.id, b, h.x_pt + 3 * c.id + b);
+      }
+    if (L.imgs_free)
+      for (int64_t q = 0; q < ds.num_image_constraints; ++q) {
+        const hsba_constraint &c = ds.image_constraints[q];
+        if (c.id < 0 || c.id >= h.I) return fail(why, "image constraint id");
+        if (c.mask & 1u) for (int b = 0; b < 3; ++b) push(1, c.id, b, h.x_img + 6 * c.id + b);
+        for (int b = 1; b < 4; ++b) if (c.mask & (1u << b)) push(1, c.id, 2 + b, h.x_img + 6 * c.id + 2 + b);
+      }
+    if (L.cams_free)
+      for (int64_t q = 0; q < ds.num_camera_constraints; ++q) {
+        const hsba_constraint &c = ds.camera_constraints[q];
+        if (c.id < 0 || c.id >= h.C) return fail(why, "camera constraint id");
+        if ((c.mask & 0x3FFu) & ~en) return fail(why, "camera constraint on a disabled group");
+        for (int j = 0; j < kFull; ++j) {
+          if (!(c.mask & (1u << j))) continue;
+          int local = -1;
+          for (int q2 = 0; q2 < d; ++q2) if (L.fidx[q2] == j) local = q2;
+          if (local < 0) return fail(why, "camera constraint on a held parameter");
+          push(2, c.id, local, h.x_cam + (int64_t)d * c.id + local);
+        }
+      }
+    wi.con_end = (int32_t)L.cons.size();
+    h.y_size = 2 * h.N + (int64_t)h.cons_xidx.size();
+    // ---- reduction chunks (sorted by window)
+    if (L.pts_free) add_chunks(L, 0, 3 * pt_off, 3 * (pt_off + h.P), (int)w);
+    if (L.imgs_free) add_chunks(L, 1, 6 * img_off, 6 * (img_off + h.I), (int)w);
+    if (L.cams_free && D) add_chunks(L, 2, (int64_t)D * cam_off, (int64_t)D * (cam_off + h.C), (int)w);
+    if (h.N > 0) add_chunks(L, 3, key_off, key_off + h.N, (int)w);
+    else { L.chunk_kind.push_back(3); L.chunk_begin.push_back((int32_t)key_off); L.chunk_end.push_back((int32_t)key_off); L.chunk_win.push_back((int)w); }
+    // ---- advance offsets
+    pt_off += h.P; img_off += h.I; cam_off += h.C; key_off += h.N;
+    r_off += wi.ld;
+    s_off += (int64_t)wi.ld * wi.ld;
+    dinv_off += (int64_t)wi.nb * 64 * 64;
+    lo_off += (int64_t)wi.n * (wi.n + 1) / 2;
+    blk_off += nblk * (nblk + 1) / 2;
+  }
+  L.pt_ptr[0] = 0;
+  // image CSR over point-major keys (each image's keys in point order)
+  for (int64_t k = 0; k < L.N; ++k) L.img_ptr[L.key_img[k] + 1]++;
+  for (int64_t i = 0; i < L.I; ++i) L.img_ptr[i + 1] += L.img_ptr[i];
+  {
+    std::vector<int32_t> fill(L.img_ptr.begin(), L.img_ptr.end() - 1);
+    for (int64_t k = 0; k < L.N; ++k) L.img_keys[fill[L.key_img[k]]++] = (int32_t)k;
+  }
+  for (int64_t i = 0; i < L.I; ++i) L.cam_ptr[L.img_cam[i] + 1]++;
+  for (int64_t c = 0; c < L.C; ++c) L.cam_ptr[c + 1] += L.cam_ptr[c];
+  {
+    std::vector<int32_t> fill(L.cam_ptr.begin(), L.cam_ptr.end() - 1);
+    for (int64_t i = 0; i < L.I; ++i) L.cam_imgs[fill[L.img_cam[i]]++] = (int32_t)i;
+  }
+  L.pair_ptr.push_back(L.incidences);
+  L.S_total = s_off; L.rows_total = r_off; L.dinv_total = dinv_off; L.lower_total = lo_off;
+  if (L.row_blk.empty()) { L.row_blk.push_back(0); L.row_sub.push_back(0); }
+  return HSBA_OK;
+}
+
+int64_t layout_device_bytes(const Layout &L, int precision) {
+  const int64_t t = precision == HSBA_PRECISION_FP64 ? 8 : 4;
+  const int64_t E = L.E, D = L.D;
+  int64_t b = 0;
+  b += L.N * (4 * 3 + 16) + (L.has_cov ? L.N * 24 : 0);           // key arrays
+  b += L.N * 2 * 3 * E * t;                                         // W, Z
+  b += L.P * (8 + 3 * 8 * 3 + t * 18);                              // points
+  b += L.I * (4 * 5 + 6 * 8 * 2 + t * (E * E + 2 * E));             // images
+  b += L.C * (4 * 3 + 8 * (2 * D + 20) + t * (D * D + D));           // cameras
+  b += L.npairs * (8 + 8 + t * E * E) + L.incidences * 4;           // pairs
+  b += (L.S_total + L.dinv_total) * 8 + L.rows_total * (8 * 8 + 8); // reduced systems
+  b += (int64_t)L.contrib.size() * sizeof(Contrib) + (int64_t)L.blk_ptr.size() * 8;
+  b += (int64_t)L.chunk_kind.size() * (16 + 32) + (int64_t)L.nwin * (sizeof(WinInfo) + sizeof(WinState));
+  return b;
+}
+
+}  // namespace hsba

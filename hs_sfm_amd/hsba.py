@@ -1,0 +1,236 @@
+"""Python host mirror of the reference's bundle-adjustment interface over the
+C ABI of libhsba.so (include/hsba.h).
+
+Class names follow hs_sfm's camera_shared family so tests read like the
+reference's own:
+
+  CameraSharedVectorFunction             operator()(x) -> y
+      (include/hs_sfm/bundle_adjustment/camera_shared_vector_function.hpp:412-632)
+  CameraSharedAnalyticalJacobianMatrixCalculator   (…_analytical_jacobian_matrix_calculator.hpp:66-332)
+  CameraSharedNormalEquationBuilder      (…_normal_equation_builder.hpp:92-462)
+  CameraSharedNormalEquationSolver       (…_normal_equation_solver.hpp:75-114)
+  CameraSharedLevenbergMarquardtOptimizor(…_levenberg_marquardt_optimizor.hpp:19-63)
+
+Errors: the reference returns `Err` 0 / -1; these wrappers return the
+library's code (0 or negative) and raise only on misuse.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import abi
+from .synthetic import Scene
+
+_dp = C.POINTER(C.c_double)
+
+
+def _opts(precision=abi.PRECISION_MIXED, solver=abi.SOLVER_AUTO, pcg_rtol=1e-6,
+          pcg_max_iterations=500, point_group_lanes=0):
+    return abi.hsba_options(precision=precision, solver=solver, pcg_rtol=pcg_rtol,
+                            pcg_max_iterations=pcg_max_iterations,
+                            point_group_lanes=point_group_lanes)
+
+
+class Problem:
+    """Device-resident problem (hsba_problem) built from a Scene."""
+
+    def __init__(self, scene: Scene, precision=abi.PRECISION_MIXED, solver=abi.SOLVER_AUTO,
+                 pcg_rtol=1e-6, point_group_lanes=0):
+        self.lib = abi.load_library()
+        self.scene = scene
+        self.desc = abi.Desc(scene)
+        self.h = C.c_void_p()
+        o = _opts(precision, solver, pcg_rtol, point_group_lanes=point_group_lanes)
+        rc = self.lib.hsba_problem_create(C.byref(self.desc.desc), C.byref(o), C.byref(self.h))
+        abi.check(rc, "hsba_problem_create")
+        xs, ys, dd, nr = (C.c_int64() for _ in range(4))
+        self.lib.hsba_problem_sizes(self.h, C.byref(xs), C.byref(ys), C.byref(dd), C.byref(nr))
+        self.x_size, self.y_size, self.cam_d, self.n_reduced = xs.value, ys.value, dd.value, nr.value
+
+    def close(self):
+        if self.h:
+            self.lib.hsba_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _d(a):
+        return abi.dptr(np.ascontiguousarray(a, np.float64))
+
+    def evaluate(self, x):
+        y = np.zeros(self.y_size)
+        abi.check(self.lib.hsba_evaluate(self.h, self._d(x), abi.dptr(y)), "hsba_evaluate")
+        return y
+
+    def jacobian(self, x):
+        N = self.scene.num_keys
+        Jp, Ji = np.zeros((N, 2, 3)), np.zeros((N, 2, 6))
+        Jc = np.zeros((N, 2, max(1, self.cam_d)))
+        abi.check(self.lib.hsba_jacobian(self.h, self._d(x), abi.dptr(Jp), abi.dptr(Ji),
+                                         abi.dptr(Jc)), "hsba_jacobian")
+        return Jp, Ji, Jc[:, :, :self.cam_d]
+
+    def build_normal(self, x, y_obs=None):
+        s = self.scene
+        y_obs = s.y_obs if y_obs is None else y_obs
+        d = self.cam_d
+        V = np.zeros((s.num_points, 3, 3))
+        Ui = np.zeros((s.num_images, 6, 6))
+        Uc = np.zeros((s.num_cameras, max(d, 1), max(d, 1)))
+        g = np.zeros(self.x_size)
+        rc = self.lib.hsba_build_normal(self.h, self._d(x), self._d(y_obs), C.byref(self.desc.ycov),
+                                        abi.dptr(V), abi.dptr(Ui), abi.dptr(Uc), abi.dptr(g))
+        abi.check(rc, "hsba_build_normal")
+        return dict(V=V, Ui=Ui, Uc=Uc[:, :d, :d] if d else Uc[:, :0, :0], g=g)
+
+    def solve(self, mu):
+        delta = np.zeros(self.x_size)
+        rc = self.lib.hsba_solve(self.h, mu, abi.dptr(delta))
+        return rc, delta
+
+    def reduced_system(self, mu):
+        n = self.n_reduced
+        S, rhs = np.zeros((n, n)), np.zeros(n)
+        rc = self.lib.hsba_reduced_system(self.h, mu, abi.dptr(S), abi.dptr(rhs))
+        return rc, S, rhs
+
+    def max_diagonal(self):
+        v = C.c_double()
+        abi.check(self.lib.hsba_max_diagonal(self.h, C.byref(v)), "hsba_max_diagonal")
+        return v.value
+
+    def gradient_inf_norm(self):
+        v = C.c_double()
+        abi.check(self.lib.hsba_gradient_inf_norm(self.h, C.byref(v)), "hsba_gradient_inf_norm")
+        return v.value
+
+    def gradient_dot(self, delta):
+        v = C.c_double()
+        abi.check(self.lib.hsba_gradient_dot(self.h, self._d(delta), C.byref(v)), "hsba_gradient_dot")
+        return v.value
+
+    def mahalanobis(self, x, y_obs=None):
+        y_obs = self.scene.y_obs if y_obs is None else y_obs
+        v = C.c_double()
+        abi.check(self.lib.hsba_mahalanobis(self.h, self._d(x), self._d(y_obs),
+                                            C.byref(self.desc.ycov), C.byref(v)), "hsba_mahalanobis")
+        return v.value
+
+    def lm(self, x0=None, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0):
+        s = self.scene
+        x0 = s.x0 if x0 is None else x0
+        opt = abi.hsba_lm_options(max_iterations=max_iterations, tau=tau, eps1=eps1, eps2=eps2)
+        x = np.zeros(self.x_size)
+        st = abi.hsba_lm_stats()
+        rc = self.lib.hsba_lm_solve(self.h, self._d(x0), self._d(s.y_obs), C.byref(self.desc.ycov),
+                                    C.byref(opt), abi.dptr(x), C.byref(st))
+        return rc, x, st
+
+
+    # stepwise LM (timing / serving)
+    def lm_begin(self, x0=None, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0):
+        s = self.scene
+        self._x0 = np.ascontiguousarray(s.x0 if x0 is None else x0, np.float64)
+        self._y = np.ascontiguousarray(s.y_obs, np.float64)
+        opt = abi.hsba_lm_options(max_iterations=max_iterations, tau=tau, eps1=eps1, eps2=eps2)
+        return self.lib.hsba_lm_begin(self.h, abi.dptr(self._x0), abi.dptr(self._y),
+                                      C.byref(self.desc.ycov), C.byref(opt))
+
+    def lm_iterate(self, iterations, timed=False):
+        t = abi.hsba_stage_times() if timed else None
+        rc = self.lib.hsba_lm_iterate(self.h, iterations, C.byref(t) if timed else None)
+        return rc, (t.as_dict() if timed else None)
+
+    def lm_end(self):
+        x = np.zeros(self.x_size)
+        st = abi.hsba_lm_stats()
+        rc = self.lib.hsba_lm_end(self.h, abi.dptr(x), C.byref(st))
+        return rc, x, st
+
+
+class Batch:
+    """Many independent windows solved by one set of launches per LM iteration."""
+
+    def __init__(self, scenes: Sequence[Scene], precision=abi.PRECISION_MIXED, point_group_lanes=0):
+        self.lib = abi.load_library()
+        self.scenes = list(scenes)
+        self.descs = [abi.Desc(s) for s in self.scenes]
+        n = len(self.scenes)
+        self.desc_arr = (abi.hsba_problem_desc * n)(*[d.desc for d in self.descs])
+        self.ycov_arr = (abi.hsba_ycov_inverse * n)(*[d.ycov for d in self.descs])
+        self.h = C.c_void_p()
+        o = _opts(precision, point_group_lanes=point_group_lanes)
+        rc = self.lib.hsba_batch_create(self.desc_arr, n, C.byref(o), C.byref(self.h))
+        abi.check(rc, "hsba_batch_create")
+
+    def close(self):
+        if self.h:
+            self.lib.hsba_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def lm(self, x0s=None, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0):
+        n = len(self.scenes)
+        x0s = [s.x0 for s in self.scenes] if x0s is None else x0s
+        self._x0 = [np.ascontiguousarray(x, np.float64) for x in x0s]
+        self._y = [np.ascontiguousarray(s.y_obs, np.float64) for s in self.scenes]
+        xs = [np.zeros(s.x_size) for s in self.scenes]
+        X0 = (_dp * n)(*[abi.dptr(a) for a in self._x0])
+        Y = (_dp * n)(*[abi.dptr(a) for a in self._y])
+        XO = (_dp * n)(*[abi.dptr(a) for a in xs])
+        st = (abi.hsba_lm_stats * n)()
+        opt = abi.hsba_lm_options(max_iterations=max_iterations, tau=tau, eps1=eps1, eps2=eps2)
+        rc = self.lib.hsba_batch_lm_solve(self.h, X0, Y, self.ycov_arr, C.byref(opt), XO, st)
+        return rc, xs, list(st)
+
+
+# Reference-named wrappers ----------------------------------------------------
+class CameraSharedVectorFunction:
+    def __init__(self, problem: Problem):
+        self.problem = problem
+
+    def __call__(self, x):
+        return 0, self.problem.evaluate(x)
+
+
+class CameraSharedAnalyticalJacobianMatrixCalculator:
+    def __call__(self, vector_function: CameraSharedVectorFunction, x):
+        return 0, vector_function.problem.jacobian(x)
+
+
+class CameraSharedNormalEquationBuilder:
+    def __call__(self, problem: Problem, x, y_obs):
+        return 0, problem.build_normal(x, y_obs)
+
+
+class CameraSharedNormalEquationSolver:
+    def __call__(self, problem: Problem, mu):
+        return problem.solve(mu)
+
+
+class CameraSharedLevenbergMarquardtOptimizor:
+    """ctor(x0, max_itr, tau, eps1, eps2); operator()(problem) -> (err, x)."""
+
+    def __init__(self, initial_x, max_itr_num=20, tau=1e-3, eps1=0.0, eps2=0.0):
+        self.x0 = np.asarray(initial_x, np.float64)
+        self.max_itr, self.tau, self.eps1, self.eps2 = max_itr_num, tau, eps1, eps2
+
+    def __call__(self, problem: Problem):
+        if self.x0.shape[0] != problem.x_size:
+            return -1, None
+        rc, x, st = problem.lm(self.x0, self.max_itr, self.tau, self.eps1, self.eps2)
+        self.stats = st
+        return rc, x

@@ -1,0 +1,310 @@
+/*
+ * hsba.h — C ABI of the MI355X Levenberg–Marquardt bundle adjuster.
+ *
+ * Drop-in boundary for the camera_shared bundle-adjustment path of hs_sfm
+ * (include/hs_sfm/bundle_adjustment/camera_shared_*.hpp).  Every entry point
+ * takes plain pointers and sizes in the reference's own parameter layouts:
+ *
+ *   x = [ points 3P | images 6I | cameras d*C ]   (a block is absent when fixed)
+ *       camera_shared_vector_function.hpp:349-354, 364-410
+ *   y = [ keys 2N | point constraints | image constraints | camera constraints ]
+ *       camera_shared_vector_function.hpp:356-362, 483-629
+ *   feature_maps[k] = (image_id, point_id)      camera_shared_vector_function.hpp:36, 433-435
+ *   image_camera_map[image] = camera group id   camera_shared_vector_function.hpp:39, 435
+ *
+ * Residuals follow the reference sign r = f(x) - y
+ * (unit_test/bundle_adjustment/test_camera_shared_normal_equation_builder.hpp:463),
+ * the step solves (N + mu I) delta = g with g = J^T Sigma^-1 r and the update
+ * is x_new = x - delta.
+ *
+ * Return codes: 0 = ok, negative = failure.  The reference uses
+ * `typedef int Err` with 0 / -1 (camera_shared_normal_equation_solver.hpp:455-464);
+ * callers that want exactly that map every nonzero code to -1.
+ *
+ * Device state lives behind the opaque handles; no device pointer crosses
+ * this boundary.  A handle is used by one host thread at a time.
+ */
+#ifndef HSBA_H_
+#define HSBA_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes ------------------------------------------------------ */
+#define HSBA_OK                 0
+#define HSBA_ERR_GENERIC       -1  /* reference's only failure value       */
+#define HSBA_ERR_INVALID       -2  /* bad sizes / ids / masks              */
+#define HSBA_ERR_NOT_SPD       -3  /* factorization met a non-positive pivot */
+#define HSBA_ERR_NAN           -4  /* non-finite value in the iteration    */
+#define HSBA_ERR_OOM           -5  /* device allocation failed             */
+#define HSBA_ERR_DEVICE        -6  /* HIP runtime error                    */
+#define HSBA_ERR_ADMISSION     -7  /* job larger than the server's HBM budget */
+#define HSBA_ERR_UNSUPPORTED   -8  /* option combination not implemented   */
+#define HSBA_ERR_NOT_READY     -9  /* server job still running             */
+
+/* ---- masks (bit i == reference enum value i) ---------------------------- */
+/* IntrinsicComputationsMask, camera_shared_common_types.hpp:14-22 */
+#define HSBA_COMPUTE_RADIAL_DISTORTION       (1u << 0)
+#define HSBA_COMPUTE_DECENTERING_DISTORTION  (1u << 1)
+#define HSBA_COMPUTE_INTRINSIC_PARAMS        (1u << 2)
+/* FixMask, camera_shared_common_types.hpp:97-104 */
+#define HSBA_FIX_POINTS   (1u << 0)
+#define HSBA_FIX_IMAGES   (1u << 1)
+#define HSBA_FIX_CAMERAS  (1u << 2)
+
+/* Full per-camera intrinsic vector, in the reference's camera-block column
+ * order [k1 k2 k3 | d1 d2 | f skew px py pr]
+ * (camera_shared_analytical_jacobian_matrix_calculator.hpp:229-271). */
+enum {
+  HSBA_K1 = 0, HSBA_K2, HSBA_K3, HSBA_D1, HSBA_D2,
+  HSBA_FOCAL, HSBA_SKEW, HSBA_PRINCIPAL_X, HSBA_PRINCIPAL_Y, HSBA_PIXEL_RATIO,
+  HSBA_NUM_INTRINSICS
+};
+
+/* Constraint records (camera_shared_common_types.hpp:24-95).
+ *   point : mask bits 0..2 = POINT_CONSTRAIN_{X,Y,Z}
+ *   image : mask bits 0..3 = IMAGE_CONSTRAIN_{ROTATION,POSITION_X,_Y,_Z}
+ *   camera: mask bits 0..2 = radial {K1,K2,K3}, bits 3..4 = decentering
+ *           {D1,D2}, bits 5..9 = intrinsic {F,SKEW,PX,PY,PR}
+ */
+typedef struct {
+  int64_t  id;
+  uint32_t mask;
+  uint32_t reserved_;
+} hsba_constraint;
+
+/* Problem description: the state of CameraSharedVectorFunction
+ * (camera_shared_vector_function.hpp:20-410). */
+typedef struct {
+  int64_t num_points;
+  int64_t num_images;
+  int64_t num_cameras;
+  int64_t num_keys;
+  const int32_t *feature_maps;      /* [num_keys][2] = (image_id, point_id) */
+  const int32_t *image_camera_map;  /* [num_images] */
+  uint32_t intrinsic_computations_mask;
+  /* Extension with no reference counterpart: which parameters of the enabled
+   * groups are free, as bits over the HSBA_K1..HSBA_PIXEL_RATIO order.  0
+   * means "all parameters of the enabled groups", which is exactly the
+   * reference layout.  Held parameters take their value from
+   * intrinsic_defaults.  Used for the 9-parameter (r, t, f, k1, k2) camera. */
+  uint32_t intrinsic_free_mask;
+  const double *intrinsic_defaults; /* [num_cameras][10] or NULL (zeros, f=pr=1) */
+  uint32_t fix_mask;
+  uint32_t reserved_;
+  const double *fix_points;         /* [P][3] when HSBA_FIX_POINTS  */
+  const double *fix_images;         /* [I][6] when HSBA_FIX_IMAGES  */
+  const double *fix_cameras;        /* [C][d] when HSBA_FIX_CAMERAS, d = free-parameter count */
+  int64_t num_point_constraints;
+  const hsba_constraint *point_constraints;
+  int64_t num_image_constraints;
+  const hsba_constraint *image_constraints;
+  int64_t num_camera_constraints;
+  const hsba_constraint *camera_constraints;
+} hsba_problem_desc;
+
+/* Inverse observation covariance: CameraSharedYCovarianceInverse
+ * (camera_shared_y_covariance_inverse.hpp:18-149). */
+typedef struct {
+  const double *key_blocks;         /* [num_keys][2][2] row-major, NULL = identity */
+  const double *constraint_weights; /* [num constraint rows], NULL = all 1 */
+} hsba_ycov_inverse;
+
+#define HSBA_PRECISION_FP64   0     /* every kernel in FP64 ("GPU-plain")   */
+#define HSBA_PRECISION_MIXED  1     /* FP32 J / Hessian / Schur, FP64 factor, cost, gain */
+
+#define HSBA_SOLVER_AUTO   0
+#define HSBA_SOLVER_DENSE  1        /* dense FP64 Cholesky of the reduced system */
+#define HSBA_SOLVER_PCG    2        /* block-Jacobi PCG on the implicit Schur complement */
+
+typedef struct {
+  int32_t precision;                /* HSBA_PRECISION_* */
+  int32_t solver;                   /* HSBA_SOLVER_*    */
+  double  pcg_rtol;                 /* relative residual tolerance (default 1e-6) */
+  int32_t pcg_max_iterations;       /* default 500 */
+  int32_t point_group_lanes;        /* lanes per point in point kernels: 0 = tuned */
+} hsba_options;
+
+/* Optimizer settings: CameraSharedLevenbergMarquardtOptimizor ctor
+ * (camera_shared_levenberg_marquardt_optimizor.hpp:36-41). */
+typedef struct {
+  int32_t max_iterations;
+  int32_t reserved_;
+  double  tau;                      /* mu0 = tau * max diag(N)  */
+  double  eps1;                     /* stop if ||g||_inf <= eps1 */
+  double  eps2;                     /* stop if ||delta|| <= eps2 (||x|| + eps2) */
+} hsba_lm_options;
+
+#define HSBA_STOP_MAX_ITERATIONS  0
+#define HSBA_STOP_GRADIENT        1
+#define HSBA_STOP_STEP            2
+#define HSBA_STOP_FAILED          3
+
+typedef struct {
+  int32_t status;                   /* HSBA_OK or a negative code            */
+  int32_t stop_reason;              /* HSBA_STOP_*                           */
+  int32_t iterations;               /* LM iterations run (accepted + rejected) */
+  int32_t accepted;
+  double  initial_cost;             /* r^T Sigma^-1 r at x0 (no 1/2, mahalanobis_distance_calculator.hpp:28-61) */
+  double  final_cost;
+  double  final_mu;
+  int64_t pcg_iterations;           /* summed over LM iterations (PCG solver) */
+} hsba_lm_stats;
+
+typedef struct hsba_problem hsba_problem;
+
+/* Library / device information. */
+const char *hsba_version(void);
+int hsba_device_count(int *count);
+
+/* Builds the device-resident problem: observation orderings, point and image
+ * CSR, camera-pair incidence lists, reduced-system layout.  Mirrors the
+ * sparsity pass of CameraSharedNormalEquationBuilder::ComputeNormalMatrix
+ * (camera_shared_normal_equation_builder.hpp:137-205). */
+int hsba_problem_create(const hsba_problem_desc *desc,
+                        const hsba_options *options,   /* NULL = defaults */
+                        hsba_problem **problem);
+int hsba_problem_destroy(hsba_problem *problem);
+
+/* Sizes: GetXSize / GetYSize (camera_shared_vector_function.hpp:349-362). */
+int hsba_problem_sizes(const hsba_problem *problem,
+                       int64_t *x_size, int64_t *y_size,
+                       int64_t *camera_params, int64_t *reduced_size);
+
+/* CameraSharedVectorFunction::operator() (camera_shared_vector_function.hpp:412-632). */
+int hsba_evaluate(hsba_problem *problem, const double *x, double *y);
+
+/* CameraSharedAnalyticalJacobianMatrixCalculator::operator()
+ * (camera_shared_analytical_jacobian_matrix_calculator.hpp:66-332).
+ * Blocks in key order, row-major: point [N][2][3], image [N][2][6],
+ * camera [N][2][d].  A pointer may be NULL to skip that block. */
+int hsba_jacobian(hsba_problem *problem, const double *x,
+                  double *point_blocks, double *image_blocks,
+                  double *camera_blocks);
+
+/* CameraSharedNormalEquationBuilder::operator()
+ * (camera_shared_normal_equation_builder.hpp:92-462) at x with observations
+ * y_obs.  Keeps N and g resident for hsba_solve and also copies out:
+ *   point_blocks  [P][3][3]   image_blocks [I][6][6]   camera_blocks [C][d][d]
+ *   gradient      [x_size]    (reference x layout)
+ * Any output pointer may be NULL. */
+int hsba_build_normal(hsba_problem *problem, const double *x,
+                      const double *y_obs, const hsba_ycov_inverse *ycov,
+                      double *point_blocks, double *image_blocks,
+                      double *camera_blocks, double *gradient);
+
+/* CameraSharedNormalEquationSolver::operator() on the normal equations of the
+ * last hsba_build_normal, damped by mu (camera_shared_normal_equation_solver.hpp:75-114). */
+int hsba_solve(hsba_problem *problem, double mu, double *delta_x);
+
+/* Reduced camera system of the last build, damped by mu, dense row-major
+ * [reduced_size][reduced_size] and rhs [reduced_size]
+ * (ComputeReducedSystem, camera_shared_normal_equation_solver.hpp:116-446). */
+int hsba_reduced_system(hsba_problem *problem, double mu,
+                        double *reduced_matrix, double *reduced_rhs);
+
+/* LM scalar helpers on the last build:
+ *   max diag(N)            camera_shared_normal_matrix_max_diagonal_value_calculator.hpp:26-38
+ *   ||g||_inf              camera_shared_gradient_infinite_norm_calculator.hpp:27-39
+ *   g^T delta              camera_shared_gradient_delta_x_dot_product_calculator.hpp:26-38
+ *   r^T Sigma^-1 r at x    camera_shared_mahalanobis_distance_calculator.hpp:28-61 */
+int hsba_max_diagonal(hsba_problem *problem, double *value);
+int hsba_gradient_inf_norm(hsba_problem *problem, double *value);
+int hsba_gradient_dot(hsba_problem *problem, const double *delta_x, double *value);
+int hsba_mahalanobis(hsba_problem *problem, const double *x, const double *y_obs,
+                     const hsba_ycov_inverse *ycov, double *value);
+
+/* CameraSharedLevenbergMarquardtOptimizor::operator()
+ * (camera_shared_levenberg_marquardt_optimizor.hpp:43-58) with the LM loop of
+ * hs::optimizor::nllso::LevenbergMarquardt run entirely on the device. */
+int hsba_lm_solve(hsba_problem *problem, const double *x0, const double *y_obs,
+                  const hsba_ycov_inverse *ycov, const hsba_lm_options *lm,
+                  double *x_out, hsba_lm_stats *stats);
+
+/* Stepwise form of hsba_lm_solve: begin uploads x0 / y and runs the initial
+ * cost, linearization and mu0; iterate runs `iterations` more LM iterations
+ * (one captured hipGraph replay each, no host round trip); end copies x and
+ * the statistics out.  No reference counterpart: used for timing and by the
+ * serving loop. */
+int hsba_lm_begin(hsba_problem *problem, const double *x0, const double *y_obs,
+                  const hsba_ycov_inverse *ycov, const hsba_lm_options *lm);
+/* Per-stage device time of one LM iteration, from HIP event nodes captured
+ * around each stage on the library's stream (milliseconds, averaged over the
+ * iterations of one hsba_lm_iterate call). */
+typedef struct {
+  double linearize_ms;    /* K1+K2 residual, Jacobian, point blocks */
+  double image_reduce_ms; /* image / camera blocks, constraints, gradient stats */
+  double point_factor_ms; /* K3 point factor + Z = L^-1 W */
+  double pairs_ms;        /* K4 camera-pair Schur products */
+  double assemble_ms;     /* reduced matrix and right-hand side */
+  double cholesky_ms;     /* K5 factorization + triangular solves */
+  double backsub_ms;      /* K6 point back-substitution */
+  double step_cost_ms;    /* K7 x - delta, candidate cost, LM control (K8) */
+  double iteration_ms;    /* whole iteration */
+  int32_t iterations;
+  int32_t reserved_;
+} hsba_stage_times;
+int hsba_lm_iterate(hsba_problem *problem, int32_t iterations, hsba_stage_times *times);
+int hsba_lm_end(hsba_problem *problem, double *x_out, hsba_lm_stats *stats);
+
+/* ---- batched independent problems (local-BA windows) ------------------- */
+/* Many independent problems solved together: one set of launches per LM
+ * iteration covers every window.  No reference counterpart (the reference
+ * solves one problem per optimizer call); each window's arithmetic is the
+ * single-problem path's. */
+typedef struct hsba_batch hsba_batch;
+
+int hsba_batch_create(const hsba_problem_desc *descs, int64_t count,
+                      const hsba_options *options, hsba_batch **batch);
+int hsba_batch_destroy(hsba_batch *batch);
+/* x0s / y_obs / x_outs: arrays of `count` host pointers, one per window. */
+int hsba_batch_lm_solve(hsba_batch *batch, const double *const *x0s,
+                        const double *const *y_obs,
+                        const hsba_ycov_inverse *ycovs, /* [count] or NULL */
+                        const hsba_lm_options *lm,
+                        double *const *x_outs, hsba_lm_stats *stats /* [count] */);
+
+/* ---- serving loop (continuous batching) -------------------------------- */
+typedef struct hsba_server hsba_server;
+
+typedef struct {
+  int64_t hbm_budget_bytes;         /* admission limit on resident job state */
+  int32_t max_active_jobs;          /* slot count                            */
+  int32_t precision;                /* HSBA_PRECISION_*                      */
+  hsba_lm_options lm;               /* per-job LM settings                   */
+} hsba_server_options;
+
+typedef struct {
+  int64_t job_id;
+  int32_t status;                   /* HSBA_OK or negative                   */
+  int32_t reserved_;
+  double  submit_time_s;            /* server clock                          */
+  double  admit_time_s;
+  double  finish_time_s;
+  hsba_lm_stats stats;
+} hsba_job_result;
+
+int hsba_server_create(const hsba_server_options *options, hsba_server **server);
+int hsba_server_destroy(hsba_server *server);
+/* Copies the job in; returns immediately. x_out must stay valid until the
+ * job's result is polled. */
+int hsba_server_submit(hsba_server *server, const hsba_problem_desc *desc,
+                       const double *x0, const double *y_obs,
+                       const hsba_ycov_inverse *ycov, double *x_out,
+                       int64_t *job_id);
+/* Fetches up to max_results finished jobs; returns how many in *count. */
+int hsba_server_poll(hsba_server *server, hsba_job_result *results,
+                     int32_t max_results, int32_t *count);
+/* Blocks until every submitted job has finished. */
+int hsba_server_drain(hsba_server *server);
+double hsba_server_clock(const hsba_server *server);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HSBA_H_ */
