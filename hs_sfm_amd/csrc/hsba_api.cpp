@@ -44,7 +44,24 @@ int make(const hsba_problem_desc *descs, int64_t count, const hsba_options *opt,
   if (o.precision != HSBA_PRECISION_FP64 && o.precision != HSBA_PRECISION_MIXED) return HSBA_ERR_INVALID;
   std::unique_ptr<EngineBase> eng(hsba::make_engine(o.precision));
   std::string why;
-  int rc = hsba::build_layout(descs, count, eng->L, &why);
+  // AUTO: dense FP64 Cholesky while the reduced system stays small enough to
+  // factor quickly, implicit-Schur PCG beyond 4096 reduced unknowns
+  bool pcg = o.solver == HSBA_SOLVER_PCG;
+  if (o.solver == HSBA_SOLVER_AUTO && !(descs[0].fix_mask & HSBA_FIX_POINTS)) {
+    uint32_t en = 0;
+    const uint32_t comp = descs[0].intrinsic_computations_mask;
+    if (comp & HSBA_COMPUTE_RADIAL_DISTORTION) en |= 0x7u;
+    if (comp & HSBA_COMPUTE_DECENTERING_DISTORTION) en |= 0x18u;
+    if (comp & HSBA_COMPUTE_INTRINSIC_PARAMS) en |= 0x3E0u;
+    const uint32_t fb = descs[0].intrinsic_free_mask ? descs[0].intrinsic_free_mask : en;
+    const int dcam = __builtin_popcount(fb);
+    for (int64_t w = 0; w < count; ++w) {
+      const int64_t n = ((descs[w].fix_mask & HSBA_FIX_IMAGES) ? 0 : 6 * descs[w].num_images) +
+                        ((descs[w].fix_mask & HSBA_FIX_CAMERAS) ? 0 : dcam * descs[w].num_cameras);
+      if (n > 4096) pcg = true;
+    }
+  }
+  int rc = hsba::build_layout(descs, count, eng->L, &why, pcg);
   if (rc) {
     if (std::getenv("HSBA_VERBOSE")) std::fprintf(stderr, "hsba: invalid problem: %s\n", why.c_str());
     return rc;
@@ -56,7 +73,7 @@ int make(const hsba_problem_desc *descs, int64_t count, const hsba_options *opt,
   // Those instantiations of the pair kernel exceed the VGPR budget (AGPRs plus
   // ~1350 spilled SGPRs).  Kept off until a cause is shown.
   if (o.precision == HSBA_PRECISION_FP64 && eng->L.pts_free && eng->L.E >= 7 &&
-      o.solver != HSBA_SOLVER_PCG && !std::getenv("HSBA_ALLOW_FP64_WIDE"))
+      !std::getenv("HSBA_ALLOW_FP64_WIDE"))
     return HSBA_ERR_UNSUPPORTED;
   eng->precision = o.precision;
   eng->pcg_rtol = o.pcg_rtol;
@@ -68,7 +85,7 @@ int make(const hsba_problem_desc *descs, int64_t count, const hsba_options *opt,
     while ((1 << g) < o.point_group_lanes && g < 6) ++g;
     eng->glog2 = g;
   }
-  eng->solver = o.solver == HSBA_SOLVER_PCG ? HSBA_SOLVER_PCG : HSBA_SOLVER_DENSE;
+  eng->solver = pcg ? HSBA_SOLVER_PCG : HSBA_SOLVER_DENSE;
   if ((rc = eng->init(nullptr))) return rc;
   out = std::move(eng);
   return HSBA_OK;

@@ -3,13 +3,24 @@
 // (include/hs_sfm/bundle_adjustment/camera_shared_normal_equation_solver.hpp:448-466).
 //
 // Storage: each window's reduced matrix is row-major with leading dimension
-// ld (a multiple of 64), lower triangle valid, padding rows/columns set to the
-// identity once.  64 x 64 tiles are staged through LDS.
-//   * large windows: one launch per stage (diag factor, panel solve, trailing
-//     update) so every CU works on the trailing update;
-//   * small windows: one workgroup per window runs the whole blocked
-//     factorization and both triangular solves (a batch of windows is one
-//     launch).
+// ld (a multiple of 64), lower triangle valid, padding rows / columns set to
+// the identity once.  64 x 64 tiles are staged through LDS.
+//
+// Diagonal tiles: one elimination sweep of [A | I] (one barrier per column)
+// yields both L_kk and L_kk^-1; panel tiles are then GEMMs with L_kk^-1.
+//
+// Large windows (more than 4 tiles): one fused launch per panel step k.
+// Every workgroup of step k owns one trailing tile (i, j), k < j <= i: it forms
+// L_ik = A_ik L_kk^-T and L_jk itself and applies A_ij -= L_ik L_jk^T; the
+// workgroup of tile (k+1, k+1) then factors that tile (look-ahead), so each
+// step is a single launch.  The right-hand side rides along: step k also
+// forms y_k = L_kk^-1 b_k and updates b_j -= L_jk y_k.  The factor goes to a
+// separate buffer (column k of A is read by every workgroup of step k).  The
+// backward solve L^T x = y is one multi-workgroup launch per block row.
+//
+// Small windows: one workgroup (256 threads) per window runs the whole
+// blocked factorization and both solves; a batch of windows is one launch.
+//
 // A non-positive pivot marks the window (WinState::chol_fail); the reference
 // reports the same condition as SimplicialLDLT's info() != Success.
 #include "hsba_kernels.h"
@@ -21,59 +32,134 @@ constexpr int LDT = NB + 1;  // LDS tile pitch (bank-conflict padding)
 
 __device__ __forceinline__ void load_tile(double (*t)[LDT], const double *A, int64_t ld, int r0, int c0) {
   for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-    const int i = e / NB, j = e - i * NB;
+    const int i = e >> 6, j = e & 63;
     t[i][j] = A[(int64_t)(r0 + i) * ld + c0 + j];
   }
 }
 __device__ __forceinline__ void store_tile(const double (*t)[LDT], double *A, int64_t ld, int r0, int c0) {
   for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-    const int i = e / NB, j = e - i * NB;
+    const int i = e >> 6, j = e & 63;
     A[(int64_t)(r0 + i) * ld + c0 + j] = t[i][j];
   }
 }
 
-// Unblocked right-looking Cholesky of the lower triangle of an LDS tile.
-__device__ void tile_potrf(double (*t)[LDT], int *fail) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  for (int j = 0; j < NB; ++j) {
-    if (tid == 0) {
-      const double djj = t[j][j];
-      if (!(djj > 0.0)) { *fail = 1; t[j][j] = 1.0; }
-      else t[j][j] = sqrt(djj);
-    }
-    __syncthreads();
-    const double ljj = t[j][j];
-    for (int i = j + 1 + tid; i < NB; i += nt) t[i][j] /= ljj;
-    __syncthreads();
-    // trailing lower triangle: thread owns column c, rows c.. stepping by nt/64
-    const int c = j + 1 + (tid & 63);
-    if (c < NB) {
-      const double lcj = t[c][j];
-      for (int i = c + (tid >> 6); i < NB; i += (nt >> 6)) t[i][c] -= t[i][j] * lcj;
-    }
-    __syncthreads();
-  }
+// Factor a 64 x 64 SPD tile (lower triangle of src, row pitch lds) and its
+// inverse in one elimination sweep of [A | I] (256 threads):
+//   step j:  d_j = a[j][j];  a[i][c] -= a[i][j] a[c][j] / d_j   (j < c <= i)
+//            x[i][c] -= (a[i][j] / d_j) x[j][c]                  (c <= j < i)
+// leaves a[i][j] = L_unit[i][j] d_j and x = L_unit^-1, so L = L_unit D^1/2 and
+// L^-1 = D^-1/2 x.  Thread t owns column c = t % 64, rows r0 + 4m
+// (r0 = t / 64, m < 16) of both a and x in registers; each step broadcasts
+// column j of a and row j of x through double-buffered LDS, one barrier per
+// column.  Writes L (full tile, zero upper) to dstL and L^-1 to dstI.
+// 1/d for d > 0: hardware reciprocal estimate refined by two Newton–Raphson
+// steps r <- r (2 - d r) (quadratic convergence, so ~1 ulp).
+__device__ __forceinline__ double recip_nr(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = r * (2.0 - d * r);
+  r = r * (2.0 - d * r);
+  return r;
 }
 
-// Inverse of the lower-triangular LDS tile l into li (thread j solves column j).
-__device__ void tile_trtri(const double (*l)[LDT], double (*li)[LDT]) {
-  const int j = threadIdx.x;
-  if (j < NB) {
-    for (int i = 0; i < j; ++i) li[i][j] = 0.0;
-    li[j][j] = 1.0 / l[j][j];
-    for (int i = j + 1; i < NB; ++i) {
-      double s = 0.0;
-      for (int c = j; c < i; ++c) s += l[i][c] * li[c][j];
-      li[i][j] = -s / l[i][i];
+__device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dstL, int64_t ldl,
+                                                    double *dstI, int64_t ldi, int *fail) {
+  __shared__ double acol[2][NB];
+  __shared__ double xrow[2][NB];
+  __shared__ double dsh[NB];
+  const int tid = threadIdx.x;
+  const int c = tid & 63, r0 = tid >> 6;
+  double a[16], x[16];
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int i = r0 + 4 * m;
+    a[m] = (i >= c) ? src[(int64_t)i * lds + c] : 0.0;
+    x[m] = (i == c) ? 1.0 : 0.0;
+  }
+  bool bad = false;
+#pragma unroll 1
+  for (int j = 0; j < NB; ++j) {
+    const int buf = j & 1;
+    // publish column j of a (owners: c == j) and row j of x (owners: r0 == j % 4)
+    if (c == j) {
+#pragma unroll
+      for (int m = 0; m < 16; ++m) acol[buf][r0 + 4 * m] = a[m];
+    }
+    if (r0 == (j & 3)) {
+      double v = x[0];
+#pragma unroll
+      for (int m = 1; m < 16; ++m) v = (m == (j >> 2)) ? x[m] : v;
+      xrow[buf][c] = v;
+    }
+    __syncthreads();
+    // every read of this step first, then branch-free updates
+    double aij[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) aij[m] = acol[buf][r0 + 4 * m];
+    const double d = acol[buf][j];
+    const double acj_raw = acol[buf][c];
+    const double xjc = xrow[buf][c];
+    if (tid == 0) dsh[j] = d;
+    const bool ok = d > 0.0;
+    if (!ok) bad = true;
+    const double inv_d = ok ? recip_nr(d) : 0.0;
+    const double acj = acj_raw * inv_d;       // a[c][j] / d
+    const double xs = xjc * inv_d;            // x[j][c] / d
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+      const int i = r0 + 4 * m;
+      const bool below = i > j;
+      const double na = a[m] - aij[m] * acj;
+      const double nx = x[m] - aij[m] * xs;
+      a[m] = (below && c > j && i >= c) ? na : a[m];
+      x[m] = (below && c <= j) ? nx : x[m];
     }
   }
   __syncthreads();
+  if (bad && tid == 0) *fail = 1;
+  // L[i][c] = a[i][c] / sqrt(d_c) (i > c), sqrt(d_c) (i == c); L^-1[i][c] = x[i][c] / sqrt(d_i)
+  const double dc = dsh[c] > 0.0 ? dsh[c] : 1.0;
+  const double sdc = sqrt(dc);
+  const double isdc = 1.0 / sdc;
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const int i = r0 + 4 * m;
+    const double di = dsh[i] > 0.0 ? dsh[i] : 1.0;
+    dstL[(int64_t)i * ldl + c] = (i > c) ? a[m] * isdc : ((i == c) ? sdc : 0.0);
+    dstI[(int64_t)i * ldi + c] = (c <= i) ? x[m] / sqrt(di) : 0.0;
+  }
 }
 
-// C (global, 64 x 64 at r0, c0) = alpha * C + beta * (a b^T) with a, b LDS tiles.
-// 256 threads, each 4 rows x 4 strided columns.
-__device__ void tile_gemm_nt(double *C, int64_t ld, int r0, int c0, const double (*a)[LDT],
-                             const double (*b)[LDT], bool accumulate_negative) {
+// out (LDS) = a * b^T for LDS tiles (256 threads, 4 x 4 outputs each).
+__device__ __forceinline__ void tile_mul_nt_lds(double (*out)[LDT], const double (*a)[LDT], const double (*b)[LDT]) {
+  const int tid = threadIdx.x;
+  const int ti = (tid >> 4) * 4, tc = tid & 15;
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int k = 0; k < NB; ++k) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) av[u] = a[ti + u][k];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) bv[v] = b[tc + 16 * v][k];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+  }
+  __syncthreads();  // a or b may alias out
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) out[ti + u][tc + 16 * v] = acc[u][v];
+  __syncthreads();
+}
+
+// C (global 64 x 64 at r0, c0) -= a b^T.
+__device__ __forceinline__ void tile_sub_nt_global(double *C, int64_t ld, int r0, int c0, const double (*a)[LDT],
+                                                   const double (*b)[LDT]) {
   const int tid = threadIdx.x;
   const int ti = (tid >> 4) * 4, tc = tid & 15;
   double acc[4][4];
@@ -95,189 +181,199 @@ __device__ void tile_gemm_nt(double *C, int64_t ld, int r0, int c0, const double
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      double *dst = C + (int64_t)(r0 + ti + u) * ld + c0 + tc + 16 * v;
-      if (accumulate_negative) *dst -= acc[u][v];
-      else *dst = acc[u][v];
-    }
+    for (int v = 0; v < 4; ++v) C[(int64_t)(r0 + ti + u) * ld + c0 + tc + 16 * v] -= acc[u][v];
 }
 
+// y = M v (transposed = false) or M^T v for a 64 x 64 global tile M with row
+// pitch ld and a 64-vector v in LDS; result to LDS out (256 threads).
+__device__ __forceinline__ void tile_gemv(const double *M, int64_t ld, const double *v, double *out, bool transposed,
+                                          double (*scratch)[NB]) {
+  const int r = threadIdx.x & 63, sl = threadIdx.x >> 6;  // 4 slices of 16
+  double acc = 0.0;
+  if (!transposed)
+    for (int k = sl * 16; k < sl * 16 + 16; ++k) acc += M[(int64_t)r * ld + k] * v[k];
+  else
+    for (int k = sl * 16; k < sl * 16 + 16; ++k) acc += M[(int64_t)k * ld + r] * v[k];
+  scratch[sl][r] = acc;
+  __syncthreads();
+  if (threadIdx.x < NB) out[threadIdx.x] = scratch[0][threadIdx.x] + scratch[1][threadIdx.x] +
+                                           scratch[2][threadIdx.x] + scratch[3][threadIdx.x];
+  __syncthreads();
+}
+
+__device__ __forceinline__ bool chol_active(const WinState &s) { return s.active && s.trial && !s.chol_fail; }
+
 // ---------------------------------------------------------------- large path
-__global__ void __launch_bounds__(256) k_chol_diag(WinInfo wi, WinState *ws, int w, double *S, double *Dinv, int kb) {
-  __shared__ double t[NB][LDT];
-  __shared__ double li[NB][LDT];
+// Diagonal tile 0 (before the first step).
+__global__ void __launch_bounds__(256) k_chol_first(WinInfo wi, WinState *ws, int w, const double *S, double *Lf,
+                                                    double *Dinv) {
   __shared__ int fail;
   WinState &s = ws[w];
-  if (!(s.active && s.trial) || s.chol_fail) return;
+  if (!chol_active(s)) return;
   if (threadIdx.x == 0) fail = 0;
-  double *A = S + wi.s_off;
-  load_tile(t, A, wi.ld, kb * NB, kb * NB);
   __syncthreads();
-  tile_potrf(t, &fail);
-  tile_trtri(t, li);
-  store_tile(t, A, wi.ld, kb * NB, kb * NB);
-  store_tile(li, Dinv + wi.dinv_off + (int64_t)kb * NB * NB, NB, 0, 0);
+  tile_factor_inverse(S + wi.s_off, wi.ld, Lf + wi.s_off, wi.ld, Dinv + wi.dinv_off, NB, &fail);
+  __syncthreads();
   if (threadIdx.x == 0 && fail) s.chol_fail = 1;
 }
 
-__global__ void __launch_bounds__(256) k_chol_panel(WinInfo wi, WinState *ws, int w, double *S, const double *Dinv,
-                                                    int kb) {
-  __shared__ double a[NB][LDT];
+// Step k: trailing tiles (i, j), k < j <= i < nb, in row order, then the
+// right-hand-side workgroups j = k+1 .. nb-1.
+__global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int w, double *S, double *Lf,
+                                                   double *Dinv, double *b, double *y, int k, int ntiles) {
   __shared__ double li[NB][LDT];
-  const WinState &s = ws[w];
-  if (!(s.active && s.trial) || s.chol_fail) return;
-  const int ib = kb + 1 + blockIdx.x;
+  __shared__ double pa[NB][LDT];
+  __shared__ double pb[NB][LDT];
+  __shared__ double vsh[3][NB];
+  __shared__ double red[4][NB];
+  __shared__ int fail;
+  WinState &s = ws[w];
+  if (!chol_active(s)) return;
+  const int64_t ld = wi.ld;
   double *A = S + wi.s_off;
-  load_tile(a, A, wi.ld, ib * NB, kb * NB);
-  load_tile(li, Dinv + wi.dinv_off + (int64_t)kb * NB * NB, NB, 0, 0);
-  __syncthreads();
-  __shared__ double out_unused;
-  (void)out_unused;
-  tile_gemm_nt(A, wi.ld, ib * NB, kb * NB, a, li, false);  // L_ib = A_ib Linv^T
-}
-
-__global__ void __launch_bounds__(256) k_chol_update(WinInfo wi, WinState *ws, int w, double *S, int kb) {
-  __shared__ double a[NB][LDT];
-  __shared__ double b[NB][LDT];
-  const WinState &s = ws[w];
-  if (!(s.active && s.trial) || s.chol_fail) return;
-  // tile index -> (ib, jb), kb < jb <= ib < nb, lower triangle in row order
+  double *L = Lf + wi.s_off;
+  const double *Li = Dinv + wi.dinv_off + (int64_t)k * NB * NB;
+  load_tile(li, Li, NB, 0, 0);
+  if (blockIdx.x >= ntiles) {
+    // right-hand side: y_k = Linv_k b_k, w = Linv_k^T y_k, b_j -= A_jk w
+    const int jb = k + 1 + (blockIdx.x - ntiles);
+    for (int e = threadIdx.x; e < NB; e += blockDim.x) vsh[0][e] = b[wi.r_off + k * NB + e];
+    __syncthreads();
+    tile_gemv(Li, NB, vsh[0], vsh[1], false, red);       // y_k
+    tile_gemv(Li, NB, vsh[1], vsh[2], true, red);        // w
+    if (jb == k + 1 && threadIdx.x < NB) y[wi.r_off + k * NB + threadIdx.x] = vsh[1][threadIdx.x];
+    tile_gemv(A + (int64_t)jb * NB * ld + k * NB, ld, vsh[2], vsh[0], false, red);
+    if (threadIdx.x < NB) b[wi.r_off + jb * NB + threadIdx.x] -= vsh[0][threadIdx.x];
+    return;
+  }
+  // tile index -> (i, j) relative to k + 1
   const int t = blockIdx.x;
   int r = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while (r * (r + 1) / 2 > t) --r;
   while ((r + 1) * (r + 2) / 2 <= t) ++r;
-  const int ib = kb + 1 + r, jb = kb + 1 + (t - r * (r + 1) / 2);
-  double *A = S + wi.s_off;
-  load_tile(a, A, wi.ld, ib * NB, kb * NB);
-  load_tile(b, A, wi.ld, jb * NB, kb * NB);
+  const int ib = k + 1 + r, jb = k + 1 + (t - r * (r + 1) / 2);
+  load_tile(pa, A, ld, ib * NB, k * NB);
+  if (jb != ib) load_tile(pb, A, ld, jb * NB, k * NB);
   __syncthreads();
-  tile_gemm_nt(A, wi.ld, ib * NB, jb * NB, a, b, true);
-}
-
-// Blocked forward (L y = b) and backward (L^T x = y) substitution of one window
-// with the diagonal-tile inverses; x overwrites dred.  1024 threads.
-__device__ void trsv_window(const WinInfo &wi, const double *S, const double *Dinv, const double *rhs,
-                            double *x) {
-  const double *A = S + wi.s_off;
-  const int64_t ld = wi.ld;
-  const int n = wi.n, nb = wi.nb;
-  __shared__ double yb[NB];
-  __shared__ double part[16][NB];
-  // y = rhs (padding rows are zero)
-  for (int i = threadIdx.x; i < ld; i += blockDim.x) x[i] = i < n ? rhs[i] : 0.0;
-  __syncthreads();
-  for (int kb = 0; kb < nb; ++kb) {
-    const double *Li = Dinv + wi.dinv_off + (int64_t)kb * NB * NB;
-    // z = Linv_kk y_kb : row r = tid & 63, partial over 16 slices of k
-    {
-      const int r = threadIdx.x & 63, sl = threadIdx.x >> 6;  // sl < 16
-      double acc = 0.0;
-      for (int k = sl * 4; k < sl * 4 + 4; ++k) acc += Li[r * NB + k] * x[kb * NB + k];
-      part[sl][r] = acc;
-    }
+  tile_mul_nt_lds(pa, pa, li);                          // L_ik = A_ik Linv^T
+  if (jb != ib) tile_mul_nt_lds(pb, pb, li);            // L_jk
+  if (jb == k + 1) store_tile(pa, L, ld, ib * NB, k * NB);
+  tile_sub_nt_global(A, ld, ib * NB, jb * NB, pa, jb != ib ? pb : pa);
+  if (ib == k + 1 && jb == k + 1) {
+    // look-ahead: factor the next diagonal tile now that it is final
+    __threadfence_block();
+    if (threadIdx.x == 0) fail = 0;
     __syncthreads();
-    if (threadIdx.x < NB) {
-      double z = 0.0;
-      for (int sl = 0; sl < 16; ++sl) z += part[sl][threadIdx.x];
-      yb[threadIdx.x] = z;
-      x[kb * NB + threadIdx.x] = z;
-    }
+    tile_factor_inverse(A + (int64_t)ib * NB * ld + ib * NB, ld, L + (int64_t)ib * NB * ld + ib * NB, ld,
+                        Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB, &fail);
     __syncthreads();
-    for (int i = (kb + 1) * NB + threadIdx.x; i < ld; i += blockDim.x) {
-      const double *row = A + (int64_t)i * ld + kb * NB;
-      double acc = 0.0;
-      for (int k = 0; k < NB; ++k) acc += row[k] * yb[k];
-      x[i] -= acc;
-    }
-    __syncthreads();
-  }
-  for (int kb = nb - 1; kb >= 0; --kb) {
-    const double *Li = Dinv + wi.dinv_off + (int64_t)kb * NB * NB;
-    // z = Linv_kk^T y_kb
-    {
-      const int r = threadIdx.x & 63, sl = threadIdx.x >> 6;
-      double acc = 0.0;
-      for (int k = sl * 4; k < sl * 4 + 4; ++k) acc += Li[k * NB + r] * x[kb * NB + k];
-      part[sl][r] = acc;
-    }
-    __syncthreads();
-    if (threadIdx.x < NB) {
-      double z = 0.0;
-      for (int sl = 0; sl < 16; ++sl) z += part[sl][threadIdx.x];
-      yb[threadIdx.x] = z;
-      x[kb * NB + threadIdx.x] = z;
-    }
-    __syncthreads();
-    // x_r -= sum_{c in block kb} L[c][r] z_c for r < kb*NB
-    for (int r = threadIdx.x; r < kb * NB; r += blockDim.x) {
-      double acc = 0.0;
-      for (int k = 0; k < NB; ++k) acc += A[(int64_t)(kb * NB + k) * ld + r] * yb[k];
-      x[r] -= acc;
-    }
-    __syncthreads();
+    if (threadIdx.x == 0 && fail) s.chol_fail = 1;
   }
 }
 
-__global__ void __launch_bounds__(1024) k_trsv_large(WinInfo wi, WinState *ws, int w, const double *S,
-                                                     const double *Dinv, const double *rhs, double *dred) {
+// Backward solve step k: x_k = Linv_k^T y_k (y_k = Linv_k b_k when k is the
+// last block), then y_i -= L_ki^T x_k for every block i < k (one workgroup each).
+__global__ void __launch_bounds__(256) k_chol_back(WinInfo wi, WinState *ws, int w, const double *Lf,
+                                                   const double *Dinv, const double *b, double *y, double *x,
+                                                   int k) {
+  __shared__ double vsh[3][NB];
+  __shared__ double red[4][NB];
   const WinState &s = ws[w];
-  if (!(s.active && s.trial) || s.chol_fail) return;
-  trsv_window(wi, S, Dinv, rhs + wi.r_off, dred + wi.r_off);
+  if (!chol_active(s)) return;
+  const int64_t ld = wi.ld;
+  const double *Li = Dinv + wi.dinv_off + (int64_t)k * NB * NB;
+  const bool last = k == wi.nb - 1;
+  for (int e = threadIdx.x; e < NB; e += blockDim.x)
+    vsh[0][e] = last ? b[wi.r_off + k * NB + e] : y[wi.r_off + k * NB + e];
+  __syncthreads();
+  if (last) {
+    tile_gemv(Li, NB, vsh[0], vsh[1], false, red);
+    if (threadIdx.x < NB) vsh[0][threadIdx.x] = vsh[1][threadIdx.x];
+    __syncthreads();
+  }
+  tile_gemv(Li, NB, vsh[0], vsh[1], true, red);  // x_k
+  if (blockIdx.x == 0 && threadIdx.x < NB) x[wi.r_off + k * NB + threadIdx.x] = vsh[1][threadIdx.x];
+  if (k == 0) return;
+  const int ib = blockIdx.x;  // < k
+  tile_gemv(Lf + wi.s_off + (int64_t)k * NB * ld + ib * NB, ld, vsh[1], vsh[2], true, red);
+  if (threadIdx.x < NB) y[wi.r_off + ib * NB + threadIdx.x] -= vsh[2][threadIdx.x];
 }
 
 // ---------------------------------------------------------------- small path
-// One workgroup (256 threads) per window: the whole blocked factorization.
+// Whole blocked factorization and both triangular solves of one window per
+// workgroup (nb <= 4).  b is the right-hand side, x receives the solution.
 __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState *ws, const int *win_list,
-                                                    double *S, double *Dinv) {
+                                                    double *S, double *Dinv, const double *rhs, double *dred) {
   __shared__ double a[NB][LDT];
-  __shared__ double b[NB][LDT];
+  __shared__ double c[NB][LDT];
+  __shared__ double vsh[3][NB];
+  __shared__ double red[4][NB];
   __shared__ int fail;
   const int w = win_list[blockIdx.x];
   const WinInfo wi = wis[w];
   WinState &s = ws[w];
-  if (!(s.active && s.trial) || s.chol_fail) return;
+  if (!chol_active(s)) return;
   if (threadIdx.x == 0) fail = 0;
   double *A = S + wi.s_off;
   const int64_t ld = wi.ld;
+  double *Dw = Dinv + wi.dinv_off;
   for (int kb = 0; kb < wi.nb; ++kb) {
     __syncthreads();
-    load_tile(a, A, ld, kb * NB, kb * NB);
+    __threadfence_block();
+    tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, A + (int64_t)kb * NB * ld + kb * NB, ld,
+                        Dw + (int64_t)kb * NB * NB, NB, &fail);
+    __threadfence_block();
     __syncthreads();
-    tile_potrf(a, &fail);
-    tile_trtri(a, b);
-    store_tile(a, A, ld, kb * NB, kb * NB);
-    store_tile(b, Dinv + wi.dinv_off + (int64_t)kb * NB * NB, NB, 0, 0);
+    load_tile(c, Dw + (int64_t)kb * NB * NB, NB, 0, 0);
     __syncthreads();
+    // panel: L_ib = A_ib Linv^T (c holds Linv)
     for (int ib = kb + 1; ib < wi.nb; ++ib) {
       load_tile(a, A, ld, ib * NB, kb * NB);
       __syncthreads();
-      tile_gemm_nt(A, ld, ib * NB, kb * NB, a, b, false);
-      __syncthreads();
+      tile_mul_nt_lds(a, a, c);
+      store_tile(a, A, ld, ib * NB, kb * NB);
     }
-    __threadfence_block();
-    for (int ib = kb + 1; ib < wi.nb; ++ib) {
+    __syncthreads();
+    for (int ib = kb + 1; ib < wi.nb; ++ib)
       for (int jb = kb + 1; jb <= ib; ++jb) {
-        __syncthreads();
         load_tile(a, A, ld, ib * NB, kb * NB);
-        load_tile(b, A, ld, jb * NB, kb * NB);
+        load_tile(c, A, ld, jb * NB, kb * NB);
         __syncthreads();
-        tile_gemm_nt(A, ld, ib * NB, jb * NB, a, b, true);
+        tile_sub_nt_global(A, ld, ib * NB, jb * NB, a, c);
+        __syncthreads();
       }
-    }
-    __threadfence_block();
   }
   __syncthreads();
   if (threadIdx.x == 0 && fail) s.chol_fail = 1;
-}
-
-__global__ void __launch_bounds__(1024) k_trsv_small(const WinInfo *wis, WinState *ws, const int *win_list,
-                                                     const double *S, const double *Dinv, const double *rhs,
-                                                     double *dred) {
-  const int w = win_list[blockIdx.x];
-  const WinInfo wi = wis[w];
-  const WinState &s = ws[w];
-  if (!(s.active && s.trial) || s.chol_fail) return;
-  trsv_window(wi, S, Dinv, rhs + wi.r_off, dred + wi.r_off);
+  if (fail) return;
+  // forward: x = rhs; for kb: x_kb = Linv_kb x_kb; x_i -= L_i,kb x_kb (i > kb)
+  double *x = dred + wi.r_off;
+  for (int i = threadIdx.x; i < ld; i += blockDim.x) x[i] = i < wi.n ? rhs[wi.r_off + i] : 0.0;
+  __syncthreads();
+  for (int kb = 0; kb < wi.nb; ++kb) {
+    for (int e = threadIdx.x; e < NB; e += blockDim.x) vsh[0][e] = x[kb * NB + e];
+    __syncthreads();
+    tile_gemv(Dw + (int64_t)kb * NB * NB, NB, vsh[0], vsh[1], false, red);
+    if (threadIdx.x < NB) x[kb * NB + threadIdx.x] = vsh[1][threadIdx.x];
+    for (int ib = kb + 1; ib < wi.nb; ++ib) {
+      tile_gemv(A + (int64_t)ib * NB * ld + kb * NB, ld, vsh[1], vsh[2], false, red);
+      if (threadIdx.x < NB) x[ib * NB + threadIdx.x] -= vsh[2][threadIdx.x];
+      __syncthreads();
+    }
+    __syncthreads();
+  }
+  for (int kb = wi.nb - 1; kb >= 0; --kb) {
+    for (int e = threadIdx.x; e < NB; e += blockDim.x) vsh[0][e] = x[kb * NB + e];
+    __syncthreads();
+    tile_gemv(Dw + (int64_t)kb * NB * NB, NB, vsh[0], vsh[1], true, red);
+    if (threadIdx.x < NB) x[kb * NB + threadIdx.x] = vsh[1][threadIdx.x];
+    for (int ib = 0; ib < kb; ++ib) {
+      tile_gemv(A + (int64_t)kb * NB * ld + ib * NB, ld, vsh[1], vsh[2], true, red);
+      if (threadIdx.x < NB) x[ib * NB + threadIdx.x] -= vsh[2][threadIdx.x];
+      __syncthreads();
+    }
+    __syncthreads();
+  }
 }
 
 // Identity on the padding rows / columns (once, after allocation).
@@ -293,34 +389,38 @@ __global__ void k_pad_identity(const WinInfo *wis, int nwin, double *S) {
   }
 }
 
+// b = rhs with zero padding (large windows: the step kernels update b in place).
+__global__ void k_rhs_copy(WinInfo wi, WinState *ws, int w, const double *rhs, double *b) {
+  const WinState &s = ws[w];
+  if (!chol_active(s)) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wi.ld; i += gridDim.x * blockDim.x)
+    b[wi.r_off + i] = i < wi.n ? rhs[wi.r_off + i] : 0.0;
+}
+
 void chol_init_padding(const CholPlan &plan, hipStream_t st) {
   if (plan.nwin == 0) return;
   hipLaunchKernelGGL(k_pad_identity, dim3(64, plan.nwin), dim3(256), 0, st, plan.wi, plan.nwin, plan.S);
 }
 
-// small_list / large windows are described by the host copy of WinInfo.
 void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int num_small,
                             const int *large_host, int num_large, hipStream_t st) {
-  if (num_small > 0) {
+  if (num_small > 0)
     hipLaunchKernelGGL(k_chol_small, dim3(num_small), dim3(256), 0, st, plan.wi, plan.ws, small_list_dev,
-                       plan.S, plan.Dinv);
-    hipLaunchKernelGGL(k_trsv_small, dim3(num_small), dim3(1024), 0, st, plan.wi, plan.ws, small_list_dev,
                        plan.S, plan.Dinv, plan.rhs, plan.dred);
-  }
   for (int q = 0; q < num_large; ++q) {
     const int w = large_host[q];
     const WinInfo wi = plan.wi_host[w];
-    for (int kb = 0; kb < wi.nb; ++kb) {
-      hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(256), 0, st, wi, plan.ws, w, plan.S, plan.Dinv, kb);
-      const int rest = wi.nb - kb - 1;
-      if (rest > 0) {
-        hipLaunchKernelGGL(k_chol_panel, dim3(rest), dim3(256), 0, st, wi, plan.ws, w, plan.S, plan.Dinv, kb);
-        hipLaunchKernelGGL(k_chol_update, dim3(rest * (rest + 1) / 2), dim3(256), 0, st, wi, plan.ws, w, plan.S,
-                           kb);
-      }
+    hipLaunchKernelGGL(k_rhs_copy, dim3(8), dim3(256), 0, st, wi, plan.ws, w, plan.rhs, plan.bwork);
+    hipLaunchKernelGGL(k_chol_first, dim3(1), dim3(256), 0, st, wi, plan.ws, w, plan.S, plan.Lf, plan.Dinv);
+    for (int k = 0; k + 1 < wi.nb; ++k) {
+      const int rest = wi.nb - k - 1;
+      const int ntiles = rest * (rest + 1) / 2;
+      hipLaunchKernelGGL(k_chol_step, dim3(ntiles + rest), dim3(256), 0, st, wi, plan.ws, w, plan.S, plan.Lf,
+                         plan.Dinv, plan.bwork, plan.ywork, k, ntiles);
     }
-    hipLaunchKernelGGL(k_trsv_large, dim3(1), dim3(1024), 0, st, wi, plan.ws, w, plan.S, plan.Dinv, plan.rhs,
-                       plan.dred);
+    for (int k = wi.nb - 1; k >= 0; --k)
+      hipLaunchKernelGGL(k_chol_back, dim3(k > 0 ? k : 1), dim3(256), 0, st, wi, plan.ws, w, plan.Lf, plan.Dinv,
+                         plan.bwork, plan.ywork, plan.dred, k);
   }
 }
 

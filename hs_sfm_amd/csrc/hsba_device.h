@@ -27,6 +27,12 @@ struct WinState {
   int32_t pcg_active, pad2_[3];
 };
 
+// Per-window state of the preconditioned conjugate gradient solver.
+struct PcgState {
+  double rz, pap, alpha, beta, bnorm2, rr, rtol2, pad_;
+  int32_t it, done, max_it, pad2_;
+};
+
 // Static per-window layout (host-built).
 struct WinInfo {
   int64_t key_off, s_off, dinv_off, blk_off, lo_off;
@@ -57,6 +63,7 @@ template <typename T>
 struct Dev {
   int32_t P, I, C, N, nwin, npairs, ncons, nchunks;
   int32_t pts_free, imgs_free, cams_free, has_cov;
+  int32_t pcg_rows, pad_dev_;
   int32_t fidx[kFull];    // free intrinsic indices (first D valid)
   // keys, point-major order
   const int32_t *key_img, *key_pt, *key_orig;
@@ -98,7 +105,9 @@ struct Dev {
   double *partial;          // [nchunks][4]
   // PCG work vectors (global reduced space)
   double *pv, *pr, *pz, *pp, *pAp, *Minv;
-  T *tpt;                   // [P][3] point-space intermediate of the implicit matvec
+  T *tpt;                   // unused
+  double *tpd;              // [P][3] point-space intermediate of the implicit matvec
+  PcgState *pcg;
 };
 
 // ---------------------------------------------------------------- math

@@ -64,6 +64,10 @@ struct Engine final : EngineBase {
     drop_stage_graphs();
     for (int i = 0; i <= kStages; ++i)
       if (ev[i]) (void)hipEventDestroy(ev[i]);
+    for (int i = 0; i < 3; ++i) {
+      if (pg_exec[i]) (void)hipGraphExecDestroy(pg_exec[i]);
+      if (pg_graph[i]) (void)hipGraphDestroy(pg_graph[i]);
+    }
     if (st) (void)hipStreamSynchronize(st);
     for (void *p : owned) (void)hipFree(p);
     if (own_stream && st) (void)hipStreamDestroy(st);
@@ -78,6 +82,15 @@ struct Engine final : EngineBase {
     gexec = nullptr;
     graph = nullptr;
     drop_stage_graphs();
+    drop_pcg_graphs();
+  }
+  void drop_pcg_graphs() {
+    for (int i = 0; i < 3; ++i) {
+      if (pg_exec[i]) (void)hipGraphExecDestroy(pg_exec[i]);
+      if (pg_graph[i]) (void)hipGraphDestroy(pg_graph[i]);
+      pg_exec[i] = nullptr;
+      pg_graph[i] = nullptr;
+    }
   }
   void drop_stage_graphs() {
     for (int i = 0; i < kStages; ++i) {
@@ -91,6 +104,7 @@ struct Engine final : EngineBase {
   int init(hipStream_t stream_in) override {
     ks = make_kernel_set<T>(L.IMG, L.D);
     if (!ks) return HSBA_ERR_UNSUPPORTED;
+    ks->pcg_params(pcg_rtol, pcg_max_iterations);
     if (stream_in) st = stream_in;
     else { HCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking)); own_stream = true; }
     std::memset(&d, 0, sizeof(d));
@@ -99,6 +113,7 @@ struct Engine final : EngineBase {
     d.nwin = L.nwin; d.npairs = (int32_t)L.npairs; d.ncons = (int32_t)L.cons.size();
     d.nchunks = (int32_t)L.chunk_kind.size();
     d.pts_free = L.pts_free; d.imgs_free = L.imgs_free; d.cams_free = L.cams_free; d.has_cov = 0;
+    d.pcg_rows = L.pcg ? (int32_t)L.rows_total : 0;
     for (int j = 0; j < kFull; ++j) d.fidx[j] = L.fidx[j];
     int rc = 0;
 #define A(ptr, n) if ((rc = dalloc(ptr, (n), owned))) return rc
@@ -135,6 +150,7 @@ struct Engine final : EngineBase {
     A(d.partial, 4LL * d.nchunks);
     A(d.pv, L.rows_total); A(d.pr, L.rows_total); A(d.pz, L.rows_total); A(d.pp, L.rows_total);
     A(d.pAp, L.rows_total); A(d.Minv, (int64_t)E * E * L.I);
+    A(d.tpd, 3 * L.P); A(d.pcg, L.nwin);
     A(small_list_dev, (int64_t)L.small_list.size());
 #undef A
     if ((rc = upload(key_img, L.key_img, st)) || (rc = upload(key_pt, L.key_pt, st)) ||
@@ -175,8 +191,16 @@ struct Engine final : EngineBase {
     }
     plan.nwin = L.nwin; plan.wi = wi; plan.ws = d.ws; plan.S = d.S; plan.rhs = d.rhs; plan.dred = d.dred;
     plan.Dinv = d.Dinv; plan.wi_host = L.wi.data();
+    plan.Lf = nullptr; plan.bwork = nullptr; plan.ywork = nullptr;
+    {
+      int64_t large_s = 0;
+      for (int w : L.large_list) large_s = std::max<int64_t>(large_s, L.wi[w].s_off + (int64_t)L.wi[w].ld * L.wi[w].ld);
+      if ((rc = dalloc(plan.Lf, large_s, owned)) || (rc = dalloc(plan.bwork, L.rows_total, owned)) ||
+          (rc = dalloc(plan.ywork, L.rows_total, owned)))
+        return rc;
+    }
     HCHK(hipMemsetAsync(d.S, 0, sizeof(double) * (size_t)std::max<int64_t>(1, L.S_total), st));
-    chol_init_padding(plan, st);
+    if (!L.pcg) chol_init_padding(plan, st);  // no dense reduced matrix in PCG mode
     HCHK(hipGetLastError());
     ws_host.assign(L.nwin, WinState());
     HCHK(hipStreamSynchronize(st));
@@ -326,6 +350,102 @@ struct Engine final : EngineBase {
                            (int)L.large_list.size(), st);
     ks->backsub(d, st, glog2);
   }
+
+  // ---- PCG form of a trial: setup graph, chunks of PCG iterations until
+  // every window has converged (host checks flags between chunks), rest.
+  static constexpr int kPcgChunk = 8;
+  hipGraphExec_t pg_exec[3] = {};
+  hipGraph_t pg_graph[3] = {};
+  std::vector<PcgState> pcg_host;
+
+  void enqueue_pcg_part(int part) {
+    if (part == 0) {
+      ks->lm_control(d, st, 1);
+      ks->point_factor(d, st);
+      ks->key_z(d, st);
+      ks->pairs(d, st);
+      ks->assemble(d, st, 0, (int)L.rows_total);
+      ks->pcg_setup(d, st, glog2);
+    } else if (part == 1) {
+      for (int i = 0; i < kPcgChunk; ++i) ks->pcg_iteration(d, st, glog2);
+    } else {
+      ks->pcg_finish(d, st);
+      ks->backsub(d, st, glog2);
+      ks->step(d, st);
+      ks->lm_control(d, st, 2);
+      ks->cost(d, st, 1);
+      ks->lm_control(d, st, 3);
+      ks->commit(d, st);
+      enqueue_linearize();
+      ks->grad_stats(d, st);
+      ks->lm_control(d, st, 4);
+    }
+  }
+
+  int capture_pcg() {
+    if (pg_exec[0]) return HSBA_OK;
+    for (int part = 0; part < 3; ++part) {
+      HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      enqueue_pcg_part(part);
+      hipError_t e1 = hipStreamEndCapture(st, &pg_graph[part]);
+      if (e1 != hipSuccess) return HSBA_ERR_DEVICE;
+      HCHK(hipGraphInstantiate(&pg_exec[part], pg_graph[part], nullptr, nullptr, 0));
+    }
+    pcg_host.resize(L.nwin);
+    return HSBA_OK;
+  }
+
+  // Runs the PCG chunks of one trial; returns after every window converged
+  // or the iteration budget is spent.
+  int pcg_solve_loop() {
+    const int chunks = (pcg_max_iterations + kPcgChunk - 1) / kPcgChunk;
+    for (int c = 0; c < chunks; ++c) {
+      HCHK(hipGraphLaunch(pg_exec[1], st));
+      HCHK(hipMemcpyAsync(pcg_host.data(), d.pcg, sizeof(PcgState) * L.nwin, hipMemcpyDeviceToHost, st));
+      HCHK(hipStreamSynchronize(st));
+      bool all = true;
+      for (const PcgState &g : pcg_host) all = all && g.done;
+      if (all) break;
+    }
+    return HSBA_OK;
+  }
+
+  int pcg_iterate(int iterations, hsba_stage_times *times) {
+    int rc = capture_pcg();
+    if (rc) return rc;
+    hipEvent_t e[4] = {};
+    if (times)
+      for (int i = 0; i < 4; ++i) HCHK(hipEventCreate(&e[i]));
+    double acc[3] = {0, 0, 0}, tot = 0;
+    for (int it = 0; it < iterations; ++it) {
+      if (times) HCHK(hipEventRecord(e[0], st));
+      HCHK(hipGraphLaunch(pg_exec[0], st));
+      if (times) HCHK(hipEventRecord(e[1], st));
+      if ((rc = pcg_solve_loop())) return rc;
+      if (times) HCHK(hipEventRecord(e[2], st));
+      HCHK(hipGraphLaunch(pg_exec[2], st));
+      if (times) {
+        HCHK(hipEventRecord(e[3], st));
+        HCHK(hipEventSynchronize(e[3]));
+        float ms;
+        for (int i = 0; i < 3; ++i) { HCHK(hipEventElapsedTime(&ms, e[i], e[i + 1])); acc[i] += ms; }
+        HCHK(hipEventElapsedTime(&ms, e[0], e[3]));
+        tot += ms;
+      }
+    }
+    HCHK(hipStreamSynchronize(st));
+    if (times) {
+      for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+      std::memset(times, 0, sizeof(*times));
+      const double inv = 1.0 / iterations;
+      times->point_factor_ms = acc[0] * inv;  // factor, Z, diagonal pairs, rhs, preconditioner
+      times->cholesky_ms = acc[1] * inv;      // PCG iterations
+      times->step_cost_ms = acc[2] * inv;     // back-substitution, step, cost, relinearization
+      times->iteration_ms = tot * inv;
+      times->iterations = iterations;
+    }
+    return HSBA_OK;
+  }
   void enqueue_iteration() {
     ks->lm_control(d, st, 1);
     enqueue_trial();
@@ -411,6 +531,7 @@ struct Engine final : EngineBase {
   int lm_iterate(int iterations, hsba_stage_times *times) override {
     if (lm_failed_early) return HSBA_ERR_GENERIC;
     if (iterations <= 0) return HSBA_OK;
+    if (L.pcg) return pcg_iterate(iterations, times);
     if (!times) {
       if (!gexec) {
         HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -559,11 +680,33 @@ struct Engine final : EngineBase {
     if (L.hw[0].unobserved) return HSBA_ERR_GENERIC;
     int rc = set_state(nullptr, mu, 1, 0, 1);
     if (rc) return rc;
+    const WinInfo &wi = L.wi[0];
+    if (L.pcg) {
+      if (S || rhs) return HSBA_ERR_UNSUPPORTED;  // no dense reduced matrix in PCG mode
+      if ((rc = capture_pcg())) return rc;
+      ks->point_factor(d, st);
+      ks->key_z(d, st);
+      ks->pairs(d, st);
+      ks->assemble(d, st, 0, (int)L.rows_total);
+      ks->pcg_setup(d, st, glog2);
+      if ((rc = pcg_solve_loop())) return rc;
+      ks->backsub(d, st, glog2);
+      HCHK(hipGetLastError());
+      if ((rc = get_state())) return rc;
+      if (ws_host[0].chol_fail) return HSBA_ERR_GENERIC;
+      const HostWindow &h = L.hw[0];
+      std::vector<double> dp(L.pts_free ? 3 * L.P : 0), dr(std::max<int64_t>(1, L.rows_total));
+      if (!dp.empty()) HCHK(hipMemcpy(dp.data(), d.dpt, 8 * dp.size(), hipMemcpyDeviceToHost));
+      HCHK(hipMemcpy(dr.data(), d.dred, 8 * dr.size(), hipMemcpyDeviceToHost));
+      if (L.pts_free) std::copy(dp.begin(), dp.end(), delta + h.x_pt);
+      const int64_t r0 = L.pts_free ? 3 * L.P : 0;
+      for (int j = 0; j < wi.n; ++j) delta[r0 + j] = dr[wi.r_off + j];
+      return HSBA_OK;
+    }
     ks->point_factor(d, st);
     ks->key_z(d, st);
     ks->pairs(d, st);
     ks->assemble(d, st, L.lower_total, (int)L.rows_total);
-    const WinInfo &wi = L.wi[0];
     if (S || rhs) {
       HCHK(hipStreamSynchronize(st));
       std::vector<double> hS((size_t)wi.ld * wi.ld), hr(wi.ld);

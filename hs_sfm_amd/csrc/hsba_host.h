@@ -44,9 +44,11 @@ struct Layout {
   int64_t P = 0, I = 0, C = 0, N = 0, npairs = 0;
   int64_t S_total = 0, rows_total = 0, dinv_total = 0, lower_total = 0;
   int64_t incidences = 0;
+  bool pcg = false;       // implicit Schur: diagonal pairs only, no dense reduced matrix
 };
 
-int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::string *why);
+int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::string *why,
+                 bool pcg = false);
 int64_t layout_device_bytes(const Layout &L, int precision);
 
 struct EngineBase {

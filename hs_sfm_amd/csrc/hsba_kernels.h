@@ -26,10 +26,11 @@ struct KernelSet {
   virtual void commit(const Dev<T> &d, hipStream_t st) = 0;
   virtual void export_keys(const Dev<T> &d, hipStream_t st, double *y, double *jp, double *ji,
                            double *jc, int want_j) = 0;
-  // implicit-Schur PCG pieces (hsba_pcg.hip)
-  virtual void pcg_precond(const Dev<T> &d, hipStream_t st) = 0;
-  virtual void pcg_matvec(const Dev<T> &d, hipStream_t st, const double *v, double *out,
-                          int glog2) = 0;
+  // implicit-Schur PCG
+  virtual void pcg_setup(const Dev<T> &d, hipStream_t st, int glog2) = 0;
+  virtual void pcg_iteration(const Dev<T> &d, hipStream_t st, int glog2) = 0;
+  virtual void pcg_finish(const Dev<T> &d, hipStream_t st) = 0;
+  virtual void pcg_params(double rtol, int max_it) = 0;
 };
 
 template <typename T>
@@ -41,6 +42,8 @@ struct CholPlan {
   const WinInfo *wi;         // device
   WinState *ws;              // device
   double *S, *rhs, *dred, *Dinv;
+  double *Lf;                // factor of large windows (same layout as S)
+  double *bwork, *ywork;     // right-hand side being reduced / forward solution
   const WinInfo *wi_host;    // host copy (large-window launch loop)
 };
 void chol_init_padding(const CholPlan &plan, hipStream_t st);

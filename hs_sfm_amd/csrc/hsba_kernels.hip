@@ -277,23 +277,42 @@ __global__ void k_point_factor(Dev<T> d) {
   d.q[3 * p + 2] = T(i20 * g0 + i21 * g1 + i22 * g2);
 }
 
-// Z_k = L_p^-1 W_k, one thread per key.
+// Z_k = L_p^-1 W_k.  A workgroup owns a chunk of 64 consecutive keys: their
+// point factors are staged in LDS and every thread produces contiguous Z
+// elements, so both W reads and Z writes are coalesced.
 template <typename T, int E>
 __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= d.N) return;
-  const int p = d.key_pt[k];
-  if (!win_trial(d.ws[d.pt_win[p]])) return;
-  const T *L = d.Linv + 6 * p;
-  const T l00 = L[0], l10 = L[1], l11 = L[2], l20 = L[3], l21 = L[4], l22 = L[5];
-  const T *w = d.W + (int64_t)k * 3 * E;
-  T *z = d.Z + (int64_t)k * 3 * E;
+  constexpr int KC = 64;
+  constexpr int BLK = 3 * E;
+  __shared__ T lsh[KC][6];
+  __shared__ int act[KC];
+  const int k0 = blockIdx.x * KC;
+  if (threadIdx.x < KC) {
+    const int k = k0 + threadIdx.x;
+    int a = 0;
+    if (k < d.N) {
+      const int p = d.key_pt[k];
+      a = win_trial(d.ws[d.pt_win[p]]) ? 1 : 0;
+      if (a)
 #pragma unroll
-  for (int j = 0; j < E; ++j) {
-    const T w0 = w[j], w1 = w[E + j], w2 = w[2 * E + j];
-    z[j] = l00 * w0;
-    z[E + j] = l10 * w0 + l11 * w1;
-    z[2 * E + j] = l20 * w0 + l21 * w1 + l22 * w2;
+        for (int j = 0; j < 6; ++j) lsh[threadIdx.x][j] = d.Linv[6 * p + j];
+    }
+    act[threadIdx.x] = a;
+  }
+  __syncthreads();
+  const int nk = min(KC, d.N - k0);
+  const T *w = d.W + (int64_t)k0 * BLK;
+  T *z = d.Z + (int64_t)k0 * BLK;
+  for (int e = threadIdx.x; e < nk * BLK; e += blockDim.x) {
+    const int kl = e / BLK, rem = e - kl * BLK;
+    if (!act[kl]) continue;
+    const int a = rem / E, j = rem - a * E;
+    const T *wk = w + kl * BLK;
+    T v;
+    if (a == 0) v = lsh[kl][0] * wk[j];
+    else if (a == 1) v = lsh[kl][1] * wk[j] + lsh[kl][2] * wk[E + j];
+    else v = lsh[kl][3] * wk[j] + lsh[kl][4] * wk[E + j] + lsh[kl][5] * wk[2 * E + j];
+    z[e] = v;
   }
 }
 
@@ -636,12 +655,12 @@ __global__ void __launch_bounds__(256) k_grad_stats(Dev<T> d) {
 //   phase 3: after the candidate cost: gain ratio, accept / reject, mu update
 //   phase 4: after a relinearization: ||g||_inf <= eps1 -> stop
 template <typename T>
-__global__ void k_lm_control(Dev<T> d, int phase) {
-  const int w = blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= d.nwin) return;
+__global__ void __launch_bounds__(64) k_lm_control(Dev<T> d, int phase) {
+  const int w = blockIdx.x;
+  const int lane = threadIdx.x;
   WinState &s = d.ws[w];
   // chunk range of this window: chunks are sorted by window
-  int cb = 0, ce = d.nchunks;
+  int cb, ce;
   {
     int lo = 0, hi = d.nchunks;
     while (lo < hi) { const int m = (lo + hi) >> 1; if (d.chunk_win[m] < w) lo = m + 1; else hi = m; }
@@ -649,14 +668,26 @@ __global__ void k_lm_control(Dev<T> d, int phase) {
     while (lo < hi) { const int m = (lo + hi) >> 1; if (d.chunk_win[m] <= w) lo = m + 1; else hi = m; }
     ce = lo;
   }
+  // lane partial sums / maxima over the window's chunks, fixed butterfly
+  double F = 0.0, gm = 0.0, dm = -1e308, dn = 0.0, xn = 0.0, gd = 0.0;
+  for (int c = cb + lane; c < ce; c += 64) {
+    if (d.chunk_kind[c] == 3) F += d.partial[4 * c + 3];
+    else {
+      if (phase == 0 || phase == 4) { gm = fmax(gm, d.partial[4 * c]); dm = fmax(dm, d.partial[4 * c + 1]); }
+      if (phase == 2) { dn += d.partial[4 * c]; xn += d.partial[4 * c + 1]; gd += d.partial[4 * c + 2]; }
+    }
+  }
+  for (int m = 1; m < 64; m <<= 1) {
+    F += __shfl_xor(F, m, 64);
+    gm = fmax(gm, __shfl_xor(gm, m, 64));
+    dm = fmax(dm, __shfl_xor(dm, m, 64));
+    dn += __shfl_xor(dn, m, 64);
+    xn += __shfl_xor(xn, m, 64);
+    gd += __shfl_xor(gd, m, 64);
+  }
+  if (lane != 0) return;
   if (phase == 0) {
     if (!s.active) return;
-    double F = 0.0, gm = 0.0, dm = -1e308;
-    for (int c = cb; c < ce; ++c) {
-      const int k = d.chunk_kind[c];
-      if (k == 3) F += d.partial[4 * c + 3];
-      else { gm = fmax(gm, d.partial[4 * c]); dm = fmax(dm, d.partial[4 * c + 1]); }
-    }
     s.cost = F + s.cost_new;  // cost_new holds the constraint part
     s.cost0 = s.cost;
     s.ginf = gm; s.maxdiag = dm;
@@ -680,10 +711,6 @@ __global__ void k_lm_control(Dev<T> d, int phase) {
     if (s.chol_fail) {
       s.status = -3; s.stop_reason = 3; s.active = 0; s.trial = 0; return;
     }
-    double dn = 0.0, xn = 0.0, gd = 0.0;
-    for (int c = cb; c < ce; ++c) {
-      if (d.chunk_kind[c] < 3) { dn += d.partial[4 * c]; xn += d.partial[4 * c + 1]; gd += d.partial[4 * c + 2]; }
-    }
     s.dnorm2 = dn; s.xnorm2 = xn; s.gain_den = gd;
     if (!(dn == dn)) { s.status = -4; s.stop_reason = 3; s.active = 0; s.trial = 0; return; }
     if (sqrt(dn) <= s.eps2 * (sqrt(xn) + s.eps2)) {
@@ -692,9 +719,6 @@ __global__ void k_lm_control(Dev<T> d, int phase) {
     s.take_step = 1;
   } else if (phase == 3) {
     if (!(s.active && s.trial && s.take_step)) return;
-    double F = 0.0;
-    for (int c = cb; c < ce; ++c)
-      if (d.chunk_kind[c] == 3) F += d.partial[4 * c + 3];
     F += s.cost_new;  // constraint part, written by k_constraint_cost
     s.cost_new = F;
     const double rho = (s.cost - F) / s.gain_den;
@@ -714,9 +738,6 @@ __global__ void k_lm_control(Dev<T> d, int phase) {
     }
   } else if (phase == 4) {
     if (!(s.active && s.need_lin)) return;
-    double gm = 0.0;
-    for (int c = cb; c < ce; ++c)
-      if (d.chunk_kind[c] < 3) gm = fmax(gm, d.partial[4 * c]);
     s.ginf = gm;
     s.need_lin = 0;
     if (gm <= s.eps1) { s.stop_reason = 1; s.active = 0; }
@@ -794,6 +815,271 @@ __global__ void k_export_keys(Dev<T> d, double *y, double *jp, double *ji, doubl
       for (int m = 0; m < D; ++m) jc[o * 2 * D + a * D + m] = Jc[a][m];
 }
 
+
+// ------------------------------------------------------------ K9 PCG
+// Block-Jacobi preconditioned conjugate gradients on the implicit reduced
+// system S = U + mu I - sum_p Z_p^T Z_p (no reference counterpart: the
+// reference only has the direct SimplicialLDLT; Ceres' ITERATIVE_SCHUR is
+// commented out at camera_shared_ceres_optimizor.hpp:476).  Every image owns
+// one super-block of E rows (its 6 image rows and the D rows of its camera;
+// one image per camera), which is also the preconditioner block.  FP32 Z,
+// FP64 vectors, dot products and recurrences.
+
+__device__ __forceinline__ bool pcg_live(const WinState &s, const PcgState &g) {
+  return s.active && s.trial && !s.chol_fail && !g.done;
+}
+
+template <int IMG, int D>
+__device__ __forceinline__ int sup_row(const int *img_row, const int *cam_row, int cam, int im, int j) {
+  return j < IMG ? img_row[im] + j : cam_row[cam] + (j - IMG);
+}
+
+// Preconditioner blocks M_i = [D_i with the camera block of U_c] + mu I - P_ii,
+// factored M_i = L L^T in FP64 registers (one thread per image).
+template <typename T, int IMG, int D>
+__global__ void k_pcg_precond(Dev<T> d) {
+  constexpr int E = IMG + D;
+  const int im = blockIdx.x * blockDim.x + threadIdx.x;
+  if (im >= d.I) return;
+  WinState &s = d.ws[d.img_win[im]];
+  if (!(s.active && s.trial) || s.chol_fail) return;
+  const int c = d.img_cam[im];
+  double M[E > 0 ? E : 1][E > 0 ? E : 1];
+  const T *Di = d.Dimg + (int64_t)im * E * E;
+  const T *P = d.pair_out + (int64_t)im * E * E;  // diagonal pair of image im
+#pragma unroll
+  for (int a = 0; a < E; ++a)
+#pragma unroll
+    for (int b = 0; b < E; ++b) {
+      double v;
+      if (a >= IMG && b >= IMG) v = (double)d.Uc[(int64_t)c * (D > 0 ? D : 1) * D + (a - IMG) * D + (b - IMG)];
+      else v = (double)Di[a * E + b];
+      M[a][b] = v - (d.pts_free ? (double)P[a * E + b] : 0.0) + (a == b ? s.mu : 0.0);
+    }
+  bool ok = true;
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    double djj = M[j][j];
+#pragma unroll
+    for (int m = 0; m < j; ++m) djj -= M[j][m] * M[j][m];
+    ok = ok && djj > 0.0;
+    const double ljj = sqrt(djj > 0.0 ? djj : 1.0);
+    M[j][j] = ljj;
+#pragma unroll
+    for (int i = j + 1; i < E; ++i) {
+      double v = M[i][j];
+#pragma unroll
+      for (int m = 0; m < j; ++m) v -= M[i][m] * M[j][m];
+      M[i][j] = v / ljj;
+    }
+  }
+  if (!ok) s.chol_fail = 1;
+  double *out = d.Minv + (int64_t)im * E * E;
+#pragma unroll
+  for (int a = 0; a < E; ++a)
+#pragma unroll
+    for (int b = 0; b < E; ++b) out[a * E + b] = b <= a ? M[a][b] : 0.0;
+}
+
+// z = M^-1 r per super-block (init = 1 also sets x = 0 and p = z).
+template <typename T, int IMG, int D>
+__global__ void k_pcg_apply(Dev<T> d, int init) {
+  constexpr int E = IMG + D;
+  const int im = blockIdx.x * blockDim.x + threadIdx.x;
+  if (im >= d.I) return;
+  const int w = d.img_win[im];
+  if (!pcg_live(d.ws[w], d.pcg[w])) return;
+  const int c = d.img_cam[im];
+  const double *L = d.Minv + (int64_t)im * E * E;
+  double v[E > 0 ? E : 1];
+#pragma unroll
+  for (int j = 0; j < E; ++j) v[j] = d.pr[sup_row<IMG, D>(d.img_row, d.cam_row, c, im, j)];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+#pragma unroll
+    for (int m = 0; m < i; ++m) v[i] -= L[i * E + m] * v[m];
+    v[i] /= L[i * E + i];
+  }
+#pragma unroll
+  for (int i = E - 1; i >= 0; --i) {
+#pragma unroll
+    for (int m = i + 1; m < E; ++m) v[i] -= L[m * E + i] * v[m];
+    v[i] /= L[i * E + i];
+  }
+#pragma unroll
+  for (int j = 0; j < E; ++j) {
+    const int r = sup_row<IMG, D>(d.img_row, d.cam_row, c, im, j);
+    d.pz[r] = v[j];
+    if (init) { d.pp[r] = v[j]; d.dred[r] = 0.0; }
+  }
+}
+
+// t_p = sum_{b in p} Z_b v_sup(b), one lane group per point.
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(256) k_pcg_points(Dev<T> d, int glog2) {
+  constexpr int E = IMG + D;
+  const int G = 1 << glog2;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = gid >> glog2;
+  const int lane = gid & (G - 1);
+  bool act = false;
+  if (p < d.P) {
+    const int w = d.pt_win[p];
+    act = pcg_live(d.ws[w], d.pcg[w]);
+  }
+  double t[3] = {0.0, 0.0, 0.0};
+  if (act) {
+    for (int k = d.pt_ptr[p] + lane; k < d.pt_ptr[p + 1]; k += G) {
+      const int im = d.key_img[k], c = d.img_cam[im];
+      const T *z = d.Z + (int64_t)k * 3 * E;
+      double v[E > 0 ? E : 1];
+#pragma unroll
+      for (int j = 0; j < E; ++j) v[j] = d.pp[sup_row<IMG, D>(d.img_row, d.cam_row, c, im, j)];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < E; ++j) t[a] += (double)z[a * E + j] * v[j];
+    }
+  }
+  for (int m = 1; m < G; m <<= 1)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) t[a] += __shfl_xor(t[a], m, 64);
+  if (act && lane == 0)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) d.tpd[3 * p + a] = t[a];
+}
+
+// Ap_i = (U_i + mu I) p_i - sum_{a in i} Z_a^T t_p(a), one workgroup per image.
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(128) k_pcg_images(Dev<T> d) {
+  constexpr int E = IMG + D;
+  const int im = blockIdx.x;
+  const int w = d.img_win[im];
+  if (!pcg_live(d.ws[w], d.pcg[w])) return;
+  const int c = d.img_cam[im];
+  double acc[E > 0 ? E : 1];
+#pragma unroll
+  for (int j = 0; j < E; ++j) acc[j] = 0.0;
+  for (int t = d.img_ptr[im] + threadIdx.x; t < d.img_ptr[im + 1]; t += blockDim.x) {
+    const int k = d.img_keys[t];
+    const int p = d.key_pt[k];
+    const T *z = d.Z + (int64_t)k * 3 * E;
+    const double t0 = d.tpd[3 * p], t1 = d.tpd[3 * p + 1], t2 = d.tpd[3 * p + 2];
+#pragma unroll
+    for (int j = 0; j < E; ++j) acc[j] += (double)z[j] * t0 + (double)z[E + j] * t1 + (double)z[2 * E + j] * t2;
+  }
+  for (int m = 1; m < 64; m <<= 1)
+#pragma unroll
+    for (int j = 0; j < E; ++j) acc[j] += __shfl_xor(acc[j], m, 64);
+  __shared__ double red[2][E > 0 ? E : 1];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+#pragma unroll
+    for (int j = 0; j < E; ++j) red[wave][j] = acc[j];
+  __syncthreads();
+  if (threadIdx.x < E) {
+    const int a = threadIdx.x;
+    const T *Di = d.Dimg + (int64_t)im * E * E;
+    double y = 0.0;
+    for (int b = 0; b < E; ++b) {
+      double u;
+      if (a >= IMG && b >= IMG) u = (double)d.Uc[(int64_t)c * (D > 0 ? D : 1) * D + (a - IMG) * D + (b - IMG)];
+      else u = (double)Di[a * E + b];
+      y += u * d.pp[sup_row<IMG, D>(d.img_row, d.cam_row, c, im, b)];
+    }
+    const int r = sup_row<IMG, D>(d.img_row, d.cam_row, c, im, a);
+    y += d.ws[w].mu * d.pp[r];
+    d.pAp[r] = y - (d.pts_free ? (red[0][a] + red[1][a]) : 0.0);
+  }
+}
+
+// Per-window dot products (one workgroup per window, fixed tree):
+//   mode 0: rz = r.z, bnorm2 = rr = r.r (start)      mode 1: alpha = rz / p.Ap
+//   mode 2: rz' = r.z, rr = r.r, beta, convergence, it++
+template <typename T>
+__global__ void __launch_bounds__(256) k_pcg_dot(Dev<T> d, int mode) {
+  const int w = blockIdx.x;
+  const WinInfo &wi = d.wi[w];
+  PcgState &g = d.pcg[w];
+  if (!pcg_live(d.ws[w], g)) return;
+  double a0 = 0.0, a1 = 0.0;
+  for (int i = threadIdx.x; i < wi.n; i += blockDim.x) {
+    const int r = wi.r_off + i;
+    if (mode == 1) a0 += d.pp[r] * d.pAp[r];
+    else { a0 += d.pr[r] * d.pz[r]; a1 += d.pr[r] * d.pr[r]; }
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) { red[0][threadIdx.x] += red[0][threadIdx.x + h]; red[1][threadIdx.x] += red[1][threadIdx.x + h]; }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    g.rz = red[0][0]; g.rr = red[1][0]; g.bnorm2 = red[1][0];
+    if (!(g.bnorm2 > 0.0)) g.done = 1;
+  } else if (mode == 1) {
+    g.pap = red[0][0];
+    g.alpha = g.pap > 0.0 ? g.rz / g.pap : 0.0;
+    if (!(g.pap > 0.0)) g.done = 1;  // exact solution reached or loss of definiteness
+  } else {
+    g.rr = red[1][0];
+    g.beta = g.rz > 0.0 ? red[0][0] / g.rz : 0.0;
+    g.rz = red[0][0];
+    g.it += 1;
+    if (g.rr <= g.rtol2 * g.bnorm2 || g.it >= g.max_it) g.done = 1;
+  }
+}
+
+// mode 0: x += alpha p, r -= alpha Ap        mode 1: p = z + beta p
+// mode 2 (start): r = rhs
+template <typename T>
+__global__ void k_pcg_vec(Dev<T> d, int mode, int rows) {
+  const int R = blockIdx.x * blockDim.x + threadIdx.x;
+  if (R >= rows) return;
+  int lo = 0, hi = d.nwin - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.wi[mid].r_off <= R) lo = mid; else hi = mid - 1;
+  }
+  const WinInfo &wi = d.wi[lo];
+  if (R - wi.r_off >= wi.n) return;
+  const PcgState &g = d.pcg[lo];
+  if (!pcg_live(d.ws[lo], g)) return;
+  if (mode == 0) {
+    d.dred[R] += g.alpha * d.pp[R];
+    d.pr[R] -= g.alpha * d.pAp[R];
+  } else if (mode == 1) {
+    d.pp[R] = d.pz[R] + g.beta * d.pp[R];
+  } else {
+    d.pr[R] = d.rhs[R];
+  }
+}
+
+// Reset the solver state of every window at the start of a trial.
+template <typename T>
+__global__ void k_pcg_reset(Dev<T> d, double rtol2, int max_it) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= d.nwin) return;
+  PcgState &g = d.pcg[w];
+  g.rz = g.pap = g.alpha = g.beta = g.bnorm2 = g.rr = 0.0;
+  g.rtol2 = rtol2;
+  g.it = 0;
+  g.max_it = max_it;
+  const WinState &s = d.ws[w];
+  g.done = (s.active && s.trial && !s.chol_fail) ? 0 : 1;
+}
+
+// Adds the PCG iteration count of the trial to the window statistics.
+template <typename T>
+__global__ void k_pcg_finish(Dev<T> d) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= d.nwin) return;
+  WinState &s = d.ws[w];
+  if (s.active && s.trial) s.pcg_iters += d.pcg[w].it;
+}
+
 // ------------------------------------------------------------ launchers
 template <typename T, int IMG, int D>
 struct KernelSetImpl final : KernelSet<T> {
@@ -822,7 +1108,7 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void key_z(const Dev<T> &d, hipStream_t st) override {
     if (!d.pts_free || E == 0 || d.N == 0) return;
-    hipLaunchKernelGGL((k_key_z<T, (E > 0 ? E : 1)>), dim3((d.N + 255) / 256), dim3(256), 0, st, d);
+    hipLaunchKernelGGL((k_key_z<T, (E > 0 ? E : 1)>), dim3((d.N + 63) / 64), dim3(256), 0, st, d);
   }
   void pairs(const Dev<T> &d, hipStream_t st) override {
     if (!d.pts_free || E == 0 || d.npairs == 0) return;
@@ -830,8 +1116,9 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) override {
     if (E == 0 || rows == 0) return;
-    hipLaunchKernelGGL((k_assemble<T, IMG, D>), dim3((lower_total + 255) / 256), dim3(256), 0, st, d,
-                       lower_total);
+    if (lower_total > 0)
+      hipLaunchKernelGGL((k_assemble<T, IMG, D>), dim3((lower_total + 255) / 256), dim3(256), 0, st, d,
+                         lower_total);
     hipLaunchKernelGGL((k_assemble_rhs<T, IMG, D>), dim3((rows + 255) / 256), dim3(256), 0, st, d, rows);
   }
   void backsub(const Dev<T> &d, hipStream_t st, int glog2) override {
@@ -850,7 +1137,7 @@ struct KernelSetImpl final : KernelSet<T> {
     hipLaunchKernelGGL((k_grad_stats<T, IMG, D>), dim3(d.nchunks), dim3(256), 0, st, d);
   }
   void lm_control(const Dev<T> &d, hipStream_t st, int phase) override {
-    hipLaunchKernelGGL((k_lm_control<T>), dim3((d.nwin + 63) / 64), dim3(64), 0, st, d, phase);
+    hipLaunchKernelGGL((k_lm_control<T>), dim3(d.nwin), dim3(64), 0, st, d, phase);
   }
   void commit(const Dev<T> &d, hipStream_t st) override {
     const int64_t n = (d.pts_free ? 3LL * d.P : 0) + (d.imgs_free ? 6LL * d.I : 0) +
@@ -858,8 +1145,40 @@ struct KernelSetImpl final : KernelSet<T> {
     if (n == 0) return;
     hipLaunchKernelGGL((k_commit<T, IMG, D>), dim3((n + 255) / 256), dim3(256), 0, st, d);
   }
-  void pcg_precond(const Dev<T> &d, hipStream_t st) override {}
-  void pcg_matvec(const Dev<T> &d, hipStream_t st, const double *v, double *out, int glog2) override {}
+  double pcg_rtol = 1e-6;
+  int pcg_max_it = 100;
+  void pcg_setup(const Dev<T> &d, hipStream_t st, int glog2) override {
+    const int rows = (int)d.pcg_rows;
+    hipLaunchKernelGGL((k_pcg_reset<T>), dim3((d.nwin + 63) / 64), dim3(64), 0, st, d, pcg_rtol * pcg_rtol,
+                       pcg_max_it);
+    if (E == 0 || rows == 0) return;
+    hipLaunchKernelGGL((k_pcg_precond<T, IMG, D>), dim3((d.I + 63) / 64), dim3(64), 0, st, d);
+    hipLaunchKernelGGL((k_pcg_reset<T>), dim3((d.nwin + 63) / 64), dim3(64), 0, st, d, pcg_rtol * pcg_rtol,
+                       pcg_max_it);
+    hipLaunchKernelGGL((k_pcg_vec<T>), dim3((rows + 255) / 256), dim3(256), 0, st, d, 2, rows);
+    hipLaunchKernelGGL((k_pcg_apply<T, IMG, D>), dim3((d.I + 63) / 64), dim3(64), 0, st, d, 1);
+    hipLaunchKernelGGL((k_pcg_dot<T>), dim3(d.nwin), dim3(256), 0, st, d, 0);
+  }
+  void pcg_iteration(const Dev<T> &d, hipStream_t st, int glog2) override {
+    const int rows = (int)d.pcg_rows;
+    if (E == 0 || rows == 0) return;
+    const int64_t threads = (int64_t)d.P << glog2;
+    if (d.pts_free && d.P)
+      hipLaunchKernelGGL((k_pcg_points<T, IMG, D>), dim3((threads + 255) / 256), dim3(256), 0, st, d, glog2);
+    hipLaunchKernelGGL((k_pcg_images<T, IMG, D>), dim3(d.I), dim3(128), 0, st, d);
+    hipLaunchKernelGGL((k_pcg_dot<T>), dim3(d.nwin), dim3(256), 0, st, d, 1);
+    hipLaunchKernelGGL((k_pcg_vec<T>), dim3((rows + 255) / 256), dim3(256), 0, st, d, 0, rows);
+    hipLaunchKernelGGL((k_pcg_apply<T, IMG, D>), dim3((d.I + 63) / 64), dim3(64), 0, st, d, 0);
+    hipLaunchKernelGGL((k_pcg_dot<T>), dim3(d.nwin), dim3(256), 0, st, d, 2);
+    hipLaunchKernelGGL((k_pcg_vec<T>), dim3((rows + 255) / 256), dim3(256), 0, st, d, 1, rows);
+  }
+  void pcg_finish(const Dev<T> &d, hipStream_t st) override {
+    hipLaunchKernelGGL((k_pcg_finish<T>), dim3((d.nwin + 63) / 64), dim3(64), 0, st, d);
+  }
+  void pcg_params(double rtol, int max_it) override {
+    pcg_rtol = rtol;
+    pcg_max_it = max_it;
+  }
   void export_keys(const Dev<T> &d, hipStream_t st, double *y, double *jp, double *ji, double *jc,
                    int want_j) override {
     if (d.N == 0) return;

@@ -51,9 +51,10 @@ void add_chunks(Layout &L, int kind, int64_t begin, int64_t end, int w) {
 
 }  // namespace
 
-int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::string *why) {
+int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::string *why, bool pcg) {
   if (!descs || count <= 0) return fail(why, "no problems");
   L = Layout();
+  L.pcg = pcg;
   L.nwin = (int)count;
   const hsba_problem_desc &d0 = descs[0];
   L.comp = d0.intrinsic_computations_mask & 7u;
@@ -191,10 +192,23 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
     wi.lo_off = lo_off;
     wi.blk_off = blk_off;
     wi.small = wi.nb <= 4 ? 1 : 0;
-    if (wi.n > 0) (wi.small ? L.small_list : L.large_list).push_back((int)w);
+    if (pcg) {
+      wi.s_off = 0;
+      wi.dinv_off = 0;
+      wi.lo_off = 0;
+      wi.blk_off = 0;
+    } else if (wi.n > 0) {
+      (wi.small ? L.small_list : L.large_list).push_back((int)w);
+    }
     for (int64_t i = 0; i < h.I && IMG; ++i) L.img_row[img_off + i] = (int32_t)(r_off + 6 * i);
     for (int64_t c = 0; c < h.C && D; ++c) L.cam_row[cam_off + c] = (int32_t)(r_off + wi.cb + D * c);
-    const int64_t nblk = (IMG ? h.I : 0) + (D ? h.C : 0);
+    const int64_t nblk = pcg ? 0 : (IMG ? h.I : 0) + (D ? h.C : 0);
+    if (pcg && D) {
+      std::vector<int> per_cam(h.C, 0);
+      for (int64_t i = 0; i < h.I; ++i) per_cam[ds.image_camera_map[i]]++;
+      for (int64_t c = 0; c < h.C; ++c)
+        if (per_cam[c] != 1) return fail(why, "PCG needs one image per camera", HSBA_ERR_UNSUPPORTED);
+    }
     auto bimg = [&](int64_t i) { return i; };
     auto bcam = [&](int64_t c) { return (IMG ? h.I : 0) + c; };
     L.row_blk.resize(r_off + wi.ld, 0);
@@ -221,9 +235,9 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
     auto add = [&](int64_t bx, int64_t by, Contrib c) {
       blocks[(size_t)(bx * (bx + 1) / 2 + by)].push_back(c);
     };
-    for (int64_t i = 0; i < h.I && IMG; ++i) add(bimg(i), bimg(i), Contrib{0, (int32_t)(img_off + i), 0, 0, 0});
-    for (int64_t c = 0; c < h.C && D; ++c) add(bcam(c), bcam(c), Contrib{1, (int32_t)(cam_off + c), 0, 0, 0});
-    if (IMG && D)
+    for (int64_t i = 0; i < h.I && IMG && !pcg; ++i) add(bimg(i), bimg(i), Contrib{0, (int32_t)(img_off + i), 0, 0, 0});
+    for (int64_t c = 0; c < h.C && D && !pcg; ++c) add(bcam(c), bcam(c), Contrib{1, (int32_t)(cam_off + c), 0, 0, 0});
+    if (IMG && D && !pcg)
       for (int64_t i = 0; i < h.I; ++i)
         add(bcam(ds.image_camera_map[i]), bimg(i), Contrib{2, (int32_t)(img_off + i), 0, 0, 0});
     if (L.pts_free && L.E > 0 && h.I > 0) {
@@ -232,7 +246,7 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
       for (int64_t p = 0; p < h.P; ++p) {
         const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
         for (int64_t u = b; u < e; ++u)
-          for (int64_t v = u; v < e; ++v)
+          for (int64_t v = u; v < (pcg ? u + 1 : e); ++v)
             cnt[(size_t)((L.key_img[u] - img_off) * h.I + (L.key_img[v] - img_off))]++;
       }
       const int64_t first_pair = L.npairs;
@@ -257,7 +271,7 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
             ys[ny++] = {bcam(ds.image_camera_map[ib]), (int16_t)IMG};
           }
           const int32_t pg = (int32_t)(L.npairs - 1);
-          for (int a = 0; a < nx; ++a)
+          for (int a = 0; a < nx && !pcg; ++a)
             for (int b2 = 0; b2 < ny; ++b2) {
               const Side X = xs[a], Y = ys[b2];
               if (X.blk >= Y.blk) add(X.blk, Y.blk, Contrib{3, pg, X.off, Y.off, 0});
@@ -268,7 +282,7 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
       for (int64_t p = 0; p < h.P; ++p) {
         const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
         for (int64_t u = b; u < e; ++u)
-          for (int64_t v = u; v < e; ++v) {
+          for (int64_t v = u; v < (pcg ? u + 1 : e); ++v) {
             const int32_t q = pid[(size_t)((L.key_img[u] - img_off) * h.I + (L.key_img[v] - img_off))];
             L.inc[cursor[q - first_pair]++] = (uint32_t)((u << 7) | (v - u));
           }
@@ -325,9 +339,11 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
     // ---- advance offsets
     pt_off += h.P; img_off += h.I; cam_off += h.C; key_off += h.N;
     r_off += wi.ld;
-    s_off += (int64_t)wi.ld * wi.ld;
-    dinv_off += (int64_t)wi.nb * 64 * 64;
-    lo_off += (int64_t)wi.n * (wi.n + 1) / 2;
+    if (!pcg) {
+      s_off += (int64_t)wi.ld * wi.ld;
+      dinv_off += (int64_t)wi.nb * 64 * 64;
+      lo_off += (int64_t)wi.n * (wi.n + 1) / 2;
+    }
     blk_off += nblk * (nblk + 1) / 2;
   }
   L.pt_ptr[0] = 0;
@@ -345,6 +361,12 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
     for (int64_t i = 0; i < L.I; ++i) L.cam_imgs[fill[L.img_cam[i]]++] = (int32_t)i;
   }
   L.pair_ptr.push_back(L.incidences);
+  if (pcg && L.pts_free && L.E > 0) {
+    // diagonal pairs in image order: pair id == image id
+    if (L.npairs != L.I) return fail(why, "PCG needs every image observed");
+    for (int64_t i = 0; i < L.I; ++i)
+      if (L.pair_img[i] != (int32_t)i) return fail(why, "PCG pair order");
+  }
   L.S_total = s_off; L.rows_total = r_off; L.dinv_total = dinv_off; L.lower_total = lo_off;
   if (L.row_blk.empty()) { L.row_blk.push_back(0); L.row_sub.push_back(0); }
   return HSBA_OK;

@@ -121,15 +121,26 @@ def angle_axis_to_matrix(r: np.ndarray) -> np.ndarray:
 
 
 def matrix_to_angle_axis(R: np.ndarray) -> np.ndarray:
-    """Logarithm of rotation matrices [..., 3, 3] (valid for angles < pi - 1e-6)."""
+    """Logarithm of rotation matrices [n, 3, 3] -> angle-axis [n, 3], including
+    angles at or near pi (axis from the symmetric part (R + I) / 2 = k k^T)."""
     R = np.asarray(R, dtype=np.float64)
+    shape = R.shape[:-2]
+    R = R.reshape(-1, 3, 3)
     cos_t = np.clip((np.trace(R, axis1=-2, axis2=-1) - 1) / 2, -1.0, 1.0)
     th = np.arccos(cos_t)
-    w = np.stack([R[..., 2, 1] - R[..., 1, 2], R[..., 0, 2] - R[..., 2, 0],
-                  R[..., 1, 0] - R[..., 0, 1]], axis=-1)
+    w = np.stack([R[:, 2, 1] - R[:, 1, 2], R[:, 0, 2] - R[:, 2, 0], R[:, 1, 0] - R[:, 0, 1]], axis=-1)
     s = np.sin(th)
-    scale = np.where(s > 1e-12, th / (2 * np.where(s > 1e-12, s, 1.0)), 0.5)
-    return w * scale[..., None]
+    out = w * np.where(s > 1e-12, th / (2 * np.where(s > 1e-12, s, 1.0)), 0.5)[:, None]
+    near_pi = th > np.pi - 1e-3
+    for n in np.nonzero(near_pi)[0]:
+        B = (R[n] + np.eye(3)) / 2
+        j = int(np.argmax(np.diag(B)))
+        k = B[:, j] / np.sqrt(max(B[j, j], 1e-300))
+        k /= np.linalg.norm(k)
+        if np.dot(k, w[n]) < 0:
+            k = -k
+        out[n] = th[n] * k
+    return out.reshape(shape + (3,))
 
 
 def look_at(centers: np.ndarray, target: np.ndarray, up=(0.0, 0.0, 1.0)):

@@ -67,6 +67,10 @@ inline T __shfl_xor(T v, int mask, int width = 64) {
 using std::fabs;
 using std::fmax;
 using std::sqrt;
+inline int min(int a, int b) { return a < b ? a : b; }
+inline int max(int a, int b) { return a > b ? a : b; }
+inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 
 typedef enum hipError_t { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 } hipError_t;
 typedef struct ihipStream_t *hipStream_t;

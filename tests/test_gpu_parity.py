@@ -151,3 +151,40 @@ def test_batch_matches_single(gpu):
         _, xo, st_o, _ = o.lm(max_iterations=8)
         assert abs(st.final_cost - st_o.final_cost) < 1e-7 * st_o.final_cost
         assert abs(syn.reprojection_rmse(s, x) - syn.reprojection_rmse(s, xo)) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [abi.PRECISION_MIXED])
+def test_pcg_step_matches_direct(gpu, prec):
+    """Block-Jacobi PCG on the implicit Schur complement vs the oracle's direct
+    solve (banded trajectory scene)."""
+    s = syn.large_scene(num_cameras=40, num_points=600)
+    o = Oracle(s)
+    o.build_normal(s.x0)
+    mu = 1e-3 * o.max_diagonal()
+    _, do = o.solve(mu)
+    p = Problem(s, precision=prec, solver=abi.SOLVER_PCG, pcg_rtol=1e-10)
+    p.build_normal(s.x0)
+    rc, dg = p.solve(mu)
+    assert rc == 0
+    assert rel(dg, do) < 1e-4
+
+
+def test_pcg_lm(gpu):
+    s = syn.large_scene(num_cameras=60, num_points=2000)
+    o = Oracle(s)
+    p = Problem(s, precision=abi.PRECISION_MIXED, solver=abi.SOLVER_PCG, pcg_rtol=1e-6)
+    rc, x, st = p.lm(max_iterations=10)
+    _, xo, sto, _ = o.lm(max_iterations=10)
+    assert rc == 0 and st.pcg_iterations > 0
+    assert abs(syn.reprojection_rmse(s, x) - syn.reprojection_rmse(s, xo)) < 1e-3
+
+
+def test_pcg_fp64_fixed_cameras(gpu):
+    """FP64 PCG (camera block width 6) on the ring scene vs the direct path."""
+    s = syn.ring_scene()
+    o = Oracle(s)
+    p = Problem(s, precision=abi.PRECISION_FP64, solver=abi.SOLVER_PCG, pcg_rtol=1e-9)
+    rc, x, st = p.lm(max_iterations=10)
+    _, xo, sto, _ = o.lm(max_iterations=10)
+    assert rc == 0
+    assert abs(st.final_cost - sto.final_cost) < 1e-6 * sto.final_cost
