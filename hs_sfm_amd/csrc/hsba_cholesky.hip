@@ -6,8 +6,9 @@
 // ld (a multiple of 64), lower triangle valid, padding rows / columns set to
 // the identity once.  64 x 64 tiles are staged through LDS.
 //
-// Diagonal tiles: one elimination sweep of [A | I] (one barrier per column)
-// yields both L_kk and L_kk^-1; panel tiles are then GEMMs with L_kk^-1.
+// Diagonal tiles: a blocked factorization with 16-wide panels (each 16 x 16
+// diagonal block eliminated by one wave without workgroup barriers) yields
+// both L_kk and L_kk^-1; panel tiles are then GEMMs with L_kk^-1.
 //
 // Large windows (more than 4 tiles): one fused launch per panel step k.
 // Every workgroup of step k owns one trailing tile (i, j), k < j <= i: it forms
@@ -43,15 +44,6 @@ __device__ __forceinline__ void store_tile(const double (*t)[LDT], double *A, in
   }
 }
 
-// Factor a 64 x 64 SPD tile (lower triangle of src, row pitch lds) and its
-// inverse in one elimination sweep of [A | I] (256 threads):
-//   step j:  d_j = a[j][j];  a[i][c] -= a[i][j] a[c][j] / d_j   (j < c <= i)
-//            x[i][c] -= (a[i][j] / d_j) x[j][c]                  (c <= j < i)
-// leaves a[i][j] = L_unit[i][j] d_j and x = L_unit^-1, so L = L_unit D^1/2 and
-// L^-1 = D^-1/2 x.  Thread t owns column c = t % 64, rows r0 + 4m
-// (r0 = t / 64, m < 16) of both a and x in registers; each step broadcasts
-// column j of a and row j of x through double-buffered LDS, one barrier per
-// column.  Writes L (full tile, zero upper) to dstL and L^-1 to dstI.
 // 1/d for d > 0: hardware reciprocal estimate refined by two Newton–Raphson
 // steps r <- r (2 - d r) (quadratic convergence, so ~1 ulp).
 __device__ __forceinline__ double recip_nr(double d) {
@@ -61,71 +53,187 @@ __device__ __forceinline__ double recip_nr(double d) {
   return r;
 }
 
-__device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dstL, int64_t ldl,
-                                                    double *dstI, int64_t ldi, int *fail) {
-  __shared__ double acol[2][NB];
-  __shared__ double xrow[2][NB];
-  __shared__ double dsh[NB];
-  const int tid = threadIdx.x;
-  const int c = tid & 63, r0 = tid >> 6;
-  double a[16], x[16];
+// Lanes of one wave see each other's earlier LDS writes after this point: the
+// LDS unit executes one wave's instructions in issue order, and the fences
+// keep the compiler from moving LDS accesses across the call.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int PB = 16;  // panel width inside a 64 x 64 tile
+
+struct FactorScratch {
+  double col[2][PB];  // column j of the diagonal block being eliminated
+  double xrow[2][PB];  // row j of its inverse
+  double d[PB];        // pivots
+};
+
+// One wave: factor the 16 x 16 diagonal block of wa at (c0, c0) (lower
+// triangle, SPD) in place into its Cholesky factor L_p, and write L_p^-1 (lower,
+// zero upper) to the same block of wx.  Elimination of [A | I] in LDL^T form:
+//   step j:  d_j = a[j][j];  a[i][c] -= a[i][j] a[c][j] / d_j   (j < c <= i)
+//            x[i][c] -= (a[i][j] / d_j) x[j][c]                  (c <= j < i)
+// leaves a[i][j] = L_unit[i][j] d_j and x = L_unit^-1, so L = L_unit D^1/2 and
+// L^-1 = D^-1/2 x.  Lane l owns column c = l % 16, rows l / 16 + 4m (m < 4);
+// column j of a and row j of x are broadcast through LDS without a barrier.
+__device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int c0, FactorScratch &fs,
+                                       bool &bad) {
+  const int l = threadIdx.x & 63;
+  const int c = l & 15, r0 = l >> 4;
+  double a[4], x[4];
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
+  for (int m = 0; m < 4; ++m) {
     const int i = r0 + 4 * m;
-    a[m] = (i >= c) ? src[(int64_t)i * lds + c] : 0.0;
+    a[m] = (i >= c) ? wa[c0 + i][c0 + c] : 0.0;
     x[m] = (i == c) ? 1.0 : 0.0;
   }
-  bool bad = false;
-#pragma unroll 1
-  for (int j = 0; j < NB; ++j) {
-    const int buf = j & 1;
-    // publish column j of a (owners: c == j) and row j of x (owners: r0 == j % 4)
+#pragma unroll
+  for (int j = 0; j < PB; ++j) {
+    const int b = j & 1;
     if (c == j) {
 #pragma unroll
-      for (int m = 0; m < 16; ++m) acol[buf][r0 + 4 * m] = a[m];
+      for (int m = 0; m < 4; ++m) fs.col[b][r0 + 4 * m] = a[m];
     }
-    if (r0 == (j & 3)) {
-      double v = x[0];
+    if (r0 == (j & 3)) fs.xrow[b][c] = x[j >> 2];
+    wave_sync();
+    double aij[4];
 #pragma unroll
-      for (int m = 1; m < 16; ++m) v = (m == (j >> 2)) ? x[m] : v;
-      xrow[buf][c] = v;
-    }
-    __syncthreads();
-    // every read of this step first, then branch-free updates
-    double aij[16];
-#pragma unroll
-    for (int m = 0; m < 16; ++m) aij[m] = acol[buf][r0 + 4 * m];
-    const double d = acol[buf][j];
-    const double acj_raw = acol[buf][c];
-    const double xjc = xrow[buf][c];
-    if (tid == 0) dsh[j] = d;
+    for (int m = 0; m < 4; ++m) aij[m] = fs.col[b][r0 + 4 * m];
+    const double d = fs.col[b][j];
+    const double acj = fs.col[b][c];
+    const double xjc = fs.xrow[b][c];
+    if (l == 0) fs.d[j] = d;
     const bool ok = d > 0.0;
-    if (!ok) bad = true;
+    bad = bad || !ok;
     const double inv_d = ok ? recip_nr(d) : 0.0;
-    const double acj = acj_raw * inv_d;       // a[c][j] / d
-    const double xs = xjc * inv_d;            // x[j][c] / d
+    const double as = acj * inv_d;  // a[c][j] / d
+    const double xs = xjc * inv_d;  // x[j][c] / d
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
+    for (int m = 0; m < 4; ++m) {
       const int i = r0 + 4 * m;
       const bool below = i > j;
-      const double na = a[m] - aij[m] * acj;
+      const double na = a[m] - aij[m] * as;
       const double nx = x[m] - aij[m] * xs;
       a[m] = (below && c > j && i >= c) ? na : a[m];
       x[m] = (below && c <= j) ? nx : x[m];
     }
   }
-  __syncthreads();
-  if (bad && tid == 0) *fail = 1;
-  // L[i][c] = a[i][c] / sqrt(d_c) (i > c), sqrt(d_c) (i == c); L^-1[i][c] = x[i][c] / sqrt(d_i)
-  const double dc = dsh[c] > 0.0 ? dsh[c] : 1.0;
+  wave_sync();
+  const double dc = fs.d[c] > 0.0 ? fs.d[c] : 1.0;
   const double sdc = sqrt(dc);
   const double isdc = 1.0 / sdc;
 #pragma unroll
-  for (int m = 0; m < 16; ++m) {
+  for (int m = 0; m < 4; ++m) {
     const int i = r0 + 4 * m;
-    const double di = dsh[i] > 0.0 ? dsh[i] : 1.0;
-    dstL[(int64_t)i * ldl + c] = (i > c) ? a[m] * isdc : ((i == c) ? sdc : 0.0);
-    dstI[(int64_t)i * ldi + c] = (c <= i) ? x[m] / sqrt(di) : 0.0;
+    const double di = fs.d[i] > 0.0 ? fs.d[i] : 1.0;
+    if (i >= c) wa[c0 + i][c0 + c] = (i > c) ? a[m] * isdc : sdc;
+    wx[c0 + i][c0 + c] = (c <= i) ? x[m] / sqrt(di) : 0.0;
+  }
+}
+
+// Factor a 64 x 64 SPD tile (lower triangle of src, row pitch lds) and invert
+// the factor: writes L (zero upper) to dstL and L^-1 (zero upper) to dstI; a
+// non-positive pivot sets *fail.  wa / wx are two LDS tiles of scratch.
+// Right-looking blocked Cholesky with 16-wide panels p = 0..3:
+//   L_pp = chol(A_pp), M_p = L_pp^-1      (one wave, diag16)
+//   L_rp = A_rp M_p^T                     (rows below the panel)
+//   A_ij -= L_ip L_jp^T                   (trailing lower triangle)
+// then the off-diagonal blocks of L^-1 by block diagonals s = i - j = 1..3:
+//   T_ij = sum_{k=j}^{i-1} L_ik X_kj,  X_ij = -M_i T_ij
+// (T is staged in the unused upper triangle of wa).  256 threads.
+__device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dstL, int64_t ldl,
+                                                    double *dstI, int64_t ldi, int *fail, double (*wa)[LDT],
+                                                    double (*wx)[LDT], FactorScratch &fs) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    wa[i][j] = src[(int64_t)i * lds + j];
+  }
+  __syncthreads();
+  bool bad = false;
+  const int rr = tid >> 4, cc = tid & 15;
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const int c0 = PB * p;
+    if (tid < 64) diag16(wa, wx, c0, fs, bad);
+    __syncthreads();
+    const int nu = 3 - p;  // 16-row blocks below the panel
+    if (nu > 0) {
+      // panel: L_rp = A_rp M_p^T (M_p zero above its diagonal)
+      double out[3];
+#pragma unroll
+      for (int u = 0; u < 3; ++u) {
+        if (u < nu) {
+          const int r = c0 + PB + rr + 16 * u;
+          double s = 0.0;
+#pragma unroll
+          for (int t = 0; t < PB; ++t) s += wa[r][c0 + t] * wx[c0 + cc][c0 + t];
+          out[u] = s;
+        }
+      }
+      wave_sync();  // the 16 lanes reading row r are this wave's
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+        if (u < nu) wa[c0 + PB + rr + 16 * u][c0 + cc] = out[u];
+      __syncthreads();
+      // trailing lower triangle: rows base + rr + 16u, columns base + cc + 16v
+      const int base = c0 + PB;
+      double acc[3][3];
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+#pragma unroll
+        for (int v = 0; v < 3; ++v) acc[u][v] = 0.0;
+#pragma unroll
+      for (int t = 0; t < PB; ++t) {
+        double li[3], lj[3];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+          li[u] = u < nu ? wa[base + rr + 16 * u][c0 + t] : 0.0;
+          lj[u] = u < nu ? wa[base + cc + 16 * u][c0 + t] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 3; ++u)
+#pragma unroll
+          for (int v = 0; v <= u; ++v) acc[u][v] += li[u] * lj[v];
+      }
+#pragma unroll
+      for (int u = 0; u < 3; ++u)
+#pragma unroll
+        for (int v = 0; v <= u; ++v)
+          if (u < nu && (u > v || rr >= cc)) wa[base + rr + 16 * u][base + cc + 16 * v] -= acc[u][v];
+      __syncthreads();
+    }
+  }
+  // off-diagonal blocks of X = L^-1, one block diagonal at a time
+#pragma unroll
+  for (int s = 1; s < 4; ++s) {
+#pragma unroll
+    for (int q = 0; q + s < 4; ++q) {
+      const int bj = q, bi = q + s;
+      double t_acc = 0.0;
+      for (int k = bj; k < bi; ++k)
+#pragma unroll
+        for (int t = 0; t < PB; ++t) t_acc += wa[PB * bi + rr][PB * k + t] * wx[PB * k + t][PB * bj + cc];
+      wa[PB * bj + rr][PB * bi + cc] = t_acc;  // upper block (bj, bi) is free
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q + s < 4; ++q) {
+      const int bj = q, bi = q + s;
+      double v = 0.0;
+#pragma unroll
+      for (int t = 0; t < PB; ++t) v += wx[PB * bi + rr][PB * bi + t] * wa[PB * bj + t][PB * bi + cc];
+      wx[PB * bi + rr][PB * bj + cc] = -v;
+    }
+    __syncthreads();
+  }
+  if (bad && tid == 0) *fail = 1;
+  for (int e = tid; e < NB * NB; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    dstL[(int64_t)i * ldl + j] = (j <= i) ? wa[i][j] : 0.0;
+    dstI[(int64_t)i * ldi + j] = (j <= i) ? wx[i][j] : 0.0;
   }
 }
 
@@ -207,12 +315,15 @@ __device__ __forceinline__ bool chol_active(const WinState &s) { return s.active
 // Diagonal tile 0 (before the first step).
 __global__ void __launch_bounds__(256) k_chol_first(WinInfo wi, WinState *ws, int w, const double *S, double *Lf,
                                                     double *Dinv) {
+  __shared__ double wa[NB][LDT];
+  __shared__ double wx[NB][LDT];
+  __shared__ FactorScratch fs;
   __shared__ int fail;
   WinState &s = ws[w];
   if (!chol_active(s)) return;
   if (threadIdx.x == 0) fail = 0;
   __syncthreads();
-  tile_factor_inverse(S + wi.s_off, wi.ld, Lf + wi.s_off, wi.ld, Dinv + wi.dinv_off, NB, &fail);
+  tile_factor_inverse(S + wi.s_off, wi.ld, Lf + wi.s_off, wi.ld, Dinv + wi.dinv_off, NB, &fail, wa, wx, fs);
   __syncthreads();
   if (threadIdx.x == 0 && fail) s.chol_fail = 1;
 }
@@ -226,6 +337,7 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
   __shared__ double pb[NB][LDT];
   __shared__ double vsh[3][NB];
   __shared__ double red[4][NB];
+  __shared__ FactorScratch fs;
   __shared__ int fail;
   WinState &s = ws[w];
   if (!chol_active(s)) return;
@@ -265,7 +377,7 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
     tile_factor_inverse(A + (int64_t)ib * NB * ld + ib * NB, ld, L + (int64_t)ib * NB * ld + ib * NB, ld,
-                        Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB, &fail);
+                        Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB, &fail, pa, pb, fs);
     __syncthreads();
     if (threadIdx.x == 0 && fail) s.chol_fail = 1;
   }
@@ -308,6 +420,7 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
   __shared__ double c[NB][LDT];
   __shared__ double vsh[3][NB];
   __shared__ double red[4][NB];
+  __shared__ FactorScratch fs;
   __shared__ int fail;
   const int w = win_list[blockIdx.x];
   const WinInfo wi = wis[w];
@@ -321,7 +434,7 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
     __syncthreads();
     __threadfence_block();
     tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, A + (int64_t)kb * NB * ld + kb * NB, ld,
-                        Dw + (int64_t)kb * NB * NB, NB, &fail);
+                        Dw + (int64_t)kb * NB * NB, NB, &fail, a, c, fs);
     __threadfence_block();
     __syncthreads();
     load_tile(c, Dw + (int64_t)kb * NB * NB, NB, 0, 0);

@@ -413,6 +413,12 @@ def large_scene(seed: int = SEED_LARGE, num_cameras: int = 2000, num_points: int
                              cams_true.reshape(-1)])
     scene = _finish(scene, rng, x_true)
     p3, i6, _ = scene.x_layout()
+    # The strip reaches kilometres from the origin, where an angle-axis
+    # perturbation of t = -R C would swing cameras by metres; x0 perturbs the
+    # rotation and the camera centre instead (a GPS-style initialisation).
+    im0 = scene.x0[p3:p3 + i6].reshape(C, 6)
+    c0 = centers + rng.normal(0.0, 0.05, (C, 3))
+    im0[:, 3:] = -np.einsum("nij,nj->ni", angle_axis_to_matrix(im0[:, :3]), c0)
     scene.x0[p3 + i6:] = np.stack([np.zeros(C), np.zeros(C), np.full(C, focal)], 1).reshape(-1)
     return scene
 

@@ -80,6 +80,8 @@ void wave_exchange(const void *in, void *out, int bytes, int src_lane) {
   p.wave_bar[t / 64]->arrive_and_wait();
 }
 
+void wave_sync() { pool().wave_bar[threadIdx_.x / 64]->arrive_and_wait(); }
+
 void run_kernel(dim3 grid, dim3 block, const std::function<void()> &body) {
   blockDim_ = block;
   gridDim_ = grid;

@@ -40,6 +40,7 @@ extern thread_local dim3 threadIdx_, blockIdx_;
 extern dim3 blockDim_, gridDim_;
 void syncthreads();
 void wave_exchange(const void *in, void *out, int bytes, int src_lane);
+void wave_sync();
 int lane_id();
 using Job = std::function<void()>;
 void run_kernel(dim3 grid, dim3 block, const std::function<void()> &body);
@@ -71,6 +72,9 @@ inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+// A wave barrier is a real barrier of the wave's 64 threads here.
+inline void __builtin_amdgcn_wave_barrier() { hipemu::wave_sync(); }
+inline void __builtin_amdgcn_fence(int, const char *) { std::atomic_thread_fence(std::memory_order_seq_cst); }
 
 typedef enum hipError_t { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorInvalidValue = 1 } hipError_t;
 typedef struct ihipStream_t *hipStream_t;
