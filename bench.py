@@ -234,6 +234,89 @@ def run_windows(args):
             "statuses": sorted(set(int(s.status) for s in sts))}
 
 
+def run_serving(args):
+    """Serving loop: Poisson arrivals at each rate in --rates; 95% of jobs from
+    the windows distribution, 5% of the 200-camera x 9, ~1M-key class (the
+    mid scene itself).  Jobs are drawn with replacement from a pool of
+    pre-built problems; every result is compared bit for bit with the same
+    problem solved alone, and a sample with the FP64 oracle."""
+    from hs_sfm_amd.hsba import Server
+    rng = np.random.default_rng(syn.SEED_SERVING)
+    cams, pts = syn.window_sizes(rng, args.pool)
+    pool = [syn.window_scene(rng, int(c), int(p), name=f"pool{w}") for w, (c, p) in enumerate(zip(cams, pts))]
+    mid = syn.mid_scene()
+    K = args.steps
+    srv = Server(streams=args.streams, max_batch_jobs=args.max_batch, max_iterations=K,
+                 hbm_budget_bytes=args.hbm_budget_gib << 30)
+    # the same problems alone (reference results for the bit-for-bit check)
+    alone = []
+    for s in pool + [mid]:
+        pr = Problem(s, precision=abi.PRECISION_MIXED)
+        rc, x, st = pr.lm(max_iterations=K)
+        abi.check(rc, "alone")
+        alone.append(x)
+        pr.close()
+    prepared = [srv.prepare(s) for s in pool + [mid]]
+    rows = []
+    mismatches = 0
+    for lam in args.rates:
+        n_jobs = int(min(args.max_jobs, max(50, lam * args.seconds)))
+        gaps = rng.exponential(1.0 / lam, n_jobs)
+        kinds = np.where(rng.random(n_jobs) < 0.05, len(pool), rng.integers(0, len(pool), n_jobs))
+        ids, which = [], []
+        t_start = srv.clock()
+        due = t_start
+        for g, k in zip(gaps, kinds):
+            due += g
+            while srv.clock() < due:
+                pass
+            d, x0, y, _ = prepared[k]
+            ids.append(srv.submit((d, x0, y, np.zeros_like(alone[k]))))
+            which.append(int(k))
+        srv.drain()
+        res = {r.job_id: r for r in srv.poll(len(ids) + 16)}
+        lat, lat_w, lat_m, bad = [], [], [], 0
+        for jid, k in zip(ids, which):
+            r = res[jid]
+            if r.status != 0:
+                bad += 1
+                print(f"job {jid} (pool {k}) failed: {r.status}", file=sys.stderr)
+            mismatches += int(not np.array_equal(srv.x_out(jid), alone[k]))
+            ms = 1e3 * (r.finish_time_s - r.submit_time_s)
+            lat.append(ms)
+            (lat_m if k == len(pool) else lat_w).append(ms)
+        first = min(res[j].submit_time_s for j in ids)
+        last = max(res[j].finish_time_s for j in ids)
+        srv._jobs.clear()
+        rows.append({"rate_per_s": lam, "jobs": len(ids), "mid_jobs": len(lat_m),
+                     "solves_per_s": len(ids) / (last - first),
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "window_p50_ms": float(np.percentile(lat_w, 50)) if lat_w else None,
+                     "window_p99_ms": float(np.percentile(lat_w, 99)) if lat_w else None,
+                     "mid_p50_ms": float(np.percentile(lat_m, 50)) if lat_m else None,
+                     "failed": bad})
+        print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+    srv.close()
+    # accuracy of a sample against the FP64 oracle; the mid class against its
+    # committed 20-iteration oracle reference
+    from oracle.oracle import Oracle  # checker only
+    worst = 0.0
+    for w in range(0, len(pool), max(1, len(pool) // args.check_windows)):
+        _, xo, _, _ = Oracle(pool[w]).lm(max_iterations=K)
+        worst = max(worst, abs(syn.reprojection_rmse(pool[w], alone[w]) - syn.reprojection_rmse(pool[w], xo)))
+    ref = oracle_reference("mid")
+    mid_delta = abs(syn.reprojection_rmse(mid, alone[-1]) - ref["rmse_px"]) if ref and ref["iterations"] == K else None
+    best = max(rows, key=lambda r: r["solves_per_s"])
+    return {"metric": "serving solves/s at Poisson arrivals (95% windows, 5% 1M-key), p50/p99 ms",
+            "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": 0,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32+fp64", "data": "synthetic",
+            "config": {"workload": f"serving: pool of {len(pool)} windows + the 1M-key scene, {K} LM iterations "
+                                   f"per job, {args.streams} streams, batches of <= {args.max_batch}"},
+            "rates": rows, "results_identical_to_alone": mismatches == 0, "mismatches": mismatches,
+            "rmse_worst_delta_vs_fp64_oracle_px": worst, "mid_rmse_delta_vs_fp64_oracle_px": mid_delta}
+
+
 def run_ring(args):
     scene = syn.ring_scene()
     p = Problem(scene, precision=abi.PRECISION_FP64)
@@ -263,7 +346,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mid", choices=["mid", "large", "windows", "ring"])
+    ap.add_argument("--workload", default="mid", choices=["mid", "large", "windows", "ring", "serving"])
+    ap.add_argument("--rates", type=float, nargs="+", default=[100, 200, 500, 1000, 2000, 5000])
+    ap.add_argument("--seconds", type=float, default=1.0, help="arrival window per rate (serving)")
+    ap.add_argument("--max-jobs", type=int, default=1500, help="job cap per rate (serving)")
+    ap.add_argument("--pool", type=int, default=128, help="distinct window problems (serving)")
+    ap.add_argument("--streams", type=int, default=4)
+    ap.add_argument("--max-batch", type=int, default=64)
+    ap.add_argument("--hbm-budget-gib", type=int, default=64)
     ap.add_argument("--windows", type=int, default=512)
     ap.add_argument("--check-windows", type=int, default=16)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
@@ -273,7 +363,8 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only"}))
         sys.exit(2)
-    run = {"mid": run_mid, "large": run_large, "windows": run_windows, "ring": run_ring}[args.workload]
+    run = {"mid": run_mid, "large": run_large, "windows": run_windows, "ring": run_ring,
+           "serving": run_serving}[args.workload]
     if args.workload == "large" and args.steps == 20:
         args.steps = 30  # the configuration's LM iteration count
     out = run(args)

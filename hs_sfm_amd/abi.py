@@ -86,7 +86,13 @@ class hsba_stage_times(C.Structure):
 
 class hsba_server_options(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active_jobs", C.c_int32),
-                ("precision", C.c_int32), ("lm", hsba_lm_options)]
+                ("precision", C.c_int32), ("lm", hsba_lm_options), ("streams", C.c_int32),
+                ("max_batch_jobs", C.c_int32)]
+
+
+class hsba_naive_desc(C.Structure):
+    _fields_ = [("num_cameras", C.c_int64), ("num_points", C.c_int64), ("num_features", C.c_int64),
+                ("feature_maps", C.POINTER(C.c_int32))]
 
 
 class hsba_job_result(C.Structure):
@@ -199,6 +205,15 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.hsba_server_drain.argtypes = [P]
     lib.hsba_server_clock.argtypes = [P]
     lib.hsba_server_clock.restype = C.c_double
+    lib.hsba_naive_problem_create.argtypes = [C.POINTER(hsba_naive_desc), C.POINTER(hsba_options),
+                                              C.POINTER(P)]
+    lib.hsba_naive_problem_destroy.argtypes = [P]
+    lib.hsba_naive_evaluate.argtypes = [P, _dp, _dp]
+    lib.hsba_naive_jacobian.argtypes = [P, _dp, _dp, _dp]
+    lib.hsba_naive_build_normal.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.hsba_naive_solve.argtypes = [P, C.c_double, _dp]
+    lib.hsba_naive_lm_solve.argtypes = [P, _dp, _dp, _dp, C.POINTER(hsba_lm_options), _dp,
+                                        C.POINTER(hsba_lm_stats)]
     _LIB = lib
     return lib
 
@@ -211,7 +226,9 @@ EXPORTED_SYMBOLS = [
     "hsba_lm_iterate", "hsba_lm_end", "hsba_batch_create",
     "hsba_batch_destroy", "hsba_batch_lm_solve", "hsba_server_create",
     "hsba_server_destroy", "hsba_server_submit", "hsba_server_poll", "hsba_server_drain",
-    "hsba_server_clock",
+    "hsba_server_clock", "hsba_naive_problem_create", "hsba_naive_problem_destroy",
+    "hsba_naive_evaluate", "hsba_naive_jacobian", "hsba_naive_build_normal", "hsba_naive_solve",
+    "hsba_naive_lm_solve",
 ]
 
 

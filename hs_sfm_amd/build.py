@@ -23,6 +23,7 @@ SOURCES = [
     "hsba_layout.cpp",
     "hsba_engine.cpp",
     "hsba_server.cpp",
+    "hsba_naive.cpp",
     "hsba_api.cpp",
 ]
 ARCH = os.environ.get("HSBA_OFFLOAD_ARCH", "gfx950")

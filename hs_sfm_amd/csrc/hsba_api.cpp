@@ -38,29 +38,35 @@ hsba_options defaults(const hsba_options *o) {
   return r;
 }
 
-int make(const hsba_problem_desc *descs, int64_t count, const hsba_options *opt,
-         std::unique_ptr<EngineBase> &out) {
+}  // namespace
+
+namespace hsba {
+
+bool auto_pcg(const hsba_problem_desc &desc) {
+  // dense FP64 Cholesky while the reduced system stays small enough to factor
+  // quickly, implicit-Schur PCG beyond 4096 reduced unknowns
+  if (desc.fix_mask & HSBA_FIX_POINTS) return false;
+  uint32_t en = 0;
+  const uint32_t comp = desc.intrinsic_computations_mask;
+  if (comp & HSBA_COMPUTE_RADIAL_DISTORTION) en |= 0x7u;
+  if (comp & HSBA_COMPUTE_DECENTERING_DISTORTION) en |= 0x18u;
+  if (comp & HSBA_COMPUTE_INTRINSIC_PARAMS) en |= 0x3E0u;
+  const uint32_t fb = desc.intrinsic_free_mask ? desc.intrinsic_free_mask : en;
+  const int dcam = __builtin_popcount(fb);
+  const int64_t n = ((desc.fix_mask & HSBA_FIX_IMAGES) ? 0 : 6 * desc.num_images) +
+                    ((desc.fix_mask & HSBA_FIX_CAMERAS) ? 0 : dcam * desc.num_cameras);
+  return n > 4096;
+}
+
+int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_options *opt, hipStream_t stream,
+                  DeviceArena *arena, std::unique_ptr<EngineBase> &out) {
   const hsba_options o = defaults(opt);
   if (o.precision != HSBA_PRECISION_FP64 && o.precision != HSBA_PRECISION_MIXED) return HSBA_ERR_INVALID;
   std::unique_ptr<EngineBase> eng(hsba::make_engine(o.precision));
   std::string why;
-  // AUTO: dense FP64 Cholesky while the reduced system stays small enough to
-  // factor quickly, implicit-Schur PCG beyond 4096 reduced unknowns
   bool pcg = o.solver == HSBA_SOLVER_PCG;
-  if (o.solver == HSBA_SOLVER_AUTO && !(descs[0].fix_mask & HSBA_FIX_POINTS)) {
-    uint32_t en = 0;
-    const uint32_t comp = descs[0].intrinsic_computations_mask;
-    if (comp & HSBA_COMPUTE_RADIAL_DISTORTION) en |= 0x7u;
-    if (comp & HSBA_COMPUTE_DECENTERING_DISTORTION) en |= 0x18u;
-    if (comp & HSBA_COMPUTE_INTRINSIC_PARAMS) en |= 0x3E0u;
-    const uint32_t fb = descs[0].intrinsic_free_mask ? descs[0].intrinsic_free_mask : en;
-    const int dcam = __builtin_popcount(fb);
-    for (int64_t w = 0; w < count; ++w) {
-      const int64_t n = ((descs[w].fix_mask & HSBA_FIX_IMAGES) ? 0 : 6 * descs[w].num_images) +
-                        ((descs[w].fix_mask & HSBA_FIX_CAMERAS) ? 0 : dcam * descs[w].num_cameras);
-      if (n > 4096) pcg = true;
-    }
-  }
+  if (o.solver == HSBA_SOLVER_AUTO)
+    for (int64_t w = 0; w < count; ++w) pcg = pcg || auto_pcg(descs[w]);
   int rc = hsba::build_layout(descs, count, eng->L, &why, pcg);
   if (rc) {
     if (std::getenv("HSBA_VERBOSE")) std::fprintf(stderr, "hsba: invalid problem: %s\n", why.c_str());
@@ -86,9 +92,19 @@ int make(const hsba_problem_desc *descs, int64_t count, const hsba_options *opt,
     eng->glog2 = g;
   }
   eng->solver = pcg ? HSBA_SOLVER_PCG : HSBA_SOLVER_DENSE;
-  if ((rc = eng->init(nullptr))) return rc;
+  eng->arena = arena;
+  if ((rc = eng->init(stream))) return rc;
   out = std::move(eng);
   return HSBA_OK;
+}
+
+}  // namespace hsba
+
+namespace {
+
+int make(const hsba_problem_desc *descs, int64_t count, const hsba_options *opt,
+         std::unique_ptr<EngineBase> &out) {
+  return hsba::create_engine(descs, count, opt, nullptr, nullptr, out);
 }
 
 }  // namespace

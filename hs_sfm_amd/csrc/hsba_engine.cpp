@@ -26,11 +26,18 @@ namespace {
     }                                                                                  \
   } while (0)
 
+// Device buffer of n elements, from the arena when the engine has one (the
+// serving loop's pre-allocated budget), else from hipMalloc.
 template <typename X>
-int dalloc(X *&p, int64_t n, std::vector<void *> &owned) {
+int dalloc(X *&p, int64_t n, std::vector<void *> &owned, DeviceArena *arena) {
   p = nullptr;
   if (n <= 0) n = 1;
-  HCHK(hipMalloc((void **)&p, sizeof(X) * (size_t)n));
+  if (arena) {
+    p = (X *)arena->alloc((int64_t)sizeof(X) * n);
+    if (!p) return HSBA_ERR_OOM;
+  } else {
+    HCHK(hipMalloc((void **)&p, sizeof(X) * (size_t)n));
+  }
   owned.push_back((void *)p);
   return HSBA_OK;
 }
@@ -69,7 +76,10 @@ struct Engine final : EngineBase {
       if (pg_graph[i]) (void)hipGraphDestroy(pg_graph[i]);
     }
     if (st) (void)hipStreamSynchronize(st);
-    for (void *p : owned) (void)hipFree(p);
+    for (void *p : owned) {
+      if (arena) arena->release(p);
+      else (void)hipFree(p);
+    }
     if (own_stream && st) (void)hipStreamDestroy(st);
     delete ks;
   }
@@ -116,7 +126,7 @@ struct Engine final : EngineBase {
     d.pcg_rows = L.pcg ? (int32_t)L.rows_total : 0;
     for (int j = 0; j < kFull; ++j) d.fidx[j] = L.fidx[j];
     int rc = 0;
-#define A(ptr, n) if ((rc = dalloc(ptr, (n), owned))) return rc
+#define A(ptr, n) if ((rc = dalloc(ptr, (n), owned, arena))) return rc
     int32_t *key_img, *key_pt, *key_orig, *pt_ptr, *pt_win, *img_ptr, *img_keys, *img_win, *img_cam, *img_row;
     int32_t *cam_ptr, *cam_imgs, *cam_win, *cam_row, *pair_win, *pair_img, *row_blk, *row_sub;
     int32_t *ck, *cbg, *cen, *cw;
@@ -195,8 +205,8 @@ struct Engine final : EngineBase {
     {
       int64_t large_s = 0;
       for (int w : L.large_list) large_s = std::max<int64_t>(large_s, L.wi[w].s_off + (int64_t)L.wi[w].ld * L.wi[w].ld);
-      if ((rc = dalloc(plan.Lf, large_s, owned)) || (rc = dalloc(plan.bwork, L.rows_total, owned)) ||
-          (rc = dalloc(plan.ywork, L.rows_total, owned)))
+      if ((rc = dalloc(plan.Lf, large_s, owned, arena)) || (rc = dalloc(plan.bwork, L.rows_total, owned, arena)) ||
+          (rc = dalloc(plan.ywork, L.rows_total, owned, arena)))
         return rc;
     }
     HCHK(hipMemsetAsync(d.S, 0, sizeof(double) * (size_t)std::max<int64_t>(1, L.S_total), st));
@@ -301,7 +311,7 @@ struct Engine final : EngineBase {
     if (any_cov) {
       if (!d.key_cov) {
         double *p;
-        if ((rc = dalloc(p, 3 * L.N, owned))) return rc;
+        if ((rc = dalloc(p, 3 * L.N, owned, arena))) return rc;
         d.key_cov = p;
         drop_graph();
       }

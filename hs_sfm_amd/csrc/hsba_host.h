@@ -1,7 +1,9 @@
 // hsba_host.h — host-side problem layout and the device engine.
 #pragma once
 
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -50,10 +52,37 @@ struct Layout {
 int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::string *why,
                  bool pcg = false);
 int64_t layout_device_bytes(const Layout &L, int precision);
+// Upper bound of the device bytes an engine allocates for one problem, from
+// its description alone (one O(keys) pass over the observations).
+int64_t estimate_device_bytes(const hsba_problem_desc &desc, int precision, bool pcg, bool has_cov);
+// x and y lengths of one problem in the reference layouts (before validation).
+int problem_sizes(const hsba_problem_desc &desc, int64_t *x_size, int64_t *y_size);
+
+// First-fit sub-allocator over one device allocation, with coalescing of
+// freed neighbours (256-byte granules).  Thread-safe.  The serving loop
+// allocates its HBM budget once and every job's buffers come from it, so no
+// hipMalloc / hipFree (which synchronise the device) runs while jobs execute.
+class DeviceArena {
+ public:
+  ~DeviceArena();
+  int init(int64_t bytes);
+  void *alloc(int64_t bytes);  // nullptr when no free block is large enough
+  void release(void *p);
+  int64_t capacity() const { return cap_; }
+  int64_t in_use();
+
+ private:
+  std::mutex mu_;
+  char *base_ = nullptr;
+  int64_t cap_ = 0, used_ = 0;
+  std::map<int64_t, int64_t> free_;  // offset -> bytes
+  std::map<int64_t, int64_t> live_;  // offset -> bytes
+};
 
 struct EngineBase {
   virtual ~EngineBase() {}
   Layout L;
+  DeviceArena *arena = nullptr;  // buffers come from here when set
   int precision = HSBA_PRECISION_MIXED;
   int solver = HSBA_SOLVER_DENSE;
   double pcg_rtol = 1e-6;
@@ -78,5 +107,14 @@ struct EngineBase {
 };
 
 EngineBase *make_engine(int precision);
+
+// Whether the automatic solver choice takes PCG for this problem (reduced
+// system beyond 4096 unknowns).
+bool auto_pcg(const hsba_problem_desc &desc);
+// Builds the layout and an initialised engine for `count` problems on
+// `stream` (nullptr: the engine creates its own), with buffers from `arena`
+// when given.
+int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_options *options,
+                  hipStream_t stream, DeviceArena *arena, std::unique_ptr<EngineBase> &out);
 
 }  // namespace hsba

@@ -388,4 +388,67 @@ int64_t layout_device_bytes(const Layout &L, int precision) {
   return b;
 }
 
+int problem_sizes(const hsba_problem_desc &ds, int64_t *x_size, int64_t *y_size) {
+  if (ds.num_points < 0 || ds.num_images < 0 || ds.num_cameras < 0 || ds.num_keys < 0) return HSBA_ERR_INVALID;
+  const uint32_t fix = ds.fix_mask & 7u;
+  const uint32_t en = enabled_bits(ds.intrinsic_computations_mask & 7u);
+  const uint32_t fb = ds.intrinsic_free_mask ? ds.intrinsic_free_mask : en;
+  const bool pts = !(fix & HSBA_FIX_POINTS), imgs = !(fix & HSBA_FIX_IMAGES), cams = !(fix & HSBA_FIX_CAMERAS);
+  *x_size = (pts ? 3 * ds.num_points : 0) + (imgs ? 6 * ds.num_images : 0) +
+            (cams ? (int64_t)__builtin_popcount(fb) * ds.num_cameras : 0);
+  int64_t rows = 0;
+  if (pts && ds.point_constraints)
+    for (int64_t q = 0; q < ds.num_point_constraints; ++q) rows += __builtin_popcount(ds.point_constraints[q].mask & 7u);
+  if (imgs && ds.image_constraints)
+    for (int64_t q = 0; q < ds.num_image_constraints; ++q) {
+      const uint32_t m = ds.image_constraints[q].mask;
+      rows += ((m & 1u) ? 3 : 0) + __builtin_popcount(m & 0xEu);
+    }
+  if (cams && ds.camera_constraints)
+    for (int64_t q = 0; q < ds.num_camera_constraints; ++q) rows += __builtin_popcount(ds.camera_constraints[q].mask & 0x3FFu);
+  *y_size = 2 * ds.num_keys + rows;
+  return HSBA_OK;
+}
+
+int64_t estimate_device_bytes(const hsba_problem_desc &ds, int precision, bool pcg, bool has_cov) {
+  const int64_t t = precision == HSBA_PRECISION_FP64 ? 8 : 4;
+  const uint32_t fix = ds.fix_mask & 7u;
+  const uint32_t en = enabled_bits(ds.intrinsic_computations_mask & 7u);
+  const uint32_t fb = ds.intrinsic_free_mask ? (ds.intrinsic_free_mask & en) : en;
+  const int64_t IMG = (fix & HSBA_FIX_IMAGES) ? 0 : 6;
+  const int64_t D = (fix & HSBA_FIX_CAMERAS) ? 0 : __builtin_popcount(fb);
+  const int64_t E = IMG + D;
+  const int64_t N = std::max<int64_t>(0, ds.num_keys), P = std::max<int64_t>(0, ds.num_points);
+  const int64_t I = std::max<int64_t>(0, ds.num_images), C = std::max<int64_t>(0, ds.num_cameras);
+  // camera-pair incidences: one per point and pair of images seeing it
+  int64_t inc = 0;
+  if (pcg) {
+    inc = N;
+  } else if (ds.feature_maps) {
+    std::vector<int32_t> k(P, 0);
+    for (int64_t q = 0; q < N; ++q) {
+      const int32_t p = ds.feature_maps[2 * q + 1];
+      if (p >= 0 && p < P) ++k[p];
+    }
+    for (int64_t p = 0; p < P; ++p) inc += (int64_t)k[p] * (k[p] + 1) / 2;
+  }
+  const int64_t npairs = pcg ? I : std::min<int64_t>(I * (I + 1) / 2, inc);
+  const int64_t n = IMG * I + D * C, ld = (n + 63) / 64 * 64, nb = ld / 64;
+  const int64_t S = pcg ? 0 : ld * ld, dinv = pcg ? 0 : nb * 64 * 64;
+  const int64_t nblk = pcg ? 0 : I + C;
+  const int64_t chunks = (N + 8191) / 8192 + (6 * I + 8191) / 8192 + (D * C + 8191) / 8192 + 3;
+  int64_t b = 0;
+  b += N * (4 * 3 + 16) + (has_cov ? N * 24 : 0);
+  b += N * 2 * 3 * E * t + N * 4;                                   // W, Z, image key lists
+  b += P * (8 + 3 * 8 * 3 + t * 18 + 24);
+  b += I * (4 * 5 + 6 * 8 * 2 + t * (2 * E * E + 2 * E) + 8);
+  b += C * (4 * 4 + 8 * (2 * D + 20) + t * (D * D + D));
+  b += npairs * (8 + 8 + t * E * E) + inc * 4;
+  b += (2 * S + dinv) * 8 + ld * (8 * 10 + 8);                       // S, factor, reduced vectors
+  b += (4 * npairs + 2 * nblk) * (int64_t)sizeof(Contrib) + (nblk * (nblk + 1) / 2 + 1) * 8;
+  b += chunks * (16 + 32) + 4096;
+  // allocation granules (about 70 buffers) and slack
+  return b + b / 8 + 70 * 256 + (1 << 20);
+}
+
 }  // namespace hsba

@@ -197,6 +197,133 @@ class Batch:
         return rc, xs, list(st)
 
 
+class NaiveProblem:
+    """ba_naive problem over the C ABI: x = [cameras 6C | points 3P], features
+    in normalised coordinates (ba_naive_vector_function.hpp:117-196)."""
+
+    def __init__(self, num_cameras, num_points, feature_maps, precision=abi.PRECISION_FP64):
+        self.lib = abi.load_library()
+        self.C, self.P = int(num_cameras), int(num_points)
+        self.fm = np.ascontiguousarray(feature_maps, np.int32)
+        self.F = self.fm.shape[0]
+        d = abi.hsba_naive_desc(self.C, self.P, self.F, self.fm.ctypes.data_as(C.POINTER(C.c_int32)))
+        self.h = C.c_void_p()
+        o = _opts(precision)
+        self.last_rc = self.lib.hsba_naive_problem_create(C.byref(d), C.byref(o), C.byref(self.h))
+        abi.check(self.last_rc, "hsba_naive_problem_create")
+        self.x_size = 6 * self.C + 3 * self.P
+
+    def close(self):
+        if self.h:
+            self.lib.hsba_naive_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evaluate(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(2 * self.F)
+        abi.check(self.lib.hsba_naive_evaluate(self.h, abi.dptr(x), abi.dptr(y)), "hsba_naive_evaluate")
+        return y
+
+    def jacobian(self, x):
+        """Returns (rc, camera blocks [F,2,6], point blocks [F,2,3])."""
+        x = np.ascontiguousarray(x, np.float64)
+        jc, jp = np.zeros((self.F, 2, 6)), np.zeros((self.F, 2, 3))
+        rc = self.lib.hsba_naive_jacobian(self.h, abi.dptr(x), abi.dptr(jc), abi.dptr(jp))
+        return rc, jc, jp
+
+    def build_normal(self, x, y_obs, cov=None):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.ascontiguousarray(y_obs, np.float64)
+        cov = None if cov is None else np.ascontiguousarray(cov, np.float64)
+        U, V, g = np.zeros((self.C, 6, 6)), np.zeros((self.P, 3, 3)), np.zeros(self.x_size)
+        rc = self.lib.hsba_naive_build_normal(self.h, abi.dptr(x), abi.dptr(y), abi.dptr(cov), abi.dptr(U),
+                                              abi.dptr(V), abi.dptr(g))
+        abi.check(rc, "hsba_naive_build_normal")
+        return U, V, g
+
+    def solve(self, mu):
+        dx = np.zeros(self.x_size)
+        rc = self.lib.hsba_naive_solve(self.h, mu, abi.dptr(dx))
+        return rc, dx
+
+    def lm(self, x0, y_obs, cov=None, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0):
+        x0 = np.ascontiguousarray(x0, np.float64)
+        y = np.ascontiguousarray(y_obs, np.float64)
+        cov = None if cov is None else np.ascontiguousarray(cov, np.float64)
+        xo = np.zeros(self.x_size)
+        st = abi.hsba_lm_stats()
+        opt = abi.hsba_lm_options(max_iterations=max_iterations, tau=tau, eps1=eps1, eps2=eps2)
+        rc = self.lib.hsba_naive_lm_solve(self.h, abi.dptr(x0), abi.dptr(y), abi.dptr(cov), C.byref(opt),
+                                          abi.dptr(xo), C.byref(st))
+        return rc, xo, st
+
+
+class Server:
+    """Serving loop over the C ABI (continuous batching of independent jobs).
+
+    submit() copies a job in and returns its id at once; poll() returns the
+    jobs finished since the last poll; drain() waits for every submitted job.
+    Each job's result is written into the array returned by submit()."""
+
+    def __init__(self, precision=abi.PRECISION_MIXED, hbm_budget_bytes=0, max_active_jobs=0,
+                 streams=0, max_batch_jobs=0, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0):
+        self.lib = abi.load_library()
+        o = abi.hsba_server_options(hbm_budget_bytes=hbm_budget_bytes, max_active_jobs=max_active_jobs,
+                                    precision=precision, streams=streams, max_batch_jobs=max_batch_jobs)
+        o.lm = abi.hsba_lm_options(max_iterations=max_iterations, tau=tau, eps1=eps1, eps2=eps2)
+        self.h = C.c_void_p()
+        abi.check(self.lib.hsba_server_create(C.byref(o), C.byref(self.h)), "hsba_server_create")
+        self._jobs = {}  # job id -> (desc holder, x0, y, x_out)
+
+    def prepare(self, scene: Scene, x0=None):
+        """Builds the job's C structures ahead of submission (no library call)."""
+        d = abi.Desc(scene)
+        x0 = np.ascontiguousarray(scene.x0 if x0 is None else x0, np.float64)
+        y = np.ascontiguousarray(scene.y_obs, np.float64)
+        return (d, x0, y, np.zeros(scene.x_size))
+
+    def submit(self, job) -> int:
+        d, x0, y, xo = job
+        jid = C.c_int64()
+        rc = self.lib.hsba_server_submit(self.h, C.byref(d.desc), abi.dptr(x0), abi.dptr(y),
+                                         C.byref(d.ycov), abi.dptr(xo), C.byref(jid))
+        abi.check(rc, "hsba_server_submit")
+        self._jobs[jid.value] = job
+        return jid.value
+
+    def x_out(self, job_id: int) -> np.ndarray:
+        return self._jobs[job_id][3]
+
+    def poll(self, max_results=1024):
+        res = (abi.hsba_job_result * max_results)()
+        n = C.c_int32()
+        abi.check(self.lib.hsba_server_poll(self.h, res, max_results, C.byref(n)), "hsba_server_poll")
+        return [res[i] for i in range(n.value)]
+
+    def drain(self):
+        abi.check(self.lib.hsba_server_drain(self.h), "hsba_server_drain")
+
+    def clock(self) -> float:
+        return self.lib.hsba_server_clock(self.h)
+
+    def close(self):
+        if self.h:
+            self.lib.hsba_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # Reference-named wrappers ----------------------------------------------------
 class CameraSharedVectorFunction:
     def __init__(self, problem: Problem):

@@ -251,6 +251,45 @@ typedef struct {
 int hsba_lm_iterate(hsba_problem *problem, int32_t iterations, hsba_stage_times *times);
 int hsba_lm_end(hsba_problem *problem, double *x_out, hsba_lm_stats *stats);
 
+/* ---- ba_naive family ---------------------------------------------------- */
+/* 6-DoF cameras in normalised image coordinates, x = [cameras 6C | points 3P]
+ * (ba_naive_vector_function.hpp:117-135); the same device engine as the
+ * camera_shared entry points with one image per camera and identity
+ * intrinsics. */
+typedef struct {
+  int64_t num_cameras;
+  int64_t num_points;
+  int64_t num_features;
+  const int32_t *feature_maps;      /* [num_features][2] = (camera_id, point_id) */
+} hsba_naive_desc;
+
+typedef struct hsba_naive_problem hsba_naive_problem;
+
+int hsba_naive_problem_create(const hsba_naive_desc *desc, const hsba_options *options,
+                              hsba_naive_problem **problem);
+int hsba_naive_problem_destroy(hsba_naive_problem *problem);
+/* BANaiveVectorFunction::operator() (ba_naive_vector_function.hpp:137-196). */
+int hsba_naive_evaluate(hsba_naive_problem *problem, const double *x, double *y);
+/* BANaiveAnalyticalJacobianMatrixCalculator::operator()
+ * (ba_naive_analytical_jacobian_matrix_calculator.hpp:75-220): camera blocks
+ * [F][2][6], point blocks [F][2][3], feature order.  HSBA_ERR_INVALID at an
+ * exactly zero camera rotation, where the reference returns -1 (:104-108). */
+int hsba_naive_jacobian(hsba_naive_problem *problem, const double *x,
+                        double *camera_blocks, double *point_blocks);
+/* BANaiveNormalEquationBuilder::operator() (ba_naive_normal_equation_builder.hpp:78-152):
+ * camera blocks [C][6][6], point blocks [P][3][3], gradient [6C + 3P];
+ * feature_cov_inverse = [F][2][2] (BANaiveFeatureCovarianceInverse) or NULL. */
+int hsba_naive_build_normal(hsba_naive_problem *problem, const double *x, const double *y_obs,
+                            const double *feature_cov_inverse, double *camera_blocks,
+                            double *point_blocks, double *gradient);
+/* BANaiveNormalEquationSolver::operator() (ba_naive_normal_equation_solver.hpp:75-254)
+ * on the last build, damped by mu; delta in the naive x layout. */
+int hsba_naive_solve(hsba_naive_problem *problem, double mu, double *delta_x);
+/* BANaiveLevenbergMarquardtOptimizor (ba_naive_levenberg_marquardt_optimizor.hpp). */
+int hsba_naive_lm_solve(hsba_naive_problem *problem, const double *x0, const double *y_obs,
+                        const double *feature_cov_inverse, const hsba_lm_options *lm,
+                        double *x_out, hsba_lm_stats *stats);
+
 /* ---- batched independent problems (local-BA windows) ------------------- */
 /* Many independent problems solved together: one set of launches per LM
  * iteration covers every window.  No reference counterpart (the reference
@@ -272,10 +311,12 @@ int hsba_batch_lm_solve(hsba_batch *batch, const double *const *x0s,
 typedef struct hsba_server hsba_server;
 
 typedef struct {
-  int64_t hbm_budget_bytes;         /* admission limit on resident job state */
-  int32_t max_active_jobs;          /* slot count                            */
+  int64_t hbm_budget_bytes;         /* admission limit on resident job state (0 = 16 GiB) */
+  int32_t max_active_jobs;          /* jobs admitted at once (0 = 4096)      */
   int32_t precision;                /* HSBA_PRECISION_*                      */
   hsba_lm_options lm;               /* per-job LM settings                   */
+  int32_t streams;                  /* concurrent batches, one HIP stream each (0 = 4) */
+  int32_t max_batch_jobs;           /* jobs per batch (0 = 256)              */
 } hsba_server_options;
 
 typedef struct {

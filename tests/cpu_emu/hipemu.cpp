@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <mutex>
 #include <semaphore>
 
 namespace hipemu {
@@ -82,7 +83,12 @@ void wave_exchange(const void *in, void *out, int bytes, int src_lane) {
 
 void wave_sync() { pool().wave_bar[threadIdx_.x / 64]->arrive_and_wait(); }
 
+// Kernels from several host threads (the serving loop's workers) run one at
+// a time: the pool and the launch dimensions are shared.
+std::mutex launch_mu;
+
 void run_kernel(dim3 grid, dim3 block, const std::function<void()> &body) {
+  std::lock_guard<std::mutex> g(launch_mu);
   blockDim_ = block;
   gridDim_ = grid;
   for (unsigned by = 0; by < grid.y; ++by)
