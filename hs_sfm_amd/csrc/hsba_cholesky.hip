@@ -31,16 +31,33 @@ namespace hsba {
 constexpr int NB = 64;
 constexpr int LDT = NB + 1;  // LDS tile pitch (bank-conflict padding)
 
-__device__ __forceinline__ void load_tile(double (*t)[LDT], const double *A, int64_t ld, int r0, int c0) {
-  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-    const int i = e >> 6, j = e & 63;
-    t[i][j] = A[(int64_t)(r0 + i) * ld + c0 + j];
+// Tile <-> LDS with 256 threads: every global load of the tile is issued
+// before the first LDS store, so the 16 loads of a thread overlap instead of
+// each waiting out the memory latency.
+__device__ __forceinline__ void fetch_tile(double v[16], const double *A, int64_t ld, int r0, int c0) {
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    v[q] = A[(int64_t)(r0 + (e >> 6)) * ld + c0 + (e & 63)];
   }
 }
+__device__ __forceinline__ void put_tile(double (*t)[LDT], const double v[16]) {
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    t[e >> 6][e & 63] = v[q];
+  }
+}
+__device__ __forceinline__ void load_tile(double (*t)[LDT], const double *A, int64_t ld, int r0, int c0) {
+  double v[16];
+  fetch_tile(v, A, ld, r0, c0);
+  put_tile(t, v);
+}
 __device__ __forceinline__ void store_tile(const double (*t)[LDT], double *A, int64_t ld, int r0, int c0) {
-  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x) {
-    const int i = e >> 6, j = e & 63;
-    A[(int64_t)(r0 + i) * ld + c0 + j] = t[i][j];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = threadIdx.x + 256 * q;
+    A[(int64_t)(r0 + (e >> 6)) * ld + c0 + (e & 63)] = t[e >> 6][e & 63];
   }
 }
 
@@ -65,72 +82,45 @@ __device__ __forceinline__ void wave_sync() {
 constexpr int PB = 16;  // panel width inside a 64 x 64 tile
 
 struct FactorScratch {
-  double col[2][PB];  // column j of the diagonal block being eliminated
+  double col[2][PB];   // column j of the diagonal block being eliminated
   double xrow[2][PB];  // row j of its inverse
   double d[PB];        // pivots
 };
 
-// One wave: factor the 16 x 16 diagonal block of wa at (c0, c0) (lower
-// triangle, SPD) in place into its Cholesky factor L_p, and write L_p^-1 (lower,
-// zero upper) to the same block of wx.  Elimination of [A | I] in LDL^T form:
+// The whole workgroup (256 threads, one element each): factor the 16 x 16
+// diagonal block of wa at (c0, c0) (lower triangle, SPD) in place into its
+// Cholesky factor L_p, and write L_p^-1 (lower, zero upper) to the same block
+// of wx.  Elimination of [A | I] in LDL^T form:
 //   step j:  d_j = a[j][j];  a[i][c] -= a[i][j] a[c][j] / d_j   (j < c <= i)
 //            x[i][c] -= (a[i][j] / d_j) x[j][c]                  (c <= j < i)
-// leaves a[i][j] = L_unit[i][j] d_j and x = L_unit^-1, so L = L_unit D^1/2 and
-// L^-1 = D^-1/2 x.  Lane l owns column c = l % 16, rows l / 16 + 4m (m < 4);
-// column j of a and row j of x are broadcast through LDS without a barrier.
+// leaves a[i][j] = Lt[i][j] d_j and x = Lt^-1, so L = Lt D^1/2 and
+// L^-1 = D^-1/2 x.  Thread t owns element (t / 16, t % 16) of both; column j
+// of a and row j of x go through double-buffered LDS, one barrier per column.
 __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int c0, FactorScratch &fs,
                                        bool &bad) {
-  const int l = threadIdx.x & 63;
-  const int c = l & 15, r0 = l >> 4;
-  double a[4], x[4];
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int i = r0 + 4 * m;
-    a[m] = (i >= c) ? wa[c0 + i][c0 + c] : 0.0;
-    x[m] = (i == c) ? 1.0 : 0.0;
-  }
+  const int tid = threadIdx.x;
+  const int i = tid >> 4, c = tid & 15;
+  double a = (i >= c) ? wa[c0 + i][c0 + c] : 0.0;
+  double x = (i == c) ? 1.0 : 0.0;
 #pragma unroll
   for (int j = 0; j < PB; ++j) {
     const int b = j & 1;
-    if (c == j) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) fs.col[b][r0 + 4 * m] = a[m];
-    }
-    if (r0 == (j & 3)) fs.xrow[b][c] = x[j >> 2];
-    wave_sync();
-    double aij[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) aij[m] = fs.col[b][r0 + 4 * m];
-    const double d = fs.col[b][j];
-    const double acj = fs.col[b][c];
-    const double xjc = fs.xrow[b][c];
-    if (l == 0) fs.d[j] = d;
+    if (c == j) fs.col[b][i] = a;
+    if (i == j) fs.xrow[b][c] = x;
+    __syncthreads();
+    const double aij = fs.col[b][i], acj = fs.col[b][c], d = fs.col[b][j], xjc = fs.xrow[b][c];
     const bool ok = d > 0.0;
     bad = bad || !ok;
-    const double inv_d = ok ? recip_nr(d) : 0.0;
-    const double as = acj * inv_d;  // a[c][j] / d
-    const double xs = xjc * inv_d;  // x[j][c] / d
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const int i = r0 + 4 * m;
-      const bool below = i > j;
-      const double na = a[m] - aij[m] * as;
-      const double nx = x[m] - aij[m] * xs;
-      a[m] = (below && c > j && i >= c) ? na : a[m];
-      x[m] = (below && c <= j) ? nx : x[m];
-    }
+    const double f = aij * (ok ? recip_nr(d) : 0.0);
+    if (tid == 0) fs.d[j] = d;
+    a = (i > j && c > j && i >= c) ? a - f * acj : a;
+    x = (i > j && c <= j) ? x - f * xjc : x;
   }
-  wave_sync();
-  const double dc = fs.d[c] > 0.0 ? fs.d[c] : 1.0;
+  __syncthreads();
+  const double dc = fs.d[c] > 0.0 ? fs.d[c] : 1.0, di = fs.d[i] > 0.0 ? fs.d[i] : 1.0;
   const double sdc = sqrt(dc);
-  const double isdc = 1.0 / sdc;
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    const int i = r0 + 4 * m;
-    const double di = fs.d[i] > 0.0 ? fs.d[i] : 1.0;
-    if (i >= c) wa[c0 + i][c0 + c] = (i > c) ? a[m] * isdc : sdc;
-    wx[c0 + i][c0 + c] = (c <= i) ? x[m] / sqrt(di) : 0.0;
-  }
+  if (i >= c) wa[c0 + i][c0 + c] = (i > c) ? a / sdc : sdc;
+  wx[c0 + i][c0 + c] = (c <= i) ? x / sqrt(di) : 0.0;
 }
 
 // Factor a 64 x 64 SPD tile (lower triangle of src, row pitch lds) and invert
@@ -147,17 +137,14 @@ __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t l
                                                     double *dstI, int64_t ldi, int *fail, double (*wa)[LDT],
                                                     double (*wx)[LDT], FactorScratch &fs) {
   const int tid = threadIdx.x;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int i = e >> 6, j = e & 63;
-    wa[i][j] = src[(int64_t)i * lds + j];
-  }
+  load_tile(wa, src, lds, 0, 0);
   __syncthreads();
   bool bad = false;
   const int rr = tid >> 4, cc = tid & 15;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     const int c0 = PB * p;
-    if (tid < 64) diag16(wa, wx, c0, fs, bad);
+    diag16(wa, wx, c0, fs, bad);
     __syncthreads();
     const int nu = 3 - p;  // 16-row blocks below the panel
     if (nu > 0) {
@@ -230,66 +217,77 @@ __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t l
     __syncthreads();
   }
   if (bad && tid == 0) *fail = 1;
-  for (int e = tid; e < NB * NB; e += 256) {
-    const int i = e >> 6, j = e & 63;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int e = tid + 256 * q, i = e >> 6, j = e & 63;
     dstL[(int64_t)i * ldl + j] = (j <= i) ? wa[i][j] : 0.0;
     dstI[(int64_t)i * ldi + j] = (j <= i) ? wx[i][j] : 0.0;
   }
 }
 
-// out (LDS) = a * b^T for LDS tiles (256 threads, 4 x 4 outputs each).
-__device__ __forceinline__ void tile_mul_nt_lds(double (*out)[LDT], const double (*a)[LDT], const double (*b)[LDT]) {
-  const int tid = threadIdx.x;
-  const int ti = (tid >> 4) * 4, tc = tid & 15;
-  double acc[4][4];
+// 64 x 64 x 64 products on the matrix cores (v_mfma_f64_16x16x4_f64).  Wave
+// w owns the 32 x 32 quadrant (rows 32 (w / 2), columns 32 (w % 2)) as 2 x 2
+// blocks of 16 x 16.  Operand lanes: A[i][k] from lane i + 16 k, B[k][j] from
+// lane j + 16 k; the result D[i][j] lands in lane j + 16 (i % 4), element
+// i / 4.  acc = a * b^T with a, b in LDS.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void mfma_nt(const double (*a)[LDT], const double (*b)[LDT], dbl4 acc[2][2]) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
+  const int li = l & 15, lk = l >> 4;
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-  for (int k = 0; k < NB; ++k) {
-    double av[4], bv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) av[u] = a[ti + u][k];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) bv[v] = b[tc + 16 * v][k];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+    for (int v = 0; v < 2; ++v) acc[u][v] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k = 0; k < NB; k += 4) {
+    const double a0 = a[r0 + li][k + lk], a1 = a[r0 + 16 + li][k + lk];
+    const double b0 = b[c0 + li][k + lk], b1 = b[c0 + 16 + li][k + lk];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
   }
+}
+
+// out (LDS) = a * b^T for LDS tiles (256 threads); out may alias a or b.
+__device__ __forceinline__ void tile_mul_nt_lds(double (*out)[LDT], const double (*a)[LDT], const double (*b)[LDT]) {
+  dbl4 acc[2][2];
+  mfma_nt(a, b, acc);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r0 = 32 * (w >> 1) + (l >> 4), c0 = 32 * (w & 1) + (l & 15);
   __syncthreads();  // a or b may alias out
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) out[ti + u][tc + 16 * v] = acc[u][v];
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) out[r0 + 16 * u + 4 * e][c0 + 16 * v] = acc[u][v][e];
   __syncthreads();
 }
 
-// C (global 64 x 64 at r0, c0) -= a b^T.
+// C (global 64 x 64 at r0, c0) -= a b^T; the tile of C is fetched before the
+// products so its loads overlap them.
 __device__ __forceinline__ void tile_sub_nt_global(double *C, int64_t ld, int r0, int c0, const double (*a)[LDT],
                                                    const double (*b)[LDT]) {
-  const int tid = threadIdx.x;
-  const int ti = (tid >> 4) * 4, tc = tid & 15;
-  double acc[4][4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int rr = r0 + 32 * (w >> 1) + (l >> 4), cc = c0 + 32 * (w & 1) + (l & 15);
+  double cv[2][2][4];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-  for (int k = 0; k < NB; ++k) {
-    double av[4], bv[4];
+    for (int v = 0; v < 2; ++v)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) av[u] = a[ti + u][k];
+      for (int e = 0; e < 4; ++e) cv[u][v][e] = C[(int64_t)(rr + 16 * u + 4 * e) * ld + cc + 16 * v];
+  dbl4 acc[2][2];
+  mfma_nt(a, b, acc);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) bv[v] = b[tc + 16 * v][k];
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 2; ++v)
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
-  }
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) C[(int64_t)(r0 + ti + u) * ld + c0 + tc + 16 * v] -= acc[u][v];
+      for (int e = 0; e < 4; ++e) C[(int64_t)(rr + 16 * u + 4 * e) * ld + cc + 16 * v] = cv[u][v][e] - acc[u][v][e];
 }
 
 // y = M v (transposed = false) or M^T v for a 64 x 64 global tile M with row
@@ -297,11 +295,15 @@ __device__ __forceinline__ void tile_sub_nt_global(double *C, int64_t ld, int r0
 __device__ __forceinline__ void tile_gemv(const double *M, int64_t ld, const double *v, double *out, bool transposed,
                                           double (*scratch)[NB]) {
   const int r = threadIdx.x & 63, sl = threadIdx.x >> 6;  // 4 slices of 16
+  double mv[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int k = sl * 16 + q;
+    mv[q] = transposed ? M[(int64_t)k * ld + r] : M[(int64_t)r * ld + k];
+  }
   double acc = 0.0;
-  if (!transposed)
-    for (int k = sl * 16; k < sl * 16 + 16; ++k) acc += M[(int64_t)r * ld + k] * v[k];
-  else
-    for (int k = sl * 16; k < sl * 16 + 16; ++k) acc += M[(int64_t)k * ld + r] * v[k];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc += mv[q] * v[sl * 16 + q];
   scratch[sl][r] = acc;
   __syncthreads();
   if (threadIdx.x < NB) out[threadIdx.x] = scratch[0][threadIdx.x] + scratch[1][threadIdx.x] +
@@ -345,7 +347,6 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
   double *A = S + wi.s_off;
   double *L = Lf + wi.s_off;
   const double *Li = Dinv + wi.dinv_off + (int64_t)k * NB * NB;
-  load_tile(li, Li, NB, 0, 0);
   if (blockIdx.x >= ntiles) {
     // right-hand side: y_k = Linv_k b_k, w = Linv_k^T y_k, b_j -= A_jk w
     const int jb = k + 1 + (blockIdx.x - ntiles);
@@ -364,8 +365,16 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
   while (r * (r + 1) / 2 > t) --r;
   while ((r + 1) * (r + 2) / 2 <= t) ++r;
   const int ib = k + 1 + r, jb = k + 1 + (t - r * (r + 1) / 2);
-  load_tile(pa, A, ld, ib * NB, k * NB);
-  if (jb != ib) load_tile(pb, A, ld, jb * NB, k * NB);
+  {
+    // all three tiles' loads in flight together
+    double v0[16], v1[16], v2[16];
+    fetch_tile(v0, Li, NB, 0, 0);
+    fetch_tile(v1, A, ld, ib * NB, k * NB);
+    if (jb != ib) fetch_tile(v2, A, ld, jb * NB, k * NB);
+    put_tile(li, v0);
+    put_tile(pa, v1);
+    if (jb != ib) put_tile(pb, v2);
+  }
   __syncthreads();
   tile_mul_nt_lds(pa, pa, li);                          // L_ik = A_ik Linv^T
   if (jb != ib) tile_mul_nt_lds(pb, pb, li);            // L_jk

@@ -58,6 +58,13 @@ struct Contrib {        // one term of a reduced-system block
   int32_t trans;        // pair block used transposed
 };
 
+// Row stride of the per-key Z blocks (3 x E values of `tbytes` each): rounded
+// up to whole 128-byte cache lines, so a gathered block touches as few lines
+// as possible and loads as 16-byte vectors.
+__host__ __device__ constexpr int z_stride(int e, int tbytes) {
+  return (3 * e * tbytes + 127) / 128 * (128 / tbytes);
+}
+
 // All device buffers, passed by value to kernels.
 template <typename T>
 struct Dev {
@@ -83,6 +90,10 @@ struct Dev {
   T *Uc, *gc;
   // per-key trial data
   T *W, *Z;
+  // PCG mode: the Z rows again in image-major order (row t = key img_keys[t]);
+  // key_ipos = each key's row there, ikey_pt = each such row's point
+  T *Zi;
+  const int32_t *key_ipos, *ikey_pt;
   // camera pairs
   const int64_t *pair_ptr;
   const uint32_t *inc;

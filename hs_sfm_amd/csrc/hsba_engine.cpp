@@ -154,7 +154,14 @@ struct Engine final : EngineBase {
     A(d.cam_free, (int64_t)std::max(D, 1) * L.C); A(d.cam_free_new, (int64_t)std::max(D, 1) * L.C);
     A(d.cam_full, kFull * L.C); A(d.cam_full_new, kFull * L.C);
     A(d.Uc, (int64_t)D * D * L.C); A(d.gc, (int64_t)D * L.C);
-    A(d.W, 3LL * E * L.N); A(d.Z, 3LL * E * L.N);
+    A(d.W, 3LL * E * L.N); A(d.Z, (int64_t)z_stride(E, sizeof(T)) * L.N);
+    d.Zi = nullptr; d.key_ipos = nullptr; d.ikey_pt = nullptr;
+    if (L.pcg && L.pts_free && E > 0) {
+      int32_t *kip, *ikp;
+      A(d.Zi, (int64_t)z_stride(E, sizeof(T)) * L.N); A(kip, L.N); A(ikp, L.N);
+      if ((rc = upload(kip, L.key_ipos, st)) || (rc = upload(ikp, L.ikey_pt, st))) return rc;
+      d.key_ipos = kip; d.ikey_pt = ikp;
+    }
     A(d.pair_out, (int64_t)E * E * L.npairs);
     A(d.S, L.S_total); A(d.rhs, L.rows_total); A(d.dred, L.rows_total); A(d.Dinv, L.dinv_total);
     A(d.partial, 4LL * d.nchunks);

@@ -31,6 +31,7 @@ struct Layout {
   std::vector<int32_t> key_img, key_pt, key_orig;
   std::vector<int32_t> pt_ptr, pt_win;
   std::vector<int32_t> img_ptr, img_keys, img_win, img_cam, img_row;
+  std::vector<int32_t> key_ipos, ikey_pt;  // PCG: key -> image-major row, row -> point
   std::vector<int32_t> cam_ptr, cam_imgs, cam_win, cam_row;
   std::vector<int64_t> pair_ptr;
   std::vector<uint32_t> inc;

@@ -300,9 +300,10 @@ __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
     act[threadIdx.x] = a;
   }
   __syncthreads();
+  constexpr int ZS = z_stride(E, sizeof(T));
   const int nk = min(KC, d.N - k0);
   const T *w = d.W + (int64_t)k0 * BLK;
-  T *z = d.Z + (int64_t)k0 * BLK;
+  T *z = d.Z + (int64_t)k0 * ZS;
   for (int e = threadIdx.x; e < nk * BLK; e += blockDim.x) {
     const int kl = e / BLK, rem = e - kl * BLK;
     if (!act[kl]) continue;
@@ -312,7 +313,8 @@ __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
     if (a == 0) v = lsh[kl][0] * wk[j];
     else if (a == 1) v = lsh[kl][1] * wk[j] + lsh[kl][2] * wk[E + j];
     else v = lsh[kl][3] * wk[j] + lsh[kl][4] * wk[E + j] + lsh[kl][5] * wk[2 * E + j];
-    z[e] = v;
+    z[kl * ZS + rem] = v;
+    if (d.Zi) d.Zi[(int64_t)d.key_ipos[k0 + kl] * ZS + rem] = v;
   }
 }
 
@@ -323,8 +325,12 @@ __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
 // fixed butterfly.  Replaces the O(I^2 P) loops of
 // camera_shared_normal_equation_solver.hpp:198-443.
 template <typename T, int E>
-__global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
-  const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(256) k_pairs(Dev<T> d, int blocks_per_xcd) {
+  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
+  // range of pairs (pairs are ordered by first image), so the Z rows of the
+  // images it works on stay in that XCD's L2.
+  const int b = (int)(blockIdx.x & 7) * blocks_per_xcd + (int)(blockIdx.x >> 3);
+  const int pair = b * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (pair >= d.npairs) return;
   if (!win_trial(d.ws[d.pair_win[pair]])) return;
@@ -340,13 +346,22 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
     const uint32_t code = d.inc[j];
     const int a = int(code >> 7);
     const int b = a + int(code & 127u);
-    const T *za = d.Z + (int64_t)a * 3 * E;
-    const T *zb = d.Z + (int64_t)b * 3 * E;
+    constexpr int ZS = z_stride(E, sizeof(T));
+    constexpr int VW = 16 / sizeof(T), NV = (3 * E + VW - 1) / VW;
+    typedef T vec __attribute__((ext_vector_type(VW)));
+    const vec *za = (const vec *)(d.Z + (int64_t)a * ZS);
+    const vec *zb = (const vec *)(d.Z + (int64_t)b * ZS);
+    vec va[NV], vb[NV];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) { va[q] = za[q]; vb[q] = zb[q]; }
     T A[3][E], B[3][E];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
 #pragma unroll
-      for (int c = 0; c < E; ++c) { A[r][c] = za[r * E + c]; B[r][c] = zb[r * E + c]; }
+      for (int c = 0; c < E; ++c) {
+        A[r][c] = va[(r * E + c) / VW][(r * E + c) % VW];
+        B[r][c] = vb[(r * E + c) / VW][(r * E + c) % VW];
+      }
     }
 #pragma unroll
     for (int r = 0; r < E; ++r) {
@@ -477,7 +492,7 @@ __global__ void __launch_bounds__(256) k_backsub(Dev<T> d, int glog2) {
   if (act && E > 0) {
     for (int k = d.pt_ptr[p] + lane; k < d.pt_ptr[p + 1]; k += G) {
       const int im = d.key_img[k];
-      const T *z = d.Z + (int64_t)k * 3 * E;
+      const T *z = d.Z + (int64_t)k * z_stride(E, sizeof(T));
       double dr[E > 0 ? E : 1];
 #pragma unroll
       for (int j = 0; j < E; ++j)
@@ -929,16 +944,22 @@ __global__ void __launch_bounds__(256) k_pcg_points(Dev<T> d, int glog2) {
   }
   double t[3] = {0.0, 0.0, 0.0};
   if (act) {
+    constexpr int ZS = z_stride(E > 0 ? E : 1, sizeof(T));
+    constexpr int VW = 16 / sizeof(T), NV = (3 * E + VW - 1) / VW;
+    typedef T vec __attribute__((ext_vector_type(VW)));
     for (int k = d.pt_ptr[p] + lane; k < d.pt_ptr[p + 1]; k += G) {
       const int im = d.key_img[k], c = d.img_cam[im];
-      const T *z = d.Z + (int64_t)k * 3 * E;
+      const vec *zr = (const vec *)(d.Z + (int64_t)k * ZS);
+      vec zv[NV > 0 ? NV : 1];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) zv[q] = zr[q];
       double v[E > 0 ? E : 1];
 #pragma unroll
       for (int j = 0; j < E; ++j) v[j] = d.pp[sup_row<IMG, D>(d.img_row, d.cam_row, c, im, j)];
 #pragma unroll
       for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int j = 0; j < E; ++j) t[a] += (double)z[a * E + j] * v[j];
+        for (int j = 0; j < E; ++j) t[a] += (double)zv[(a * E + j) / VW][(a * E + j) % VW] * v[j];
     }
   }
   for (int m = 1; m < G; m <<= 1)
@@ -950,9 +971,14 @@ __global__ void __launch_bounds__(256) k_pcg_points(Dev<T> d, int glog2) {
 }
 
 // Ap_i = (U_i + mu I) p_i - sum_{a in i} Z_a^T t_p(a), one workgroup per image.
+// The image's Z rows are contiguous in the image-major copy (coalesced 128-byte
+// rows); t_p is a 24-byte gather from a small, cache-resident vector.
 template <typename T, int IMG, int D>
 __global__ void __launch_bounds__(128) k_pcg_images(Dev<T> d) {
   constexpr int E = IMG + D;
+  constexpr int ZS = z_stride(E, sizeof(T));
+  constexpr int VW = 16 / sizeof(T), NV = (3 * E + VW - 1) / VW;
+  typedef T vec __attribute__((ext_vector_type(VW)));
   const int im = blockIdx.x;
   const int w = d.img_win[im];
   if (!pcg_live(d.ws[w], d.pcg[w])) return;
@@ -961,12 +987,16 @@ __global__ void __launch_bounds__(128) k_pcg_images(Dev<T> d) {
 #pragma unroll
   for (int j = 0; j < E; ++j) acc[j] = 0.0;
   for (int t = d.img_ptr[im] + threadIdx.x; t < d.img_ptr[im + 1]; t += blockDim.x) {
-    const int k = d.img_keys[t];
-    const int p = d.key_pt[k];
-    const T *z = d.Z + (int64_t)k * 3 * E;
+    const int p = d.ikey_pt[t];
+    const vec *zr = (const vec *)(d.Zi + (int64_t)t * ZS);
+    vec zv[NV > 0 ? NV : 1];
+#pragma unroll
+    for (int q = 0; q < NV; ++q) zv[q] = zr[q];
     const double t0 = d.tpd[3 * p], t1 = d.tpd[3 * p + 1], t2 = d.tpd[3 * p + 2];
 #pragma unroll
-    for (int j = 0; j < E; ++j) acc[j] += (double)z[j] * t0 + (double)z[E + j] * t1 + (double)z[2 * E + j] * t2;
+    for (int j = 0; j < E; ++j)
+      acc[j] += (double)zv[j / VW][j % VW] * t0 + (double)zv[(E + j) / VW][(E + j) % VW] * t1 +
+                (double)zv[(2 * E + j) / VW][(2 * E + j) % VW] * t2;
   }
   for (int m = 1; m < 64; m <<= 1)
 #pragma unroll
@@ -1112,7 +1142,10 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void pairs(const Dev<T> &d, hipStream_t st) override {
     if (!d.pts_free || E == 0 || d.npairs == 0) return;
-    hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3((d.npairs + 3) / 4), dim3(256), 0, st, d);
+    {
+      const int blocks = (d.npairs + 3) / 4, per_xcd = (blocks + 7) / 8;
+      hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3(8 * per_xcd), dim3(256), 0, st, d, per_xcd);
+    }
   }
   void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) override {
     if (E == 0 || rows == 0) return;

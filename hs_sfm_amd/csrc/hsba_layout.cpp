@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 
 #include "hsba_host.h"
 
@@ -39,6 +40,24 @@ int fail(std::string *why, const std::string &msg, int code = HSBA_ERR_INVALID) 
 }
 
 constexpr int kChunk = 8192;
+
+// Host threads for the O(keys x track) passes: one per 64k keys, at most 16.
+int layout_threads(int64_t keys) {
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::max<int64_t>(1, std::min<int64_t>({16, hw, keys / 65536}));
+}
+
+template <class F>
+void run_threads(int nt, F &&f) {
+  if (nt <= 1) {
+    f(0);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(f, t);
+  f(0);
+  for (auto &x : th) x.join();
+}
 
 void add_chunks(Layout &L, int kind, int64_t begin, int64_t end, int w) {
   for (int64_t b = begin; b < end; b += kChunk) {
@@ -155,22 +174,32 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
       }
     }
     int max_track = 0;
-    for (int64_t p = 0; p < h.P; ++p) {
-      const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
-      max_track = std::max<int>(max_track, (int)(e - b));
-      // insertion sort of the point's keys by image (stable in key order)
-      for (int64_t a = b + 1; a < e; ++a) {
-        int32_t im = L.key_img[a], ko = L.key_orig[a];
-        int64_t c = a - 1;
-        while (c >= b && L.key_img[c] > im) {
-          L.key_img[c + 1] = L.key_img[c]; L.key_orig[c + 1] = L.key_orig[c]; --c;
+    {
+      const int nt = layout_threads(h.N);
+      std::vector<int> tmax(nt, 0), tdup(nt, 0);
+      run_threads(nt, [&](int t) {
+        for (int64_t p = h.P * t / nt; p < h.P * (t + 1) / nt; ++p) {
+          const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
+          tmax[t] = std::max<int>(tmax[t], (int)(e - b));
+          // insertion sort of the point's keys by image (stable in key order)
+          for (int64_t a = b + 1; a < e; ++a) {
+            int32_t im = L.key_img[a], ko = L.key_orig[a];
+            int64_t c = a - 1;
+            while (c >= b && L.key_img[c] > im) {
+              L.key_img[c + 1] = L.key_img[c]; L.key_orig[c + 1] = L.key_orig[c]; --c;
+            }
+            L.key_img[c + 1] = im; L.key_orig[c + 1] = ko;
+          }
+          for (int64_t a = b + 1; a < e; ++a)
+            if (L.key_img[a] == L.key_img[a - 1]) tdup[t] = 1;
+          L.pt_ptr[pt_off + p + 1] = (int32_t)e;
+          L.pt_win[pt_off + p] = (int32_t)w;
         }
-        L.key_img[c + 1] = im; L.key_orig[c + 1] = ko;
+      });
+      for (int t = 0; t < nt; ++t) {
+        if (tdup[t]) return fail(why, "duplicate (image, point) key");
+        max_track = std::max(max_track, tmax[t]);
       }
-      for (int64_t a = b + 1; a < e; ++a)
-        if (L.key_img[a] == L.key_img[a - 1]) return fail(why, "duplicate (image, point) key");
-      L.pt_ptr[pt_off + p + 1] = (int32_t)e;
-      L.pt_win[pt_off + p] = (int32_t)w;
     }
     if (h.P == 0) L.pt_ptr[pt_off] = (int32_t)key_off;
     if (L.pts_free && L.E > 0 && max_track > 127)
@@ -241,14 +270,29 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
       for (int64_t i = 0; i < h.I; ++i)
         add(bcam(ds.image_camera_map[i]), bimg(i), Contrib{2, (int32_t)(img_off + i), 0, 0, 0});
     if (L.pts_free && L.E > 0 && h.I > 0) {
-      std::vector<int32_t> pid((size_t)(h.I * h.I), -1);
-      std::vector<int64_t> cnt((size_t)(h.I * h.I), 0);
-      for (int64_t p = 0; p < h.P; ++p) {
-        const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
-        for (int64_t u = b; u < e; ++u)
-          for (int64_t v = u; v < (pcg ? u + 1 : e); ++v)
-            cnt[(size_t)((L.key_img[u] - img_off) * h.I + (L.key_img[v] - img_off))]++;
-      }
+      // Incidences are counted and filled by threads over point ranges; a
+      // pair's list is the concatenation of the threads' parts in thread
+      // order, i.e. in point order, the same as a serial pass.
+      const int64_t II = h.I * h.I;
+      const int nt = (pcg || II > (int64_t(1) << 20)) ? 1 : layout_threads(h.N);
+      std::vector<int64_t> pbeg(nt + 1);
+      for (int t = 0; t <= nt; ++t) pbeg[t] = h.P * t / nt;
+      std::vector<std::vector<int32_t>> tcnt(nt);
+      run_threads(nt, [&](int t) {
+        tcnt[t].assign((size_t)II, 0);
+        int32_t *cn = tcnt[t].data();
+        for (int64_t p = pbeg[t]; p < pbeg[t + 1]; ++p) {
+          const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
+          for (int64_t u = b; u < e; ++u) {
+            const int64_t row = (L.key_img[u] - img_off) * h.I - img_off;
+            for (int64_t v = u; v < (pcg ? u + 1 : e); ++v) cn[row + L.key_img[v]]++;
+          }
+        }
+      });
+      std::vector<int32_t> pid((size_t)II, -1);
+      std::vector<int64_t> cnt((size_t)II, 0);
+      for (int t = 0; t < nt; ++t)
+        for (int64_t q = 0; q < II; ++q) cnt[q] += tcnt[t][q];
       const int64_t first_pair = L.npairs;
       std::vector<int64_t> cursor;
       for (int64_t ia = 0; ia < h.I; ++ia)
@@ -279,14 +323,32 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
             }
         }
       L.inc.resize(L.incidences);
-      for (int64_t p = 0; p < h.P; ++p) {
-        const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
-        for (int64_t u = b; u < e; ++u)
-          for (int64_t v = u; v < (pcg ? u + 1 : e); ++v) {
-            const int32_t q = pid[(size_t)((L.key_img[u] - img_off) * h.I + (L.key_img[v] - img_off))];
-            L.inc[cursor[q - first_pair]++] = (uint32_t)((u << 7) | (v - u));
+      const int64_t npw = L.npairs - first_pair;
+      std::vector<std::vector<int64_t>> tcur(nt, std::vector<int64_t>((size_t)npw));
+      for (int64_t ia = 0, qq = 0; ia < h.I; ++ia)
+        for (int64_t ib = ia; ib < h.I; ++ib) {
+          const int64_t q = ia * h.I + ib;
+          if (pid[(size_t)q] < 0) continue;
+          int64_t off = cursor[qq];
+          for (int t = 0; t < nt; ++t) {
+            tcur[t][qq] = off;
+            off += tcnt[t][q];
           }
-      }
+          ++qq;
+        }
+      run_threads(nt, [&](int t) {
+        int64_t *cur = tcur[t].data();
+        for (int64_t p = pbeg[t]; p < pbeg[t + 1]; ++p) {
+          const int64_t b = key_off + pcount[p], e = key_off + pcount[p + 1];
+          for (int64_t u = b; u < e; ++u) {
+            const int64_t row = (L.key_img[u] - img_off) * h.I - img_off;
+            for (int64_t v = u; v < (pcg ? u + 1 : e); ++v) {
+              const int32_t q = pid[(size_t)(row + L.key_img[v])];
+              L.inc[cur[q - first_pair]++] = (uint32_t)((u << 7) | (v - u));
+            }
+          }
+        }
+      });
     }
     // flatten contribution lists
     for (auto &bl : blocks) {
@@ -354,6 +416,14 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
     std::vector<int32_t> fill(L.img_ptr.begin(), L.img_ptr.end() - 1);
     for (int64_t k = 0; k < L.N; ++k) L.img_keys[fill[L.key_img[k]]++] = (int32_t)k;
   }
+  if (pcg) {
+    L.key_ipos.resize(L.N);
+    L.ikey_pt.resize(L.N);
+    for (int64_t t = 0; t < L.N; ++t) {
+      L.key_ipos[L.img_keys[t]] = (int32_t)t;
+      L.ikey_pt[t] = L.key_pt[L.img_keys[t]];
+    }
+  }
   for (int64_t i = 0; i < L.I; ++i) L.cam_ptr[L.img_cam[i] + 1]++;
   for (int64_t c = 0; c < L.C; ++c) L.cam_ptr[c + 1] += L.cam_ptr[c];
   {
@@ -377,7 +447,8 @@ int64_t layout_device_bytes(const Layout &L, int precision) {
   const int64_t E = L.E, D = L.D;
   int64_t b = 0;
   b += L.N * (4 * 3 + 16) + (L.has_cov ? L.N * 24 : 0);           // key arrays
-  b += L.N * 2 * 3 * E * t;                                         // W, Z
+  b += L.N * (3 * E * t + z_stride((int)E, (int)t) * t);           // W, Z
+  if (L.pcg) b += L.N * (z_stride((int)E, (int)t) * t + 8);        // image-major Z copy
   b += L.P * (8 + 3 * 8 * 3 + t * 18);                              // points
   b += L.I * (4 * 5 + 6 * 8 * 2 + t * (E * E + 2 * E));             // images
   b += L.C * (4 * 3 + 8 * (2 * D + 20) + t * (D * D + D));           // cameras
@@ -439,7 +510,8 @@ int64_t estimate_device_bytes(const hsba_problem_desc &ds, int precision, bool p
   const int64_t chunks = (N + 8191) / 8192 + (6 * I + 8191) / 8192 + (D * C + 8191) / 8192 + 3;
   int64_t b = 0;
   b += N * (4 * 3 + 16) + (has_cov ? N * 24 : 0);
-  b += N * 2 * 3 * E * t + N * 4;                                   // W, Z, image key lists
+  b += N * (3 * E * t + z_stride((int)E, (int)t) * t) + N * 4;     // W, Z, image key lists
+  if (pcg) b += N * (z_stride((int)E, (int)t) * t + 8);             // image-major Z copy
   b += P * (8 + 3 * 8 * 3 + t * 18 + 24);
   b += I * (4 * 5 + 6 * 8 * 2 + t * (2 * E * E + 2 * E) + 8);
   b += C * (4 * 4 + 8 * (2 * D + 20) + t * (D * D + D));

@@ -7,20 +7,22 @@ HERE=$(cd "$(dirname "$0")" && pwd)
 SRC=$HERE/../../hs_sfm_amd/csrc
 OUT=$HERE/_build
 mkdir -p "$OUT"
-FLAGS="-std=c++20 -O1 -g -fPIC -fsanitize=address -fno-omit-frame-pointer -I$HERE/include -Wno-unused-result"
+CXX=/opt/rocm/llvm/bin/clang++
+FLAGS="-std=c++20 -O1 -g -fPIC -fsanitize=address -fno-omit-frame-pointer -I$HERE/include -Wno-unused-result -Wno-psabi -Wno-unknown-attributes"
 pids=()
 for f in hsba_kernels.hip hsba_cholesky.hip hsba_layout.cpp hsba_engine.cpp hsba_server.cpp hsba_naive.cpp hsba_api.cpp; do
   if [ "$SRC/$f" -nt "$OUT/$f.o" ] || [ "$HERE/include/hip/hip_runtime.h" -nt "$OUT/$f.o" ] || [ "$SRC/hsba_device.h" -nt "$OUT/$f.o" ] || [ "$SRC/hsba_kernels.h" -nt "$OUT/$f.o" ] || [ "$SRC/hsba_host.h" -nt "$OUT/$f.o" ]; then
-    g++ $FLAGS -x c++ -c "$SRC/$f" -o "$OUT/$f.o" &
+    $CXX $FLAGS -x c++ -c "$SRC/$f" -o "$OUT/$f.o" &
     pids+=($!)
   fi
 done
 if [ "$HERE/hipemu.cpp" -nt "$OUT/hipemu.o" ] || [ "$HERE/include/hip/hip_runtime.h" -nt "$OUT/hipemu.o" ]; then
-  g++ $FLAGS -c "$HERE/hipemu.cpp" -o "$OUT/hipemu.o" &
+  $CXX $FLAGS -c "$HERE/hipemu.cpp" -o "$OUT/hipemu.o" &
   pids+=($!)
 fi
 status=0
 for p in "${pids[@]}"; do wait "$p" || status=1; done
 if [ $status -ne 0 ]; then echo "cpu emulation build failed" >&2; exit 1; fi
-g++ -shared -fsanitize=address -o "$OUT/libhsba_cpu.so" "$OUT"/*.o -lpthread
+$CXX -shared -fsanitize=address -shared-libasan -o "$OUT/libhsba_cpu.so" "$OUT"/*.o -lpthread
+echo "preload: $($CXX -print-file-name=libclang_rt.asan-x86_64.so)" >&2
 echo "$OUT/libhsba_cpu.so"

@@ -81,6 +81,15 @@ void wave_exchange(const void *in, void *out, int bytes, int src_lane) {
   p.wave_bar[t / 64]->arrive_and_wait();
 }
 
+void wave_gather(const void *in, void *all, int bytes) {
+  Pool &p = pool();
+  const unsigned t = threadIdx_.x, base = t & ~63u;
+  std::memcpy(&p.xbuf[8 * t], in, bytes);
+  p.wave_bar[t / 64]->arrive_and_wait();
+  for (unsigned q = 0; q < 64; ++q) std::memcpy((char *)all + q * bytes, &p.xbuf[8 * (base + q)], bytes);
+  p.wave_bar[t / 64]->arrive_and_wait();
+}
+
 void wave_sync() { pool().wave_bar[threadIdx_.x / 64]->arrive_and_wait(); }
 
 // Kernels from several host threads (the serving loop's workers) run one at

@@ -41,6 +41,7 @@ extern dim3 blockDim_, gridDim_;
 void syncthreads();
 void wave_exchange(const void *in, void *out, int bytes, int src_lane);
 void wave_sync();
+void wave_gather(const void *in, void *all, int bytes);
 int lane_id();
 using Job = std::function<void()>;
 void run_kernel(dim3 grid, dim3 block, const std::function<void()> &body);
@@ -72,6 +73,18 @@ inline int min(int a, int b) { return a < b ? a : b; }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+// v_mfma_f64_16x16x4_f64: A[i][k] from lane i + 16 k, B[k][j] from lane
+// j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4.
+typedef double hipemu_dbl4 __attribute__((ext_vector_type(4)));
+inline hipemu_dbl4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, hipemu_dbl4 c, int, int, int) {
+  double A[64], B[64];
+  hipemu::wave_gather(&a, A, 8);
+  hipemu::wave_gather(&b, B, 8);
+  const int l = hipemu::lane_id() & 63, j = l & 15, i0 = l >> 4;
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 4; ++k) c[r] += A[i0 + 4 * r + 16 * k] * B[j + 16 * k];
+  return c;
+}
 // A wave barrier is a real barrier of the wave's 64 threads here.
 inline void __builtin_amdgcn_wave_barrier() { hipemu::wave_sync(); }
 inline void __builtin_amdgcn_fence(int, const char *) { std::atomic_thread_fence(std::memory_order_seq_cst); }
