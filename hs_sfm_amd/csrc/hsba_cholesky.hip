@@ -7,8 +7,8 @@
 // the identity once.  64 x 64 tiles are staged through LDS.
 //
 // Diagonal tiles: a blocked factorization with 16-wide panels (each 16 x 16
-// diagonal block eliminated by one wave without workgroup barriers) yields
-// both L_kk and L_kk^-1; panel tiles are then GEMMs with L_kk^-1.
+// diagonal block eliminated by the whole workgroup, one element per thread)
+// yields both L_kk and L_kk^-1; panel tiles are then GEMMs with L_kk^-1.
 //
 // Large windows (more than 4 tiles): one fused launch per panel step k.
 // Every workgroup of step k owns one trailing tile (i, j), k < j <= i: it forms
@@ -17,7 +17,7 @@
 // step is a single launch.  The right-hand side rides along: step k also
 // forms y_k = L_kk^-1 b_k and updates b_j -= L_jk y_k.  The factor goes to a
 // separate buffer (column k of A is read by every workgroup of step k).  The
-// backward solve L^T x = y is one multi-workgroup launch per block row.
+// backward solve L^T x = y is one multi-workgroup launch per four block rows.
 //
 // Small windows: one workgroup (256 threads) per window runs the whole
 // blocked factorization and both solves; a batch of windows is one launch.
@@ -172,7 +172,7 @@ __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t l
       for (int u = 0; u < 3; ++u)
 #pragma unroll
         for (int v = 0; v < 3; ++v) acc[u][v] = 0.0;
-#pragma unroll
+#pragma unroll 4
       for (int t = 0; t < PB; ++t) {
         double li[3], lj[3];
 #pragma unroll
@@ -201,7 +201,7 @@ __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t l
       const int bj = q, bi = q + s;
       double t_acc = 0.0;
       for (int k = bj; k < bi; ++k)
-#pragma unroll
+#pragma unroll 4
         for (int t = 0; t < PB; ++t) t_acc += wa[PB * bi + rr][PB * k + t] * wx[PB * k + t][PB * bj + cc];
       wa[PB * bj + rr][PB * bi + cc] = t_acc;  // upper block (bj, bi) is free
     }
@@ -210,14 +210,14 @@ __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t l
     for (int q = 0; q + s < 4; ++q) {
       const int bj = q, bi = q + s;
       double v = 0.0;
-#pragma unroll
+#pragma unroll 4
       for (int t = 0; t < PB; ++t) v += wx[PB * bi + rr][PB * bi + t] * wa[PB * bj + t][PB * bi + cc];
       wx[PB * bi + rr][PB * bj + cc] = -v;
     }
     __syncthreads();
   }
   if (bad && tid == 0) *fail = 1;
-#pragma unroll
+#pragma unroll 4
   for (int q = 0; q < 16; ++q) {
     const int e = tid + 256 * q, i = e >> 6, j = e & 63;
     dstL[(int64_t)i * ldl + j] = (j <= i) ? wa[i][j] : 0.0;
@@ -392,32 +392,49 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
   }
 }
 
-// Backward solve step k: x_k = Linv_k^T y_k (y_k = Linv_k b_k when k is the
-// last block), then y_i -= L_ki^T x_k for every block i < k (one workgroup each).
+// Backward solve of the block rows k_hi, k_hi - 1, ..., k_lo (at most four)
+// in one launch.  Every workgroup forms that short chain itself,
+//   v = y_k - sum_{m = k+1..k_hi} L_mk^T x_m   (y_k = Linv_k b_k for the last block),
+//   x_k = Linv_k^T v,
+// then workgroup ib < k_lo applies y_ib -= sum_{m = k_lo..k_hi} L_m,ib^T x_m.
 __global__ void __launch_bounds__(256) k_chol_back(WinInfo wi, WinState *ws, int w, const double *Lf,
                                                    const double *Dinv, const double *b, double *y, double *x,
-                                                   int k) {
+                                                   int k_hi, int k_lo) {
+  __shared__ double xs[4][NB];
   __shared__ double vsh[3][NB];
   __shared__ double red[4][NB];
   const WinState &s = ws[w];
   if (!chol_active(s)) return;
   const int64_t ld = wi.ld;
-  const double *Li = Dinv + wi.dinv_off + (int64_t)k * NB * NB;
-  const bool last = k == wi.nb - 1;
-  for (int e = threadIdx.x; e < NB; e += blockDim.x)
-    vsh[0][e] = last ? b[wi.r_off + k * NB + e] : y[wi.r_off + k * NB + e];
-  __syncthreads();
-  if (last) {
-    tile_gemv(Li, NB, vsh[0], vsh[1], false, red);
-    if (threadIdx.x < NB) vsh[0][threadIdx.x] = vsh[1][threadIdx.x];
+  const double *L = Lf + wi.s_off;
+  const int t = threadIdx.x;
+  for (int k = k_hi; k >= k_lo; --k) {
+    const double *Li = Dinv + wi.dinv_off + (int64_t)k * NB * NB;
+    const bool last = k == wi.nb - 1;
+    if (t < NB) vsh[0][t] = last ? b[wi.r_off + k * NB + t] : y[wi.r_off + k * NB + t];
     __syncthreads();
+    if (last) {
+      tile_gemv(Li, NB, vsh[0], vsh[1], false, red);
+      if (t < NB) vsh[0][t] = vsh[1][t];
+      __syncthreads();
+    }
+    for (int m = k_hi; m > k; --m) {
+      tile_gemv(L + (int64_t)m * NB * ld + k * NB, ld, xs[k_hi - m], vsh[2], true, red);
+      if (t < NB) vsh[0][t] -= vsh[2][t];
+      __syncthreads();
+    }
+    tile_gemv(Li, NB, vsh[0], xs[k_hi - k], true, red);  // x_k
   }
-  tile_gemv(Li, NB, vsh[0], vsh[1], true, red);  // x_k
-  if (blockIdx.x == 0 && threadIdx.x < NB) x[wi.r_off + k * NB + threadIdx.x] = vsh[1][threadIdx.x];
-  if (k == 0) return;
-  const int ib = blockIdx.x;  // < k
-  tile_gemv(Lf + wi.s_off + (int64_t)k * NB * ld + ib * NB, ld, vsh[1], vsh[2], true, red);
-  if (threadIdx.x < NB) y[wi.r_off + ib * NB + threadIdx.x] -= vsh[2][threadIdx.x];
+  if (blockIdx.x == 0 && t < NB)
+    for (int k = k_lo; k <= k_hi; ++k) x[wi.r_off + k * NB + t] = xs[k_hi - k][t];
+  if (k_lo == 0) return;
+  const int ib = blockIdx.x;  // < k_lo
+  double acc = 0.0;
+  for (int m = k_lo; m <= k_hi; ++m) {
+    tile_gemv(L + (int64_t)m * NB * ld + ib * NB, ld, xs[k_hi - m], vsh[2], true, red);
+    if (t < NB) acc += vsh[2][t];
+  }
+  if (t < NB) y[wi.r_off + ib * NB + t] -= acc;
 }
 
 // ---------------------------------------------------------------- small path
@@ -540,9 +557,11 @@ void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int
       hipLaunchKernelGGL(k_chol_step, dim3(ntiles + rest), dim3(256), 0, st, wi, plan.ws, w, plan.S, plan.Lf,
                          plan.Dinv, plan.bwork, plan.ywork, k, ntiles);
     }
-    for (int k = wi.nb - 1; k >= 0; --k)
-      hipLaunchKernelGGL(k_chol_back, dim3(k > 0 ? k : 1), dim3(256), 0, st, wi, plan.ws, w, plan.Lf, plan.Dinv,
-                         plan.bwork, plan.ywork, plan.dred, k);
+    for (int k_hi = wi.nb - 1; k_hi >= 0; k_hi -= 4) {
+      const int k_lo = k_hi >= 3 ? k_hi - 3 : 0;
+      hipLaunchKernelGGL(k_chol_back, dim3(k_lo > 0 ? k_lo : 1), dim3(256), 0, st, wi, plan.ws, w, plan.Lf,
+                         plan.Dinv, plan.bwork, plan.ywork, plan.dred, k_hi, k_lo);
+    }
   }
 }
 

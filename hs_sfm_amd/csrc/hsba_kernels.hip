@@ -325,12 +325,11 @@ __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
 // fixed butterfly.  Replaces the O(I^2 P) loops of
 // camera_shared_normal_equation_solver.hpp:198-443.
 template <typename T, int E>
-__global__ void __launch_bounds__(256) k_pairs(Dev<T> d, int blocks_per_xcd) {
-  // Workgroups are dealt to the 8 XCDs round-robin; give each XCD a contiguous
-  // range of pairs (pairs are ordered by first image), so the Z rows of the
-  // images it works on stay in that XCD's L2.
-  const int b = (int)(blockIdx.x & 7) * blocks_per_xcd + (int)(blockIdx.x >> 3);
-  const int pair = b * 4 + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
+  constexpr int ZS = z_stride(E, sizeof(T));
+  constexpr int VW = 16 / sizeof(T), NV = (3 * E + VW - 1) / VW;
+  typedef T vec __attribute__((ext_vector_type(VW)));
+  const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (pair >= d.npairs) return;
   if (!win_trial(d.ws[d.pair_win[pair]])) return;
@@ -342,37 +341,43 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d, int blocks_per_xcd) {
 #pragma unroll
   for (int j = 0; j < E; ++j) racc[j] = 0;
   const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
-  for (int64_t j = jb + lane; j < je; j += 64) {
+  // the next incidence's rows are fetched while the current one is multiplied
+  auto fetch = [&](int64_t j, vec za[NV], vec zb[NV], T qv[3]) {
     const uint32_t code = d.inc[j];
     const int a = int(code >> 7);
     const int b = a + int(code & 127u);
-    constexpr int ZS = z_stride(E, sizeof(T));
-    constexpr int VW = 16 / sizeof(T), NV = (3 * E + VW - 1) / VW;
-    typedef T vec __attribute__((ext_vector_type(VW)));
-    const vec *za = (const vec *)(d.Z + (int64_t)a * ZS);
-    const vec *zb = (const vec *)(d.Z + (int64_t)b * ZS);
-    vec va[NV], vb[NV];
+    const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
+    const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
 #pragma unroll
-    for (int q = 0; q < NV; ++q) { va[q] = za[q]; vb[q] = zb[q]; }
-    T A[3][E], B[3][E];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-#pragma unroll
-      for (int c = 0; c < E; ++c) {
-        A[r][c] = va[(r * E + c) / VW][(r * E + c) % VW];
-        B[r][c] = vb[(r * E + c) / VW][(r * E + c) % VW];
-      }
-    }
-#pragma unroll
-    for (int r = 0; r < E; ++r) {
-#pragma unroll
-      for (int c = 0; c < E; ++c) acc[r * E + c] += A[0][r] * B[0][c] + A[1][r] * B[1][c] + A[2][r] * B[2][c];
-    }
+    for (int q = 0; q < NV; ++q) { za[q] = ra[q]; zb[q] = rb[q]; }
     if (diag) {
       const int p = d.key_pt[a];
-      const T q0 = d.q[3 * p], q1 = d.q[3 * p + 1], q2 = d.q[3 * p + 2];
+      qv[0] = d.q[3 * p]; qv[1] = d.q[3 * p + 1]; qv[2] = d.q[3 * p + 2];
+    }
+  };
+  vec va[NV], vb[NV];
+  T qc[3] = {0, 0, 0};
+  int64_t j = jb + lane;
+  if (j < je) fetch(j, va, vb, qc);
+  for (; j < je; j += 64) {
+    vec na[NV], nb[NV];
+    T qn[3] = {0, 0, 0};
+    const bool more = j + 64 < je;
+    if (more) fetch(j + 64, na, nb, qn);
 #pragma unroll
-      for (int r = 0; r < E; ++r) racc[r] += A[0][r] * q0 + A[1][r] * q1 + A[2][r] * q2;
+    for (int r = 0; r < E; ++r) {
+      const T a0 = va[r / VW][r % VW], a1 = va[(E + r) / VW][(E + r) % VW];
+      const T a2 = va[(2 * E + r) / VW][(2 * E + r) % VW];
+#pragma unroll
+      for (int c = 0; c < E; ++c)
+        acc[r * E + c] += a0 * vb[c / VW][c % VW] + a1 * vb[(E + c) / VW][(E + c) % VW] +
+                          a2 * vb[(2 * E + c) / VW][(2 * E + c) % VW];
+      if (diag) racc[r] += a0 * qc[0] + a1 * qc[1] + a2 * qc[2];
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) { va[q] = na[q]; vb[q] = nb[q]; }
+      qc[0] = qn[0]; qc[1] = qn[1]; qc[2] = qn[2];
     }
   }
   for (int m = 1; m < 64; m <<= 1) {
@@ -1142,10 +1147,7 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void pairs(const Dev<T> &d, hipStream_t st) override {
     if (!d.pts_free || E == 0 || d.npairs == 0) return;
-    {
-      const int blocks = (d.npairs + 3) / 4, per_xcd = (blocks + 7) / 8;
-      hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3(8 * per_xcd), dim3(256), 0, st, d, per_xcd);
-    }
+    hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3((d.npairs + 3) / 4), dim3(256), 0, st, d);
   }
   void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) override {
     if (E == 0 || rows == 0) return;

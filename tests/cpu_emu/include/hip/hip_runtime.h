@@ -85,6 +85,12 @@ inline hipemu_dbl4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, hipe
     for (int k = 0; k < 4; ++k) c[r] += A[i0 + 4 * r + 16 * k] * B[j + 16 * k];
   return c;
 }
+// v_readlane: the value of `lane` of this thread's wave.
+inline int __builtin_amdgcn_readlane(int v, int lane) {
+  int out;
+  hipemu::wave_exchange(&v, &out, 4, (hipemu::lane_id() & ~63) | lane);
+  return out;
+}
 // A wave barrier is a real barrier of the wave's 64 threads here.
 inline void __builtin_amdgcn_wave_barrier() { hipemu::wave_sync(); }
 inline void __builtin_amdgcn_fence(int, const char *) { std::atomic_thread_fence(std::memory_order_seq_cst); }
