@@ -7,8 +7,9 @@
 // the identity once.  64 x 64 tiles are staged through LDS.
 //
 // Diagonal tiles: a blocked factorization with 16-wide panels (each 16 x 16
-// diagonal block eliminated by the whole workgroup, one element per thread)
-// yields both L_kk and L_kk^-1; panel tiles are then GEMMs with L_kk^-1.
+// diagonal block eliminated by the whole workgroup, one element per thread,
+// two columns per barrier) yields both L_kk and L_kk^-1; panel tiles are then
+// GEMMs with L_kk^-1.
 //
 // Large windows (more than 4 tiles): one fused launch per panel step k.
 // Every workgroup of step k owns one trailing tile (i, j), k < j <= i: it forms
@@ -82,20 +83,23 @@ __device__ __forceinline__ void wave_sync() {
 constexpr int PB = 16;  // panel width inside a 64 x 64 tile
 
 struct FactorScratch {
-  double col[2][PB];   // column j of the diagonal block being eliminated
-  double xrow[2][PB];  // row j of its inverse
-  double d[PB];        // pivots
+  double col[2][2][PB];   // columns j, j+1 of the diagonal block being eliminated
+  double xrow[2][2][PB];  // rows j, j+1 of its inverse
+  double d[PB];           // pivots
 };
 
 // The whole workgroup (256 threads, one element each): factor the 16 x 16
 // diagonal block of wa at (c0, c0) (lower triangle, SPD) in place into its
 // Cholesky factor L_p, and write L_p^-1 (lower, zero upper) to the same block
-// of wx.  Elimination of [A | I] in LDL^T form:
-//   step j:  d_j = a[j][j];  a[i][c] -= a[i][j] a[c][j] / d_j   (j < c <= i)
-//            x[i][c] -= (a[i][j] / d_j) x[j][c]                  (c <= j < i)
-// leaves a[i][j] = Lt[i][j] d_j and x = Lt^-1, so L = Lt D^1/2 and
-// L^-1 = D^-1/2 x.  Thread t owns element (t / 16, t % 16) of both; column j
-// of a and row j of x go through double-buffered LDS, one barrier per column.
+// of wx.  Elimination of [A | I] in LDL^T form, two columns per barrier:
+// columns j, j+1 and rows j, j+1 of x are published as they stand, and every
+// thread eliminates both itself,
+//   d_j = a[j][j], l = a[j+1][j] / d_j, d_j+1 = a[j+1][j+1] - l a[j+1][j]
+//   f_i = a[i][j] / d_j, u_i = a[i][j+1] - f_i a[j+1][j], g_i = u_i / d_j+1
+//   a[i][c] -= f_i a[c][j] + g_i u_c              (c > j+1, i >= c)
+//   x[i][:] -= f_i x[j][:] + g_i (x[j+1][:] - l x[j][:])   (i > j+1)
+// which leaves a[i][j] = Lt[i][j] d_j and x = Lt^-1, so L = Lt D^1/2 and
+// L^-1 = D^-1/2 x.  Thread t owns element (t / 16, t % 16).
 __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int c0, FactorScratch &fs,
                                        bool &bad) {
   const int tid = threadIdx.x;
@@ -103,18 +107,32 @@ __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int
   double a = (i >= c) ? wa[c0 + i][c0 + c] : 0.0;
   double x = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-  for (int j = 0; j < PB; ++j) {
-    const int b = j & 1;
-    if (c == j) fs.col[b][i] = a;
-    if (i == j) fs.xrow[b][c] = x;
+  for (int j = 0; j < PB; j += 2) {
+    const int b = (j >> 1) & 1;
+    if (c == j) fs.col[b][0][i] = a;
+    if (c == j + 1) fs.col[b][1][i] = a;
+    if (i == j) fs.xrow[b][0][c] = x;
+    if (i == j + 1) fs.xrow[b][1][c] = x;
     __syncthreads();
-    const double aij = fs.col[b][i], acj = fs.col[b][c], d = fs.col[b][j], xjc = fs.xrow[b][c];
-    const bool ok = d > 0.0;
-    bad = bad || !ok;
-    const double f = aij * (ok ? recip_nr(d) : 0.0);
-    if (tid == 0) fs.d[j] = d;
-    a = (i > j && c > j && i >= c) ? a - f * acj : a;
-    x = (i > j && c <= j) ? x - f * xjc : x;
+    const double aij = fs.col[b][0][i], ai1 = fs.col[b][1][i];
+    const double acj = fs.col[b][0][c], ac1 = fs.col[b][1][c];
+    const double d0 = fs.col[b][0][j], e = fs.col[b][0][j + 1], d1r = fs.col[b][1][j + 1];
+    const double x0c = fs.xrow[b][0][c], x1c = fs.xrow[b][1][c];
+    const bool ok0 = d0 > 0.0;
+    const double inv0 = ok0 ? recip_nr(d0) : 0.0;
+    const double l = e * inv0;
+    const double d1 = d1r - e * l;
+    const bool ok1 = d1 > 0.0;
+    const double inv1 = ok1 ? recip_nr(d1) : 0.0;
+    bad = bad || !ok0 || !ok1;
+    if (tid == 0) { fs.d[j] = d0; fs.d[j + 1] = d1; }
+    const double fi = aij * inv0, ui = ai1 - fi * e, gi = ui * inv1;
+    const double fc = acj * inv0, uc = ac1 - fc * e;
+    const double xh = x1c - l * x0c;
+    const bool lower = i >= c;
+    a = (lower && c > j + 1) ? a - (fi * acj + gi * uc) : a;
+    a = (lower && c == j + 1) ? (i == j + 1 ? d1 : ui) : a;
+    x = (i > j + 1) ? x - (fi * x0c + gi * xh) : (i == j + 1 ? xh : x);
   }
   __syncthreads();
   const double dc = fs.d[c] > 0.0 ? fs.d[c] : 1.0, di = fs.d[i] > 0.0 ? fs.d[i] : 1.0;

@@ -341,9 +341,7 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
 #pragma unroll
   for (int j = 0; j < E; ++j) racc[j] = 0;
   const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
-  // the next incidence's rows are fetched while the current one is multiplied
-  auto fetch = [&](int64_t j, vec za[NV], vec zb[NV], T qv[3]) {
-    const uint32_t code = d.inc[j];
+  auto fetch = [&](uint32_t code, vec za[NV], vec zb[NV], T qv[3]) {
     const int a = int(code >> 7);
     const int b = a + int(code & 127u);
     const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
@@ -355,15 +353,7 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
       qv[0] = d.q[3 * p]; qv[1] = d.q[3 * p + 1]; qv[2] = d.q[3 * p + 2];
     }
   };
-  vec va[NV], vb[NV];
-  T qc[3] = {0, 0, 0};
-  int64_t j = jb + lane;
-  if (j < je) fetch(j, va, vb, qc);
-  for (; j < je; j += 64) {
-    vec na[NV], nb[NV];
-    T qn[3] = {0, 0, 0};
-    const bool more = j + 64 < je;
-    if (more) fetch(j + 64, na, nb, qn);
+  auto mult = [&](const vec va[NV], const vec vb[NV], const T qc[3]) {
 #pragma unroll
     for (int r = 0; r < E; ++r) {
       const T a0 = va[r / VW][r % VW], a1 = va[(E + r) / VW][(E + r) % VW];
@@ -374,11 +364,25 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
                           a2 * vb[(2 * E + c) / VW][(2 * E + c) % VW];
       if (diag) racc[r] += a0 * qc[0] + a1 * qc[1] + a2 * qc[2];
     }
+  };
+  vec va[NV], vb[NV];
+  T qc[3] = {0, 0, 0};
+  int64_t j = jb + lane;
+  if (j < je) fetch(d.inc[j], va, vb, qc);
+  uint32_t cn = (j + 64 < je) ? d.inc[j + 64] : 0u;
+  for (; j < je; j += 64) {
+    vec na[NV], nb[NV];
+    T qn[3] = {0, 0, 0};
+    const bool more = j + 64 < je;
+    const uint32_t cnn = (j + 128 < je) ? d.inc[j + 128] : 0u;
+    if (more) fetch(cn, na, nb, qn);
+    mult(va, vb, qc);
     if (more) {
 #pragma unroll
       for (int q = 0; q < NV; ++q) { va[q] = na[q]; vb[q] = nb[q]; }
       qc[0] = qn[0]; qc[1] = qn[1]; qc[2] = qn[2];
     }
+    cn = cnn;
   }
   for (int m = 1; m < 64; m <<= 1) {
 #pragma unroll

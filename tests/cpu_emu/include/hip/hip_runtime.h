@@ -85,6 +85,18 @@ inline hipemu_dbl4 __builtin_amdgcn_mfma_f64_16x16x4f64(double a, double b, hipe
     for (int k = 0; k < 4; ++k) c[r] += A[i0 + 4 * r + 16 * k] * B[j + 16 * k];
   return c;
 }
+// v_mfma_f32_16x16x4f32: operands as above; D[i][j] in lane j + 16 (i / 4),
+// element i % 4.
+typedef float hipemu_flt4 __attribute__((ext_vector_type(4)));
+inline hipemu_flt4 __builtin_amdgcn_mfma_f32_16x16x4f32(float a, float b, hipemu_flt4 c, int, int, int) {
+  float A[64], B[64];
+  hipemu::wave_gather(&a, A, 4);
+  hipemu::wave_gather(&b, B, 4);
+  const int l = hipemu::lane_id() & 63, j = l & 15, i0 = 4 * (l >> 4);
+  for (int r = 0; r < 4; ++r)
+    for (int k = 0; k < 4; ++k) c[r] += A[i0 + r + 16 * k] * B[j + 16 * k];
+  return c;
+}
 // v_readlane: the value of `lane` of this thread's wave.
 inline int __builtin_amdgcn_readlane(int v, int lane) {
   int out;
