@@ -404,6 +404,115 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
   }
 }
 
+// K4 on the matrix cores for wide camera blocks (7 <= E <= 15).  For a pair
+// (a, b), P_ab = A^T B where A and B stack the Z rows of the shared points (3
+// per point); for a diagonal pair column E of B carries q_p, so column E of
+// the product is rp_i.  One workgroup per pair walks its incidences 64 at a
+// time: the 256 threads gather the 128 Z rows of a stage into LDS with
+// 16-byte loads (4 threads per incidence), then each wavefront multiplies 16
+// incidences (48 stacked rows) with v_mfma_{f32,f64}_16x16x4, lane i + 16k
+// supplying A^T[i][k] = Z_a[row k][i] and B[k][i].  The product P[i][j] lands
+// in lane j + 16 (i / 4), element i % 4 for the f32 form and in lane
+// j + 16 (i % 4), element i / 4 for the f64 form (both checked on the
+// device).  The four wavefronts' partial products are summed in a fixed order.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <typename T>
+struct Mma;
+template <>
+struct Mma<float> {
+  typedef f32x4 acc_t;
+  static __device__ __forceinline__ int row(int kr, int e) { return 4 * kr + e; }
+  static __device__ __forceinline__ acc_t op(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mma<double> {
+  typedef f64x4 acc_t;
+  static __device__ __forceinline__ int row(int kr, int e) { return kr + 4 * e; }
+  static __device__ __forceinline__ acc_t op(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d) {
+  static_assert(E >= 7 && E <= 15, "wide camera blocks, rhs column");
+  constexpr int ZS = z_stride(E, sizeof(T));
+  constexpr int VW = 16 / sizeof(T), NV = ZS / VW;
+  constexpr int SI = 64;  // incidences per stage
+  typedef T vec __attribute__((ext_vector_type(VW)));
+  typedef typename Mma<T>::acc_t acc_t;
+  __shared__ vec sz[2][SI][NV];
+  __shared__ T sq[SI][4];
+  __shared__ T part[3][64][4];
+  const int pair = blockIdx.x;
+  if (!win_trial(d.ws[d.pair_win[pair]])) return;
+  const bool diag = d.pair_img[pair] >= 0;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int col = lane & 15, kr = lane >> 4;
+  const int ls = t >> 2, lq = t & 3;  // loader: incidence, quarter of its 2 NV vectors
+  const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
+  acc_t acc = {0, 0, 0, 0};
+  for (int64_t base = jb; base < je; base += SI) {
+    const int n = je - base < SI ? (int)(je - base) : SI;
+    if (ls < n) {
+      const uint32_t code = d.inc[base + ls];
+      const int a = int(code >> 7), b = a + int(code & 127u);
+      const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
+      const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
+      vec v[2 * NV / 4];
+#pragma unroll
+      for (int u = 0; u < 2 * NV / 4; ++u) {
+        const int q = lq + 4 * u;
+        v[u] = q < NV ? ra[q] : rb[q - NV];
+      }
+#pragma unroll
+      for (int u = 0; u < 2 * NV / 4; ++u) {
+        const int q = lq + 4 * u;
+        sz[q < NV ? 0 : 1][ls][q < NV ? q : q - NV] = v[u];
+      }
+      if (diag && lq == 0) {
+        const int p = d.key_pt[a];
+        sq[ls][0] = d.q[3 * p]; sq[ls][1] = d.q[3 * p + 1]; sq[ls][2] = d.q[3 * p + 2];
+      }
+    }
+    __syncthreads();
+    const int rows = 3 * max(0, min(16, n - 16 * wave));
+    for (int g0 = 0; g0 < rows; g0 += 4) {
+      const int g = g0 + kr;
+      T av = 0, bv = 0;
+      if (g < rows) {
+        const int s = 16 * wave + g / 3, r = g - 3 * (g / 3);
+        const T *za = (const T *)sz[0][s], *zb = (const T *)sz[1][s];
+        if (col < E) {
+          av = za[r * E + col];
+          bv = zb[r * E + col];
+        } else if (diag && col == E) {
+          bv = sq[s][r];
+        }
+      }
+      acc = Mma<T>::op(av, bv, acc);
+    }
+    __syncthreads();
+  }
+  if (wave > 0)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) part[wave - 1][lane][e] = acc[e];
+  __syncthreads();
+  if (wave > 0) return;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const T v = acc[e] + part[0][lane][e] + part[1][lane][e] + part[2][lane][e];
+    const int row = Mma<T>::row(kr, e);
+    if (row < E) {
+      if (col < E) d.pair_out[(int64_t)pair * E * E + row * E + col] = v;
+      else if (diag && col == E) d.rp[(int64_t)d.pair_img[pair] * E + row] = v;
+    }
+  }
+}
+
 // ------------------------------------------------------------ assembly
 // One thread per lower-triangle entry of a window's reduced matrix (FP64):
 // S = U + mu I (+ image-camera coupling) - sum of pair blocks, by the
@@ -1151,7 +1260,10 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void pairs(const Dev<T> &d, hipStream_t st) override {
     if (!d.pts_free || E == 0 || d.npairs == 0) return;
-    hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3((d.npairs + 3) / 4), dim3(256), 0, st, d);
+    if constexpr (E >= 7 && E <= 15)
+      hipLaunchKernelGGL((k_pairs_mma<T, E>), dim3(d.npairs), dim3(256), 0, st, d);
+    else
+      hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3((d.npairs + 3) / 4), dim3(256), 0, st, d);
   }
   void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) override {
     if (E == 0 || rows == 0) return;
