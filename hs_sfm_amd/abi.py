@@ -51,7 +51,8 @@ class hsba_problem_desc(C.Structure):
 
 
 class hsba_ycov_inverse(C.Structure):
-    _fields_ = [("key_blocks", _dp), ("constraint_weights", _dp)]
+    _fields_ = [("key_blocks", _dp), ("constraint_weights", _dp),
+                ("point_constraint_blocks", _dp)]
 
 
 class hsba_options(C.Structure):
@@ -93,6 +94,11 @@ class hsba_server_options(C.Structure):
 class hsba_naive_desc(C.Structure):
     _fields_ = [("num_cameras", C.c_int64), ("num_points", C.c_int64), ("num_features", C.c_int64),
                 ("feature_maps", C.POINTER(C.c_int32))]
+
+
+class hsba_gcp_desc(C.Structure):
+    _fields_ = [("num_cameras", C.c_int64), ("num_points", C.c_int64), ("num_features", C.c_int64),
+                ("feature_maps", C.POINTER(C.c_int32)), ("num_gcps", C.c_int64)]
 
 
 class hsba_job_result(C.Structure):
@@ -153,7 +159,9 @@ class Desc:
         self.kb = arr(scene.key_blocks.reshape(-1) if scene.key_blocks is not None else None,
                       np.float64)
         self.cw = arr(scene.constraint_weights, np.float64)
-        self.ycov = hsba_ycov_inverse(dptr(self.kb), dptr(self.cw))
+        pb = getattr(scene, "point_constraint_blocks", None)
+        self.pb = arr(None if pb is None else np.asarray(pb).reshape(-1), np.float64)
+        self.ycov = hsba_ycov_inverse(dptr(self.kb), dptr(self.cw), dptr(self.pb))
         self._keep = keep
 
 
@@ -214,6 +222,16 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.hsba_naive_solve.argtypes = [P, C.c_double, _dp]
     lib.hsba_naive_lm_solve.argtypes = [P, _dp, _dp, _dp, C.POINTER(hsba_lm_options), _dp,
                                         C.POINTER(hsba_lm_stats)]
+    lib.hsba_gcp_problem_create.argtypes = [C.POINTER(hsba_gcp_desc), C.POINTER(hsba_options),
+                                            C.POINTER(P)]
+    lib.hsba_gcp_problem_destroy.argtypes = [P]
+    lib.hsba_gcp_evaluate.argtypes = [P, _dp, _dp]
+    lib.hsba_gcp_jacobian.argtypes = [P, _dp, _dp, _dp]
+    lib.hsba_gcp_build_normal.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    lib.hsba_gcp_solve.argtypes = [P, C.c_double, _dp]
+    lib.hsba_gcp_mahalanobis.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
+    lib.hsba_gcp_lm_solve.argtypes = [P, _dp, _dp, _dp, _dp, C.POINTER(hsba_lm_options), _dp,
+                                      C.POINTER(hsba_lm_stats)]
     _LIB = lib
     return lib
 
@@ -228,7 +246,9 @@ EXPORTED_SYMBOLS = [
     "hsba_server_destroy", "hsba_server_submit", "hsba_server_poll", "hsba_server_drain",
     "hsba_server_clock", "hsba_naive_problem_create", "hsba_naive_problem_destroy",
     "hsba_naive_evaluate", "hsba_naive_jacobian", "hsba_naive_build_normal", "hsba_naive_solve",
-    "hsba_naive_lm_solve",
+    "hsba_naive_lm_solve", "hsba_gcp_problem_create", "hsba_gcp_problem_destroy", "hsba_gcp_evaluate",
+    "hsba_gcp_jacobian", "hsba_gcp_build_normal", "hsba_gcp_solve", "hsba_gcp_mahalanobis",
+    "hsba_gcp_lm_solve",
 ]
 
 

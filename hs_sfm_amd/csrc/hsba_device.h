@@ -47,8 +47,14 @@ struct ConstraintRow {  // y-order constraint row
   int32_t entity;       // global point / image / camera id
   int32_t param;        // index inside the entity's x block
   int32_t win;
-  double weight;
+  double weight;        // diagonal weight (block == 0)
   double y;             // observed value
+  // block != 0: one of the three rows of a point constraint weighted by a
+  // full 3x3 block; brow = row `param` of the block, the other two rows of
+  // the record are the neighbouring entries (index - param + j).
+  double brow[3];
+  int32_t block;
+  int32_t rec_mask;     // coordinate mask of the point constraint record this row belongs to
 };
 
 struct Contrib {        // one term of a reduced-system block

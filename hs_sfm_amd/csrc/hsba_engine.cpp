@@ -307,10 +307,26 @@ struct Engine final : EngineBase {
           cov[3 * k] = l00; cov[3 * k + 1] = l10; cov[3 * k + 2] = l11;
         }
       }
+      const double *pb = ycovs ? ycovs[w].point_constraint_blocks : nullptr;
       for (int q = wi.con_begin; q < wi.con_end; ++q) {
         const int local = q - wi.con_begin;
-        L.cons[q].y = y[2 * h.N + local];
-        L.cons[q].weight = (ycovs && ycovs[w].constraint_weights) ? ycovs[w].constraint_weights[local] : 1.0;
+        ConstraintRow &cr = L.cons[q];
+        cr.y = y[2 * h.N + local];
+        cr.weight = (ycovs && ycovs[w].constraint_weights) ? ycovs[w].constraint_weights[local] : 1.0;
+        cr.block = 0;
+        cr.brow[0] = cr.brow[1] = cr.brow[2] = 0.0;
+        if (pb && cr.kind == 0) {
+          // point constraints come first in y order; each must hold all three
+          // coordinates, so rows local, local+1, local+2 form one record
+          if (cr.rec_mask != 7) return HSBA_ERR_INVALID;
+          const int r0 = local - cr.param;
+          const double *B = pb + 3 * r0;  // record r0 / 3, row-major 3x3
+          for (int j = 0; j < 3; ++j) {
+            if (B[3 * cr.param + j] != B[3 * j + cr.param]) return HSBA_ERR_INVALID;  // symmetric
+            cr.brow[j] = B[3 * cr.param + j];
+          }
+          cr.block = 1;
+        }
       }
     }
     int rc = upload((double *)d.key_uv, uv, st);

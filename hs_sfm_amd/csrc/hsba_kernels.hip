@@ -223,7 +223,16 @@ __global__ void k_constraints(Dev<T> d) {
     const ConstraintRow cr = d.cons[q];
     const WinState &s = d.ws[cr.win];
     if (!(s.active && s.need_lin)) continue;
-    if (cr.kind == 0) {
+    if (cr.kind == 0 && cr.block) {
+      // row `param` of a 3x3 block over the record's three rows: N gets the
+      // block's upper triangle once (V holds 00 01 02 11 12 22), g gets B r
+      // (ba_gcp_constrained_normal_equation_builder.hpp:64-71)
+      const ConstraintRow *rec = d.cons + (q - cr.param);
+      double t = 0.0;
+      for (int j = 0; j < 3; ++j) t += cr.brow[j] * (d.pts[3 * cr.entity + j] - rec[j].y);
+      for (int j = cr.param; j < 3; ++j) d.V[6 * cr.entity + (cr.param == 0 ? j : (cr.param == 1 ? j + 2 : 5))] += T(cr.brow[j]);
+      d.gp[3 * cr.entity + cr.param] += T(t);
+    } else if (cr.kind == 0) {
       const double r = d.pts[3 * cr.entity + cr.param] - cr.y;
       const int vi = cr.param == 0 ? 0 : (cr.param == 1 ? 3 : 5);
       d.V[6 * cr.entity + vi] += T(cr.weight);
@@ -887,6 +896,16 @@ __global__ void k_constraint_cost(Dev<T> d, int use_new) {
     WinState &s = d.ws[cr.win];
     const bool act = use_new ? (s.active && s.trial && s.take_step) : (s.active != 0);
     if (!act) continue;
+    if (cr.kind == 0 && cr.block) {
+      // r_param * (B r)_param: summed over the record's rows this is r^T B r
+      // (ba_gcp_constrained_mahalanobis_distance_calculator.hpp:50-58)
+      const double *X = (use_new ? d.pts_new : d.pts) + 3 * cr.entity;
+      const ConstraintRow *rec = d.cons + (q - cr.param);
+      double t = 0.0;
+      for (int j = 0; j < 3; ++j) t += cr.brow[j] * (X[j] - rec[j].y);
+      s.cost_new += (X[cr.param] - cr.y) * t;
+      continue;
+    }
     double x;
     if (cr.kind == 0) x = (use_new ? d.pts_new : d.pts)[3 * cr.entity + cr.param];
     else if (cr.kind == 1) x = (use_new ? d.imgs_new : d.imgs)[6 * cr.entity + cr.param];

@@ -359,6 +359,7 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
     auto push = [&](int kind, int64_t ent_local, int param, int64_t xidx) {
       ConstraintRow cr;
       cr.kind = kind; cr.param = param; cr.win = (int32_t)w; cr.weight = 1.0; cr.y = 0.0;
+      cr.brow[0] = cr.brow[1] = cr.brow[2] = 0.0; cr.block = 0; cr.rec_mask = 0;
       cr.entity = (int32_t)(ent_local + (kind == 0 ? pt_off : (kind == 1 ? img_off : cam_off)));
       L.cons.push_back(cr);
       h.cons_xidx.push_back(xidx);
@@ -368,7 +369,11 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
       for (int64_t q = 0; q < ds.num_point_constraints; ++q) {
         const hsba_constraint &c = ds.point_constraints[q];
         if (c.id < 0 || c.id >= h.P) return fail(why, "point constraint id");
-        for (int b = 0; b < 3; ++b) if (c.mask & (1u << b)) push(0, c.id, b, h.x_pt + 3 * c.id + b);
+        for (int b = 0; b < 3; ++b)
+          if (c.mask & (1u << b)) {
+            push(0, c.id, b, h.x_pt + 3 * c.id + b);
+            L.cons.back().rec_mask = (int32_t)(c.mask & 7u);
+          }
       }
     if (L.imgs_free)
       for (int64_t q = 0; q < ds.num_image_constraints; ++q) {

@@ -18,6 +18,12 @@
 // (ba_naive_analytical_jacobian_matrix_calculator.hpp:104-108); the entry
 // points that evaluate it here report the same condition as
 // HSBA_ERR_INVALID.
+//
+// ba_gcp_constrained (ba_gcp_constrained_*.hpp) is ba_naive whose last G
+// points are ground control points with observed coordinates: y gains 3G
+// rows equal to those points' x entries, weighted by 3x3 blocks.  These are
+// camera_shared point constraints on all three coordinates with full 3x3
+// weight blocks (hsba_ycov_inverse::point_constraint_blocks), in GCP order.
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -27,12 +33,14 @@
 
 struct hsba_naive_problem {
   hsba_problem *p = nullptr;
-  int64_t C = 0, P = 0, F = 0;
+  int64_t C = 0, P = 0, F = 0, G = 0;
   std::vector<double> xs;  // camera_shared-layout scratch
   ~hsba_naive_problem() {
     if (p) hsba_problem_destroy(p);
   }
 };
+
+struct hsba_gcp_problem : hsba_naive_problem {};
 
 namespace {
 
@@ -53,6 +61,34 @@ bool zero_rotation(const hsba_naive_problem &q, const double *x) {
   return false;
 }
 
+// The camera_shared problem behind a naive (G = 0) or GCP-constrained one.
+int create_shared(hsba_naive_problem &q, const int32_t *feature_maps, const hsba_options *options) {
+  std::vector<int32_t> icm(q.C);
+  for (int64_t c = 0; c < q.C; ++c) icm[c] = (int32_t)c;
+  std::vector<hsba_constraint> gcps(q.G);
+  for (int64_t i = 0; i < q.G; ++i) {
+    gcps[i].id = q.P - q.G + i;
+    gcps[i].mask = 7u;  // X, Y and Z observed
+    gcps[i].reserved_ = 0;
+  }
+  hsba_problem_desc d;
+  std::memset(&d, 0, sizeof(d));
+  d.num_points = q.P;
+  d.num_images = q.C;
+  d.num_cameras = q.C;
+  d.num_keys = q.F;
+  d.feature_maps = feature_maps;       // (camera, point) == (image, point)
+  d.image_camera_map = icm.data();
+  d.intrinsic_computations_mask = 0;   // identity intrinsics: normalised coordinates
+  d.fix_mask = 0;
+  d.num_point_constraints = q.G;
+  d.point_constraints = q.G ? gcps.data() : nullptr;
+  const int rc = hsba_problem_create(&d, options, &q.p);
+  if (rc) return rc;
+  q.xs.resize(3 * q.P + 6 * q.C);
+  return HSBA_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -67,21 +103,8 @@ int hsba_naive_problem_create(const hsba_naive_desc *desc, const hsba_options *o
   q->C = desc->num_cameras;
   q->P = desc->num_points;
   q->F = desc->num_features;
-  std::vector<int32_t> icm(q->C);
-  for (int64_t c = 0; c < q->C; ++c) icm[c] = (int32_t)c;
-  hsba_problem_desc d;
-  std::memset(&d, 0, sizeof(d));
-  d.num_points = q->P;
-  d.num_images = q->C;
-  d.num_cameras = q->C;
-  d.num_keys = q->F;
-  d.feature_maps = desc->feature_maps;  // (camera, point) == (image, point)
-  d.image_camera_map = icm.data();
-  d.intrinsic_computations_mask = 0;   // identity intrinsics: normalised coordinates
-  d.fix_mask = 0;
-  const int rc = hsba_problem_create(&d, options, &q->p);
+  const int rc = create_shared(*q, desc->feature_maps, options);
   if (rc) return rc;
-  q->xs.resize(3 * q->P + 6 * q->C);
   *problem = q.release();
   return HSBA_OK;
 }
@@ -110,7 +133,7 @@ int hsba_naive_build_normal(hsba_naive_problem *q, const double *x, const double
   if (!q || !x || !y_obs) return HSBA_ERR_INVALID;
   if (zero_rotation(*q, x)) return HSBA_ERR_INVALID;
   to_shared(*q, x, q->xs.data());
-  hsba_ycov_inverse w = {feature_cov_inverse, nullptr};
+  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, nullptr};
   std::vector<double> gs(gradient ? q->xs.size() : 0);
   const int rc = hsba_build_normal(q->p, q->xs.data(), y_obs, &w, point_blocks, camera_blocks, nullptr,
                                    gradient ? gs.data() : nullptr);
@@ -132,7 +155,78 @@ int hsba_naive_lm_solve(hsba_naive_problem *q, const double *x0, const double *y
   if (!q || !x0 || !y_obs || !lm || !x_out) return HSBA_ERR_INVALID;
   if (zero_rotation(*q, x0)) return HSBA_ERR_INVALID;
   to_shared(*q, x0, q->xs.data());
-  hsba_ycov_inverse w = {feature_cov_inverse, nullptr};
+  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, nullptr};
+  std::vector<double> xo(q->xs.size());
+  const int rc = hsba_lm_solve(q->p, q->xs.data(), y_obs, &w, lm, xo.data(), stats);
+  to_naive(*q, xo.data(), x_out);
+  return rc;
+}
+
+// ---- ba_gcp_constrained -------------------------------------------------
+
+int hsba_gcp_problem_create(const hsba_gcp_desc *desc, const hsba_options *options,
+                            hsba_gcp_problem **problem) {
+  if (!desc || !problem || !desc->feature_maps || desc->num_cameras <= 0 || desc->num_points < 0 ||
+      desc->num_features < 0 || desc->num_gcps < 0 || desc->num_gcps > desc->num_points)
+    return HSBA_ERR_INVALID;
+  *problem = nullptr;
+  std::unique_ptr<hsba_gcp_problem> q(new hsba_gcp_problem());
+  q->C = desc->num_cameras;
+  q->P = desc->num_points;
+  q->F = desc->num_features;
+  q->G = desc->num_gcps;
+  const int rc = create_shared(*q, desc->feature_maps, options);
+  if (rc) return rc;
+  *problem = q.release();
+  return HSBA_OK;
+}
+
+int hsba_gcp_problem_destroy(hsba_gcp_problem *problem) {
+  delete problem;
+  return HSBA_OK;
+}
+
+int hsba_gcp_evaluate(hsba_gcp_problem *q, const double *x, double *y) {
+  // camera_shared y = [keys 2F | point constraint rows], i.e. the GCP
+  // coordinates in GCP order
+  return hsba_naive_evaluate(q, x, y);
+}
+
+int hsba_gcp_jacobian(hsba_gcp_problem *q, const double *x, double *camera_blocks, double *point_blocks) {
+  return hsba_naive_jacobian(q, x, camera_blocks, point_blocks);
+}
+
+int hsba_gcp_build_normal(hsba_gcp_problem *q, const double *x, const double *y_obs,
+                          const double *feature_cov_inverse, const double *gcp_blocks, double *camera_blocks,
+                          double *point_blocks, double *gradient) {
+  if (!q || !x || !y_obs) return HSBA_ERR_INVALID;
+  if (zero_rotation(*q, x)) return HSBA_ERR_INVALID;
+  to_shared(*q, x, q->xs.data());
+  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
+  std::vector<double> gs(gradient ? q->xs.size() : 0);
+  const int rc = hsba_build_normal(q->p, q->xs.data(), y_obs, &w, point_blocks, camera_blocks, nullptr,
+                                   gradient ? gs.data() : nullptr);
+  if (!rc && gradient) to_naive(*q, gs.data(), gradient);
+  return rc;
+}
+
+int hsba_gcp_solve(hsba_gcp_problem *q, double mu, double *delta_x) { return hsba_naive_solve(q, mu, delta_x); }
+
+int hsba_gcp_mahalanobis(hsba_gcp_problem *q, const double *x, const double *y_obs,
+                         const double *feature_cov_inverse, const double *gcp_blocks, double *value) {
+  if (!q || !x || !y_obs || !value) return HSBA_ERR_INVALID;
+  to_shared(*q, x, q->xs.data());
+  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
+  return hsba_mahalanobis(q->p, q->xs.data(), y_obs, &w, value);
+}
+
+int hsba_gcp_lm_solve(hsba_gcp_problem *q, const double *x0, const double *y_obs,
+                      const double *feature_cov_inverse, const double *gcp_blocks, const hsba_lm_options *lm,
+                      double *x_out, hsba_lm_stats *stats) {
+  if (!q || !x0 || !y_obs || !lm || !x_out) return HSBA_ERR_INVALID;
+  if (zero_rotation(*q, x0)) return HSBA_ERR_INVALID;
+  to_shared(*q, x0, q->xs.data());
+  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
   std::vector<double> xo(q->xs.size());
   const int rc = hsba_lm_solve(q->p, q->xs.data(), y_obs, &w, lm, xo.data(), stats);
   to_naive(*q, xo.data(), x_out);

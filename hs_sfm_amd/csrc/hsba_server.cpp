@@ -111,7 +111,7 @@ struct Job {
   std::vector<int32_t> fm, icm;
   std::vector<double> intr, fixp, fixi, fixc;
   std::vector<hsba_constraint> pc, ic, cc;
-  std::vector<double> x0, y, kb, cw;
+  std::vector<double> x0, y, kb, cw, pb;
   hsba_ycov_inverse ycov;
   double *x_out = nullptr;
   hsba_job_result res;
@@ -294,6 +294,8 @@ int hsba_server_submit(hsba_server *s, const hsba_problem_desc *desc, const doub
     copy_in(j->cw, ycov->constraint_weights, yn - 2 * d.num_keys);
     j->ycov.key_blocks = j->kb.empty() ? nullptr : j->kb.data();
     j->ycov.constraint_weights = j->cw.empty() ? nullptr : j->cw.data();
+    copy_in(j->pb, ycov->point_constraint_blocks, 9 * d.num_point_constraints);
+    j->ycov.point_constraint_blocks = j->pb.empty() ? nullptr : j->pb.data();
   }
   j->desc = d;
   j->desc.feature_maps = j->fm.data();

@@ -264,6 +264,81 @@ class NaiveProblem:
         return rc, xo, st
 
 
+class GCPProblem:
+    """ba_gcp_constrained problem over the C ABI: ba_naive whose last
+    num_gcps points carry observed coordinates with 3x3 inverse covariances;
+    y = [features 2F | GCP coordinates 3G]
+    (ba_gcp_constrained_vector_function.hpp:73-113)."""
+
+    def __init__(self, num_cameras, num_points, feature_maps, num_gcps, precision=abi.PRECISION_FP64):
+        self.lib = abi.load_library()
+        self.C, self.P, self.G = int(num_cameras), int(num_points), int(num_gcps)
+        self.fm = np.ascontiguousarray(feature_maps, np.int32)
+        self.F = self.fm.shape[0]
+        d = abi.hsba_gcp_desc(self.C, self.P, self.F, self.fm.ctypes.data_as(C.POINTER(C.c_int32)), self.G)
+        self.h = C.c_void_p()
+        o = _opts(precision)
+        self.last_rc = self.lib.hsba_gcp_problem_create(C.byref(d), C.byref(o), C.byref(self.h))
+        abi.check(self.last_rc, "hsba_gcp_problem_create")
+        self.x_size = 6 * self.C + 3 * self.P
+        self.y_size = 2 * self.F + 3 * self.G
+
+    def close(self):
+        if self.h:
+            self.lib.hsba_gcp_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _arr(a):
+        return None if a is None else np.ascontiguousarray(a, np.float64)
+
+    def evaluate(self, x):
+        y = np.zeros(self.y_size)
+        abi.check(self.lib.hsba_gcp_evaluate(self.h, abi.dptr(self._arr(x)), abi.dptr(y)), "hsba_gcp_evaluate")
+        return y
+
+    def jacobian(self, x):
+        """Returns (rc, camera blocks [F,2,6], point blocks [F,2,3])."""
+        jc, jp = np.zeros((self.F, 2, 6)), np.zeros((self.F, 2, 3))
+        rc = self.lib.hsba_gcp_jacobian(self.h, abi.dptr(self._arr(x)), abi.dptr(jc), abi.dptr(jp))
+        return rc, jc, jp
+
+    def build_normal(self, x, y_obs, cov=None, gcp_blocks=None):
+        U, V, g = np.zeros((self.C, 6, 6)), np.zeros((self.P, 3, 3)), np.zeros(self.x_size)
+        rc = self.lib.hsba_gcp_build_normal(self.h, abi.dptr(self._arr(x)), abi.dptr(self._arr(y_obs)),
+                                            abi.dptr(self._arr(cov)), abi.dptr(self._arr(gcp_blocks)),
+                                            abi.dptr(U), abi.dptr(V), abi.dptr(g))
+        abi.check(rc, "hsba_gcp_build_normal")
+        return U, V, g
+
+    def solve(self, mu):
+        dx = np.zeros(self.x_size)
+        rc = self.lib.hsba_gcp_solve(self.h, mu, abi.dptr(dx))
+        return rc, dx
+
+    def mahalanobis(self, x, y_obs, cov=None, gcp_blocks=None):
+        v = C.c_double()
+        rc = self.lib.hsba_gcp_mahalanobis(self.h, abi.dptr(self._arr(x)), abi.dptr(self._arr(y_obs)),
+                                           abi.dptr(self._arr(cov)), abi.dptr(self._arr(gcp_blocks)), C.byref(v))
+        abi.check(rc, "hsba_gcp_mahalanobis")
+        return v.value
+
+    def lm(self, x0, y_obs, cov=None, gcp_blocks=None, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0):
+        xo = np.zeros(self.x_size)
+        st = abi.hsba_lm_stats()
+        opt = abi.hsba_lm_options(max_iterations=max_iterations, tau=tau, eps1=eps1, eps2=eps2)
+        rc = self.lib.hsba_gcp_lm_solve(self.h, abi.dptr(self._arr(x0)), abi.dptr(self._arr(y_obs)),
+                                        abi.dptr(self._arr(cov)), abi.dptr(self._arr(gcp_blocks)), C.byref(opt),
+                                        abi.dptr(xo), C.byref(st))
+        return rc, xo, st
+
+
 class Server:
     """Serving loop over the C ABI (continuous batching of independent jobs).
 

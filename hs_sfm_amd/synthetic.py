@@ -63,6 +63,7 @@ class Scene:
     y_obs: Optional[np.ndarray] = None
     key_blocks: Optional[np.ndarray] = None          # [N, 2, 2] or None = identity
     constraint_weights: Optional[np.ndarray] = None
+    point_constraint_blocks: Optional[np.ndarray] = None  # [n, 3, 3]: full weights of mask-7 point constraints
     name: str = ""
 
     @property

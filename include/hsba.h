@@ -112,6 +112,14 @@ typedef struct {
 typedef struct {
   const double *key_blocks;         /* [num_keys][2][2] row-major, NULL = identity */
   const double *constraint_weights; /* [num constraint rows], NULL = all 1 */
+  /* Full inverse covariance of each point constraint, [num_point_constraints][3][3]
+   * row-major and symmetric, or NULL.  When given, every point constraint must
+   * constrain all three coordinates; its three rows then take this block and
+   * their constraint_weights entries are ignored.  This is the GCP weighting of
+   * BAGCPConstrainedYCovarianceInverse::gcp_blocks
+   * (ba_gcp_constrained_y_covariance_inverse.hpp:14-23); camera_shared itself
+   * only has diagonal constraint weights. */
+  const double *point_constraint_blocks;
 } hsba_ycov_inverse;
 
 #define HSBA_PRECISION_FP64   0     /* every kernel in FP64 ("GPU-plain")   */
@@ -289,6 +297,60 @@ int hsba_naive_solve(hsba_naive_problem *problem, double mu, double *delta_x);
 int hsba_naive_lm_solve(hsba_naive_problem *problem, const double *x0, const double *y_obs,
                         const double *feature_cov_inverse, const hsba_lm_options *lm,
                         double *x_out, hsba_lm_stats *stats);
+
+/* ---- ba_gcp_constrained family ------------------------------------------ */
+/* ba_naive plus ground-control-point priors: the last num_gcps points are
+ * GCPs whose coordinates are also observed, so
+ *   y = [features 2F | GCP coordinates 3G]
+ * (ba_gcp_constrained_vector_function.hpp:73-113) with a 3x3 inverse
+ * covariance per GCP (gcp_blocks, [G][3][3], symmetric).  The GCP rows of the
+ * Jacobian are identity rows on the GCP points
+ * (ba_gcp_constrained_jacobian_matrix.hpp:50-71); the builder adds each block
+ * to its point block and B r to its gradient segment
+ * (ba_gcp_constrained_normal_equation_builder.hpp:59-72).  x is the naive
+ * layout [cameras 6C | points 3P]. */
+typedef struct {
+  int64_t num_cameras;
+  int64_t num_points;               /* including the GCPs */
+  int64_t num_features;
+  const int32_t *feature_maps;      /* [num_features][2] = (camera_id, point_id) */
+  int64_t num_gcps;                 /* the last num_gcps points */
+} hsba_gcp_desc;
+
+typedef struct hsba_gcp_problem hsba_gcp_problem;
+
+/* BAGCPConstrainedVectorFunction ctor (ba_gcp_constrained_vector_function.hpp:43-50). */
+int hsba_gcp_problem_create(const hsba_gcp_desc *desc, const hsba_options *options,
+                            hsba_gcp_problem **problem);
+int hsba_gcp_problem_destroy(hsba_gcp_problem *problem);
+/* BAGCPConstrainedVectorFunction::operator() (ba_gcp_constrained_vector_function.hpp:73-113):
+ * y [2F + 3G]. */
+int hsba_gcp_evaluate(hsba_gcp_problem *problem, const double *x, double *y);
+/* BAGCPConstrainedAnalyticalJacobianMatrixCalculator::operator()
+ * (ba_gcp_constrained_analytical_jacobian_matrix_calculator.hpp:39-49): the
+ * naive feature blocks ([F][2][6] camera, [F][2][3] point); the GCP rows are
+ * the implicit identity rows.  HSBA_ERR_INVALID at a zero camera rotation. */
+int hsba_gcp_jacobian(hsba_gcp_problem *problem, const double *x,
+                      double *camera_blocks, double *point_blocks);
+/* BAGCPConstrainedNormalEquationBuilder::operator()
+ * (ba_gcp_constrained_normal_equation_builder.hpp:42-76): camera blocks
+ * [C][6][6], point blocks [P][3][3], gradient [6C + 3P]. */
+int hsba_gcp_build_normal(hsba_gcp_problem *problem, const double *x, const double *y_obs,
+                          const double *feature_cov_inverse, const double *gcp_blocks,
+                          double *camera_blocks, double *point_blocks, double *gradient);
+/* BANaiveNormalEquationSolver on the last build (the GCP family reuses the
+ * naive solver, ba_gcp_constrained_nllso_meta_levenberg_marquardt.hpp:21). */
+int hsba_gcp_solve(hsba_gcp_problem *problem, double mu, double *delta_x);
+/* BAGCPConstrainedMahalanobisDistanceCalculator::operator()
+ * (ba_gcp_constrained_mahalanobis_distance_calculator.hpp:33-62) at x. */
+int hsba_gcp_mahalanobis(hsba_gcp_problem *problem, const double *x, const double *y_obs,
+                         const double *feature_cov_inverse, const double *gcp_blocks,
+                         double *value);
+/* BAGCPConstrainedLevenbergMarquardtOptimizor::operator()
+ * (ba_gcp_constrained_levenberg_marquardt_optimizor.hpp:44-62). */
+int hsba_gcp_lm_solve(hsba_gcp_problem *problem, const double *x0, const double *y_obs,
+                      const double *feature_cov_inverse, const double *gcp_blocks,
+                      const hsba_lm_options *lm, double *x_out, hsba_lm_stats *stats);
 
 /* ---- batched independent problems (local-BA windows) ------------------- */
 /* Many independent problems solved together: one set of launches per LM

@@ -59,6 +59,11 @@ struct OProblem {
   std::vector<double> fix_points, fix_images, fix_cameras;
   std::vector<Constraint> cons;
   int64_t n_point_cons = 0, n_image_cons = 0, n_camera_cons = 0;
+  // Full 3x3 weight blocks of the point constraints ([records][3][3], each
+  // record constraining all three coordinates), as the GCP blocks of
+  // BAGCPConstrainedYCovarianceInverse (BA/ba_gcp_constrained_y_covariance_inverse.hpp:14-23).
+  // Empty: diagonal per-row weights.
+  std::vector<double> pblocks;
   int64_t x_pt = 0, x_img = 0, x_cam = 0, x_size = 0, y_size = 0;
   bool pts_free = true, imgs_free = true, cams_free = true;
 
@@ -491,8 +496,25 @@ int build_normal(OProblem &o, const double *x, const double *y_obs, const double
       for (int a = 0; a < 6; ++a)
         for (int b = 0; b < d; ++b) o.Wic[(6 * i + a) * d + b] += Ji[a] * LJc[b] + Ji[6 + a] * LJc[d + b];
   }
+  // point constraints with 3x3 blocks: N_pp += B, g_p += B r
+  // (BA/ba_gcp_constrained_normal_equation_builder.hpp:59-72)
+  const size_t q_first = o.pblocks.empty() ? 0 : (size_t)o.n_point_cons;
+  for (size_t q0 = 0; q0 < q_first; q0 += 3) {
+    const double *B = &o.pblocks[3 * q0];
+    const int64_t p = o.cons[q0].entity;
+    double r[3];
+    for (int b = 0; b < 3; ++b) r[b] = x[o.cons[q0 + b].x_index] - y_obs[2 * o.N + q0 + b];
+    for (int a = 0; a < 3; ++a) {
+      double s = 0.0;
+      for (int b = 0; b < 3; ++b) {
+        o.V[9 * p + 3 * a + b] += B[3 * a + b];
+        s += B[3 * a + b] * r[b];
+      }
+      o.g[o.x_pt + 3 * p + a] += s;
+    }
+  }
   // constraint rows: weights on the diagonal, w * r in the gradient (:309-349, :418-459)
-  for (size_t q = 0; q < o.cons.size(); ++q) {
+  for (size_t q = q_first; q < o.cons.size(); ++q) {
     const Constraint &c = o.cons[q];
     double w = cons_w(cw, q);
     double r = x[c.x_index] - y_obs[2 * o.N + q];
@@ -516,7 +538,15 @@ double mahalanobis(const OProblem &o, const double *x, const double *y_obs, cons
     F += r0 * (key_w(kb, k, 0, 0) * r0 + key_w(kb, k, 0, 1) * r1) +
          r1 * (key_w(kb, k, 1, 0) * r0 + key_w(kb, k, 1, 1) * r1);
   }
-  for (size_t q = 0; q < o.cons.size(); ++q) {
+  // 3x3 point blocks: r^T B r (BA/ba_gcp_constrained_mahalanobis_distance_calculator.hpp:50-58)
+  const size_t q_first = o.pblocks.empty() ? 0 : (size_t)o.n_point_cons;
+  for (size_t q0 = 0; q0 < q_first; q0 += 3) {
+    const double *B = &o.pblocks[3 * q0];
+    double r[3];
+    for (int b = 0; b < 3; ++b) r[b] = x[o.cons[q0 + b].x_index] - y_obs[2 * o.N + q0 + b];
+    for (int a = 0; a < 3; ++a) F += r[a] * (B[3 * a] * r[0] + B[3 * a + 1] * r[1] + B[3 * a + 2] * r[2]);
+  }
+  for (size_t q = q_first; q < o.cons.size(); ++q) {
     double r = x[o.cons[q].x_index] - y_obs[2 * o.N + q];
     F += r * cons_w(cw, q) * r;
   }
@@ -911,6 +941,18 @@ int oracle_sizes(void *h, int64_t *x_size, int64_t *y_size, int64_t *d, int64_t 
   return 0;
 }
 
+// Full 3x3 weight blocks for the point constraints ([records][3][3]); every
+// point constraint must constrain all three coordinates.  NULL clears them.
+int oracle_set_point_blocks(void *h, const double *blocks) {
+  OProblem &o = *static_cast<OProblem *>(h);
+  if (!blocks) { o.pblocks.clear(); return 0; }
+  if (o.n_point_cons % 3) return HSBA_ERR_INVALID;
+  for (int64_t q = 0; q < o.n_point_cons; ++q)
+    if (o.cons[q].param != q % 3) return HSBA_ERR_INVALID;
+  o.pblocks.assign(blocks, blocks + 3 * o.n_point_cons);
+  return 0;
+}
+
 int oracle_evaluate(void *h, const double *x, double *y) {
   evaluate(*static_cast<OProblem *>(h), x, y);
   return 0;
@@ -1060,7 +1102,21 @@ int oracle_dense_normal(void *h, const double *x, const double *y_obs, const dou
   for (int64_t k = 0; k < o.N; ++k)
     for (int a = 0; a < 2; ++a)
       Wr[2 * k + a] = key_w(key_blocks, k, a, 0) * r[2 * k] + key_w(key_blocks, k, a, 1) * r[2 * k + 1];
-  for (size_t q = 0; q < o.cons.size(); ++q) Wr[2 * o.N + q] = cons_w(cons_weights, q) * r[2 * o.N + q];
+  // the inverse covariance of the constraint rows: diagonal weights, or 3x3
+  // blocks over each point constraint's three rows
+  auto cons_weight = [&](size_t q, size_t j) -> double {
+    if (!o.pblocks.empty() && q < (size_t)o.n_point_cons) {
+      const size_t q0 = q - q % 3;
+      if (j < q0 || j >= q0 + 3) return 0.0;
+      return o.pblocks[3 * q0 + 3 * (q - q0) + (j - q0)];
+    }
+    return j == q ? cons_w(cons_weights, q) : 0.0;
+  };
+  for (size_t q = 0; q < o.cons.size(); ++q) {
+    double s = 0.0;
+    for (size_t j = 0; j < o.cons.size(); ++j) s += cons_weight(q, j) * r[2 * o.N + j];
+    Wr[2 * o.N + q] = s;
+  }
   if (J) std::copy(Jd.begin(), Jd.end(), J);
   if (g)
     for (int64_t c = 0; c < nx; ++c) {
@@ -1077,8 +1133,11 @@ int oracle_dense_normal(void *h, const double *x, const double *y_obs, const dou
           WJ[(2 * k + a) * nx + c] = key_w(key_blocks, k, a, 0) * Jd[(2 * k) * nx + c] +
                                      key_w(key_blocks, k, a, 1) * Jd[(2 * k + 1) * nx + c];
     for (size_t q = 0; q < o.cons.size(); ++q)
-      for (int64_t c = 0; c < nx; ++c)
-        WJ[(2 * o.N + q) * nx + c] = cons_w(cons_weights, q) * Jd[(2 * o.N + q) * nx + c];
+      for (int64_t c = 0; c < nx; ++c) {
+        double s = 0.0;
+        for (size_t j = 0; j < o.cons.size(); ++j) s += cons_weight(q, j) * Jd[(2 * o.N + j) * nx + c];
+        WJ[(2 * o.N + q) * nx + c] = s;
+      }
     for (int64_t a = 0; a < nx; ++a)
       for (int64_t b = 0; b < nx; ++b) {
         double s = 0.0;

@@ -84,3 +84,68 @@ def naive_from_scene(scene) -> NaiveScene:
 
     return NaiveScene(C, P, scene.feature_maps.astype(np.int32), conv(scene.x0), conv(scene.x_true), y,
                       name=scene.name + "-naive")
+
+
+# ---------------------------------------------------------------- GCP family
+@dataclass
+class GCPScene:
+    """ba_gcp_constrained problem: a NaiveScene whose last num_gcps points are
+    ground control points with observed coordinates."""
+    naive: NaiveScene
+    num_gcps: int
+    gcp_blocks: np.ndarray    # [G, 3, 3] symmetric positive definite inverse covariances
+    y_obs: np.ndarray         # [2F + 3G]: features, then GCP coordinates
+
+
+def gcp_evaluate(x: np.ndarray, scene: GCPScene) -> np.ndarray:
+    """BAGCPConstrainedVectorFunction::operator()
+    (ba_gcp_constrained_vector_function.hpp:73-113): the naive features, then
+    the x entries of the last G points in GCP order."""
+    n = scene.naive
+    y = naive_evaluate(x, n.num_cameras, n.num_points, n.feature_maps)
+    G = scene.num_gcps
+    gx = x[6 * n.num_cameras + 3 * (n.num_points - G):6 * n.num_cameras + 3 * n.num_points]
+    return np.concatenate([y, gx])
+
+
+def gcp_mahalanobis(x: np.ndarray, scene: GCPScene, feature_cov=None) -> float:
+    """BAGCPConstrainedMahalanobisDistanceCalculator
+    (ba_gcp_constrained_mahalanobis_distance_calculator.hpp:33-62):
+    sum r_f^T W_f r_f over features plus sum r_g^T B_g r_g over GCPs."""
+    r = gcp_evaluate(x, scene) - scene.y_obs
+    F = scene.naive.feature_maps.shape[0]
+    rf = r[:2 * F].reshape(-1, 2)
+    W = np.broadcast_to(np.eye(2), (F, 2, 2)) if feature_cov is None else np.asarray(feature_cov).reshape(F, 2, 2)
+    rg = r[2 * F:].reshape(-1, 3)
+    return float(np.einsum("ki,kij,kj->", rf, W, rf) + np.einsum("gi,gij,gj->", rg, scene.gcp_blocks, rg))
+
+
+def make_gcp_scene(n: NaiveScene, num_gcps: int, seed: int = 7, gcp_noise: float = 0.0) -> GCPScene:
+    """GCPs = the last num_gcps points, observed at their true coordinates
+    plus N(0, gcp_noise^2) noise, each with a random SPD 3x3 weight block."""
+    rng = np.random.default_rng(seed)
+    C, P = n.num_cameras, n.num_points
+    pts = n.x_true[6 * C:6 * C + 3 * P].reshape(P, 3)
+    g = pts[P - num_gcps:] + gcp_noise * rng.standard_normal((num_gcps, 3))
+    A = rng.standard_normal((num_gcps, 3, 3))
+    B = np.einsum("gij,gkj->gik", A, A) + 3.0 * np.eye(3)
+    B = 0.5 * (B + np.transpose(B, (0, 2, 1)))
+    return GCPScene(n, num_gcps, np.ascontiguousarray(B), np.concatenate([n.y_obs, g.reshape(-1)]))
+
+
+def gcp_shared_scene(gs: GCPScene):
+    """The camera_shared form the oracle runs: one image per camera, identity
+    intrinsics, and the GCPs as point constraints on all three coordinates
+    with full 3x3 weight blocks."""
+    from hs_sfm_amd import synthetic as syn
+    n = gs.naive
+    C, P, G = n.num_cameras, n.num_points, gs.num_gcps
+    t = syn.Scene(num_points=P, num_images=C, num_cameras=C, feature_maps=n.feature_maps.astype(np.int32),
+                  image_camera_map=np.arange(C, dtype=np.int32), comp_mask=0, free_mask=0,
+                  intrinsic_defaults=None, fix_mask=0, name=n.name + "-gcp-shared")
+    t.point_constraints = np.array([[P - G + i, 7] for i in range(G)], np.int64).reshape(-1, 2)
+    t.point_constraint_blocks = gs.gcp_blocks.copy()
+    t.x0 = naive_to_shared(n.x0, C, P)
+    t.x_true = naive_to_shared(n.x_true, C, P)
+    t.y_obs = gs.y_obs.copy()
+    return t

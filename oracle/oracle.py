@@ -57,6 +57,7 @@ def lib():
         L.oracle_lm.argtypes = [P, _dp, _dp, _dp, _dp, C.POINTER(hsba_lm_options), C.c_int, _dp,
                                 C.POINTER(hsba_lm_stats), C.POINTER(LMTrace)]
         L.oracle_dense_normal.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_set_point_blocks.argtypes = [P, _dp]
         _lib = L
     return _lib
 
@@ -74,6 +75,10 @@ class Oracle:
         xs, ys, dd, nr = (C.c_int64() for _ in range(4))
         lib().oracle_sizes(self.h, C.byref(xs), C.byref(ys), C.byref(dd), C.byref(nr))
         self.x_size, self.y_size, self.cam_d, self.n_reduced = xs.value, ys.value, dd.value, nr.value
+        if self.d.pb is not None:
+            rc = lib().oracle_set_point_blocks(self.h, dptr(self.d.pb))
+            if rc:
+                raise ValueError(f"oracle_set_point_blocks failed: {rc}")
 
     def __del__(self):
         if getattr(self, "h", None):
