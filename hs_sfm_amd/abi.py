@@ -26,6 +26,7 @@ ERRORS = {
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 SOLVER_AUTO, SOLVER_DENSE, SOLVER_PCG = 0, 1, 2
+LOSS_TRIVIAL, LOSS_HUBER = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -57,7 +58,8 @@ class hsba_ycov_inverse(C.Structure):
 
 class hsba_options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("solver", C.c_int32), ("pcg_rtol", C.c_double),
-                ("pcg_max_iterations", C.c_int32), ("point_group_lanes", C.c_int32)]
+                ("pcg_max_iterations", C.c_int32), ("point_group_lanes", C.c_int32),
+                ("loss", C.c_int32), ("reserved_", C.c_int32), ("loss_scale", C.c_double)]
 
 
 class hsba_lm_options(C.Structure):
@@ -88,7 +90,8 @@ class hsba_stage_times(C.Structure):
 class hsba_server_options(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active_jobs", C.c_int32),
                 ("precision", C.c_int32), ("lm", hsba_lm_options), ("streams", C.c_int32),
-                ("max_batch_jobs", C.c_int32)]
+                ("max_batch_jobs", C.c_int32), ("loss", C.c_int32), ("reserved2_", C.c_int32),
+                ("loss_scale", C.c_double)]
 
 
 class hsba_naive_desc(C.Structure):

@@ -35,6 +35,7 @@ hsba_options defaults(const hsba_options *o) {
     if (r.pcg_rtol <= 0) r.pcg_rtol = 1e-6;
     if (r.pcg_max_iterations <= 0) r.pcg_max_iterations = 500;
   }
+  if (!(r.loss_scale > 0.0)) r.loss_scale = 1.0;
   return r;
 }
 
@@ -62,6 +63,7 @@ int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_opti
                   DeviceArena *arena, std::unique_ptr<EngineBase> &out) {
   const hsba_options o = defaults(opt);
   if (o.precision != HSBA_PRECISION_FP64 && o.precision != HSBA_PRECISION_MIXED) return HSBA_ERR_INVALID;
+  if (o.loss != HSBA_LOSS_TRIVIAL && o.loss != HSBA_LOSS_HUBER) return HSBA_ERR_INVALID;
   std::unique_ptr<EngineBase> eng(hsba::make_engine(o.precision));
   std::string why;
   bool pcg = o.solver == HSBA_SOLVER_PCG;
@@ -84,6 +86,8 @@ int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_opti
   eng->precision = o.precision;
   eng->pcg_rtol = o.pcg_rtol;
   eng->pcg_max_iterations = o.pcg_max_iterations;
+  eng->loss = o.loss;
+  eng->loss_scale = o.loss_scale;
   // a point group of 8 lanes covers the 2..40-view tracks of the benchmark scenes
   eng->glog2 = 3;
   if (o.point_group_lanes > 0) {

@@ -71,13 +71,28 @@ __host__ __device__ constexpr int z_stride(int e, int tbytes) {
   return (3 * e * tbytes + 127) / 128 * (128 / tbytes);
 }
 
+// Huber loss on a key's squared whitened residual s (scale a):
+// rho(s) = s for s <= a^2, 2 a sqrt(s) - a^2 beyond; rho'(s) = min(1, a / sqrt(s)).
+// rho'' <= 0 everywhere, so the robustified Gauss-Newton model is plain
+// reweighting: residual and Jacobian rows times sqrt(rho'(s)).
+__host__ __device__ inline double robust_cost(int loss, double a, double s) {
+  if (loss == 1 /* HSBA_LOSS_HUBER */ && s > a * a) return 2.0 * a * sqrt(s) - a * a;
+  return s;
+}
+__host__ __device__ inline double robust_row_scale(int loss, double a, double s) {
+  if (loss == 1 /* HSBA_LOSS_HUBER */ && s > a * a) return sqrt(a / sqrt(s));
+  return 1.0;
+}
+
 // All device buffers, passed by value to kernels.
 template <typename T>
 struct Dev {
   int32_t P, I, C, N, nwin, npairs, ncons, nchunks;
   int32_t pts_free, imgs_free, cams_free, has_cov;
-  int32_t pcg_rows, pad_dev_;
+  int32_t pcg_rows;
+  int32_t loss;           // HSBA_LOSS_*: robust loss on each key's squared whitened residual
   int32_t fidx[kFull];    // free intrinsic indices (first D valid)
+  double loss_scale;      // Huber a
   // keys, point-major order
   const int32_t *key_img, *key_pt, *key_orig;
   const double *key_uv;   // [N][2]

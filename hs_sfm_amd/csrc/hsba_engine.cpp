@@ -124,6 +124,8 @@ struct Engine final : EngineBase {
     d.nchunks = (int32_t)L.chunk_kind.size();
     d.pts_free = L.pts_free; d.imgs_free = L.imgs_free; d.cams_free = L.cams_free; d.has_cov = 0;
     d.pcg_rows = L.pcg ? (int32_t)L.rows_total : 0;
+    d.loss = loss;
+    d.loss_scale = loss_scale;
     for (int j = 0; j < kFull; ++j) d.fidx[j] = L.fidx[j];
     int rc = 0;
 #define A(ptr, n) if ((rc = dalloc(ptr, (n), owned, arena))) return rc

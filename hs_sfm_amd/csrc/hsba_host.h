@@ -88,6 +88,8 @@ struct EngineBase {
   int solver = HSBA_SOLVER_DENSE;
   double pcg_rtol = 1e-6;
   int pcg_max_iterations = 500;
+  int loss = HSBA_LOSS_TRIVIAL;
+  double loss_scale = 1.0;
   int glog2 = 3;
   virtual int init(hipStream_t stream) = 0;
   virtual int upload_x(const double *const *xs) = 0;

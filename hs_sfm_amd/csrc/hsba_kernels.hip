@@ -65,6 +65,19 @@ __global__ void __launch_bounds__(256) k_linearize(Dev<T> d, int glog2) {
 #pragma unroll
         for (int m = 0; m < (D > 0 ? D : 1); ++m) whiten2(lt, Jc[0][m], Jc[1][m]);
       }
+      if (d.loss) {
+        const double sw = robust_row_scale(d.loss, d.loss_scale, res[0] * res[0] + res[1] * res[1]);
+        res[0] *= sw; res[1] *= sw;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) Jp[r][m] *= sw;
+#pragma unroll
+          for (int m = 0; m < 6; ++m) Ji[r][m] *= sw;
+#pragma unroll
+          for (int m = 0; m < (D > 0 ? D : 1); ++m) Jc[r][m] *= sw;
+        }
+      }
       v[0] += T(Jp[0][0] * Jp[0][0] + Jp[1][0] * Jp[1][0]);
       v[1] += T(Jp[0][0] * Jp[0][1] + Jp[1][0] * Jp[1][1]);
       v[2] += T(Jp[0][0] * Jp[0][2] + Jp[1][0] * Jp[1][2]);
@@ -136,6 +149,17 @@ __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
       for (int m = 0; m < 6; ++m) whiten2(lt, Ji[0][m], Ji[1][m]);
 #pragma unroll
       for (int m = 0; m < (D > 0 ? D : 1); ++m) whiten2(lt, Jc[0][m], Jc[1][m]);
+    }
+    if (d.loss) {
+      const double sw = robust_row_scale(d.loss, d.loss_scale, res[0] * res[0] + res[1] * res[1]);
+      res[0] *= sw; res[1] *= sw;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) Ji[r][m] *= sw;
+#pragma unroll
+        for (int m = 0; m < (D > 0 ? D : 1); ++m) Jc[r][m] *= sw;
+      }
     }
     double jr[2][E > 0 ? E : 1];
 #pragma unroll
@@ -732,7 +756,7 @@ __global__ void __launch_bounds__(256) k_cost(Dev<T> d, int use_new) {
       double res[2], Jp[2][3], Ji[2][6], Jc[2][1];
       key_linearize<0, false>(X, img, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
       if (d.has_cov) whiten2(d.key_cov + 3 * k, res[0], res[1]);
-      acc += res[0] * res[0] + res[1] * res[1];
+      acc += robust_cost(d.loss, d.loss_scale, res[0] * res[0] + res[1] * res[1]);
     }
   }
   red[threadIdx.x] = acc;

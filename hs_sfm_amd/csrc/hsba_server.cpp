@@ -183,6 +183,8 @@ struct hsba_server {
     std::memset(&o, 0, sizeof(o));
     o.precision = opt.precision;
     o.solver = HSBA_SOLVER_AUTO;
+    o.loss = opt.loss;
+    o.loss_scale = opt.loss_scale;
     int rc = st ? HSBA_OK : HSBA_ERR_DEVICE;
     std::unique_ptr<hsba::EngineBase> eng;
     if (!rc) rc = hsba::create_engine(descs.data(), n, &o, st, &arena, eng);
@@ -234,6 +236,7 @@ int hsba_server_create(const hsba_server_options *options, hsba_server **server)
   if (options) s->opt = *options;
   hsba_server_options &o = s->opt;
   if (o.precision != HSBA_PRECISION_FP64 && o.precision != HSBA_PRECISION_MIXED) return HSBA_ERR_INVALID;
+  if (o.loss != HSBA_LOSS_TRIVIAL && o.loss != HSBA_LOSS_HUBER) return HSBA_ERR_INVALID;
   if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = int64_t(16) << 30;
   if (o.max_active_jobs <= 0) o.max_active_jobs = 4096;
   if (o.streams <= 0) o.streams = 4;

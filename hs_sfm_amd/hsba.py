@@ -28,22 +28,23 @@ _dp = C.POINTER(C.c_double)
 
 
 def _opts(precision=abi.PRECISION_MIXED, solver=abi.SOLVER_AUTO, pcg_rtol=1e-6,
-          pcg_max_iterations=500, point_group_lanes=0):
+          pcg_max_iterations=500, point_group_lanes=0, loss=abi.LOSS_TRIVIAL, loss_scale=1.0):
     return abi.hsba_options(precision=precision, solver=solver, pcg_rtol=pcg_rtol,
                             pcg_max_iterations=pcg_max_iterations,
-                            point_group_lanes=point_group_lanes)
+                            point_group_lanes=point_group_lanes, loss=loss, loss_scale=loss_scale)
 
 
 class Problem:
     """Device-resident problem (hsba_problem) built from a Scene."""
 
     def __init__(self, scene: Scene, precision=abi.PRECISION_MIXED, solver=abi.SOLVER_AUTO,
-                 pcg_rtol=1e-6, point_group_lanes=0):
+                 pcg_rtol=1e-6, point_group_lanes=0, loss=abi.LOSS_TRIVIAL, loss_scale=1.0):
         self.lib = abi.load_library()
         self.scene = scene
         self.desc = abi.Desc(scene)
         self.h = C.c_void_p()
-        o = _opts(precision, solver, pcg_rtol, point_group_lanes=point_group_lanes)
+        o = _opts(precision, solver, pcg_rtol, point_group_lanes=point_group_lanes, loss=loss,
+                  loss_scale=loss_scale)
         rc = self.lib.hsba_problem_create(C.byref(self.desc.desc), C.byref(o), C.byref(self.h))
         abi.check(rc, "hsba_problem_create")
         xs, ys, dd, nr = (C.c_int64() for _ in range(4))
@@ -347,10 +348,12 @@ class Server:
     Each job's result is written into the array returned by submit()."""
 
     def __init__(self, precision=abi.PRECISION_MIXED, hbm_budget_bytes=0, max_active_jobs=0,
-                 streams=0, max_batch_jobs=0, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0):
+                 streams=0, max_batch_jobs=0, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0,
+                 loss=abi.LOSS_TRIVIAL, loss_scale=1.0):
         self.lib = abi.load_library()
         o = abi.hsba_server_options(hbm_budget_bytes=hbm_budget_bytes, max_active_jobs=max_active_jobs,
-                                    precision=precision, streams=streams, max_batch_jobs=max_batch_jobs)
+                                    precision=precision, streams=streams, max_batch_jobs=max_batch_jobs,
+                                    loss=loss, loss_scale=loss_scale)
         o.lm = abi.hsba_lm_options(max_iterations=max_iterations, tau=tau, eps1=eps1, eps2=eps2)
         self.h = C.c_void_p()
         abi.check(self.lib.hsba_server_create(C.byref(o), C.byref(self.h)), "hsba_server_create")

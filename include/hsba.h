@@ -129,12 +129,28 @@ typedef struct {
 #define HSBA_SOLVER_DENSE  1        /* dense FP64 Cholesky of the reduced system */
 #define HSBA_SOLVER_PCG    2        /* block-Jacobi PCG on the implicit Schur complement */
 
+/* Loss on each key's squared whitened residual s = r^T Sigma^-1 r.  The
+ * in-house LM path has none (TRIVIAL: rho(s) = s); the production caller
+ * (sfm_pipeline/bundle_adjustment_optimizor.hpp) runs Ceres with
+ * ceres::HuberLoss(1.0) per key (camera_shared_ceres_optimizor.hpp:230-481).
+ * HUBER with scale a: rho(s) = s for s <= a^2, 2 a sqrt(s) - a^2 beyond, so the
+ * cost is sum rho(s) (no 1/2, like the rest of this ABI).  Its second
+ * derivative is never positive, so the robustified Gauss-Newton model
+ * (Triggs et al. 2000, "Bundle adjustment — a modern synthesis", sec. 4.3)
+ * reduces to reweighting: each key's residual and Jacobian rows are scaled by
+ * sqrt(rho'(s)).  Constraint rows are never robustified. */
+#define HSBA_LOSS_TRIVIAL  0
+#define HSBA_LOSS_HUBER    1
+
 typedef struct {
   int32_t precision;                /* HSBA_PRECISION_* */
   int32_t solver;                   /* HSBA_SOLVER_*    */
   double  pcg_rtol;                 /* relative residual tolerance (default 1e-6) */
   int32_t pcg_max_iterations;       /* default 500 */
   int32_t point_group_lanes;        /* lanes per point in point kernels: 0 = tuned */
+  int32_t loss;                     /* HSBA_LOSS_* (0 = trivial)             */
+  int32_t reserved_;
+  double  loss_scale;               /* a of the Huber loss (0 = 1.0)         */
 } hsba_options;
 
 /* Optimizer settings: CameraSharedLevenbergMarquardtOptimizor ctor
@@ -379,6 +395,9 @@ typedef struct {
   hsba_lm_options lm;               /* per-job LM settings                   */
   int32_t streams;                  /* concurrent batches, one HIP stream each (0 = 4) */
   int32_t max_batch_jobs;           /* jobs per batch (0 = 256)              */
+  int32_t loss;                     /* HSBA_LOSS_* for every job             */
+  int32_t reserved2_;
+  double  loss_scale;               /* Huber scale (0 = 1.0)                 */
 } hsba_server_options;
 
 typedef struct {

@@ -64,6 +64,12 @@ struct OProblem {
   // BAGCPConstrainedYCovarianceInverse (BA/ba_gcp_constrained_y_covariance_inverse.hpp:14-23).
   // Empty: diagonal per-row weights.
   std::vector<double> pblocks;
+  // Robust loss on each key's s = r^T Lambda r (0 none, 1 Huber with scale
+  // loss_a): cost rho(s), normal equations reweighted by rho'(s).  The
+  // reference's in-house path has none; its production caller uses
+  // ceres::HuberLoss(1.0) (BA/camera_shared_ceres_optimizor.hpp:253).
+  int loss = 0;
+  double loss_a = 1.0;
   int64_t x_pt = 0, x_img = 0, x_cam = 0, x_size = 0, y_size = 0;
   bool pts_free = true, imgs_free = true, cams_free = true;
 
@@ -429,6 +435,14 @@ void key_jacobian_cs(const OProblem &o, int64_t k, const double *x, double Jp[6]
     for (int q = 0; q < o.d; ++q) cs(cam[o.free_idx[q]], Jc + q, o.d);
 }
 
+// Huber: rho(s) = s (s <= a^2), 2 a sqrt(s) - a^2 (s > a^2); rho'(s) = min(1, a / sqrt(s)).
+double rho(const OProblem &o, double s) {
+  return (o.loss == 1 && s > o.loss_a * o.loss_a) ? 2.0 * o.loss_a * std::sqrt(s) - o.loss_a * o.loss_a : s;
+}
+double rho1(const OProblem &o, double s) {
+  return (o.loss == 1 && s > o.loss_a * o.loss_a) ? o.loss_a / std::sqrt(s) : 1.0;
+}
+
 double key_w(const double *kb, int64_t k, int a, int b) {
   if (!kb) return a == b ? 1.0 : 0.0;
   return kb[4 * k + 2 * a + b];
@@ -451,6 +465,13 @@ int build_normal(OProblem &o, const double *x, const double *y_obs, const double
     project_key(o, k, x, uv);
     double r[2] = {uv[0] - y_obs[2 * k], uv[1] - y_obs[2 * k + 1]};
     double L[2][2] = {{key_w(kb, k, 0, 0), key_w(kb, k, 0, 1)}, {key_w(kb, k, 1, 0), key_w(kb, k, 1, 1)}};
+    if (o.loss) {
+      // iteratively reweighted: Lambda -> rho'(s) Lambda
+      const double s = r[0] * (L[0][0] * r[0] + L[0][1] * r[1]) + r[1] * (L[1][0] * r[0] + L[1][1] * r[1]);
+      const double w = rho1(o, s);
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) L[a][b] *= w;
+    }
     double Lr[2] = {L[0][0] * r[0] + L[0][1] * r[1], L[1][0] * r[0] + L[1][1] * r[1]};
     // L * Ji, L * Jc
     double LJi[12], LJp[6];
@@ -535,8 +556,8 @@ double mahalanobis(const OProblem &o, const double *x, const double *y_obs, cons
     double uv[2];
     project_key(o, k, x, uv);
     double r0 = uv[0] - y_obs[2 * k], r1 = uv[1] - y_obs[2 * k + 1];
-    F += r0 * (key_w(kb, k, 0, 0) * r0 + key_w(kb, k, 0, 1) * r1) +
-         r1 * (key_w(kb, k, 1, 0) * r0 + key_w(kb, k, 1, 1) * r1);
+    F += rho(o, r0 * (key_w(kb, k, 0, 0) * r0 + key_w(kb, k, 0, 1) * r1) +
+                r1 * (key_w(kb, k, 1, 0) * r0 + key_w(kb, k, 1, 1) * r1));
   }
   // 3x3 point blocks: r^T B r (BA/ba_gcp_constrained_mahalanobis_distance_calculator.hpp:50-58)
   const size_t q_first = o.pblocks.empty() ? 0 : (size_t)o.n_point_cons;
@@ -953,6 +974,15 @@ int oracle_set_point_blocks(void *h, const double *blocks) {
   return 0;
 }
 
+// Robust loss on the key residuals: 0 none, 1 Huber with scale a.
+int oracle_set_loss(void *h, int loss, double a) {
+  OProblem &o = *static_cast<OProblem *>(h);
+  if (loss != 0 && loss != 1) return HSBA_ERR_INVALID;
+  o.loss = loss;
+  o.loss_a = a > 0.0 ? a : 1.0;
+  return 0;
+}
+
 int oracle_evaluate(void *h, const double *x, double *y) {
   evaluate(*static_cast<OProblem *>(h), x, y);
   return 0;
@@ -1099,9 +1129,17 @@ int oracle_dense_normal(void *h, const double *x, const double *y_obs, const dou
   for (size_t q = 0; q < o.cons.size(); ++q) Jd[(2 * o.N + q) * nx + o.cons[q].x_index] = 1.0;
   evaluate(o, x, y.data());
   for (int64_t j = 0; j < ny; ++j) r[j] = y[j] - y_obs[j];
+  // per-key robust weight rho'(s), s = r^T Lambda r (1 without a loss)
+  std::vector<double> kw(o.N, 1.0);
+  for (int64_t k = 0; k < o.N; ++k) {
+    double s = 0.0;
+    for (int a = 0; a < 2; ++a)
+      for (int b = 0; b < 2; ++b) s += r[2 * k + a] * key_w(key_blocks, k, a, b) * r[2 * k + b];
+    kw[k] = rho1(o, s);
+  }
   for (int64_t k = 0; k < o.N; ++k)
     for (int a = 0; a < 2; ++a)
-      Wr[2 * k + a] = key_w(key_blocks, k, a, 0) * r[2 * k] + key_w(key_blocks, k, a, 1) * r[2 * k + 1];
+      Wr[2 * k + a] = kw[k] * (key_w(key_blocks, k, a, 0) * r[2 * k] + key_w(key_blocks, k, a, 1) * r[2 * k + 1]);
   // the inverse covariance of the constraint rows: diagonal weights, or 3x3
   // blocks over each point constraint's three rows
   auto cons_weight = [&](size_t q, size_t j) -> double {
@@ -1130,8 +1168,8 @@ int oracle_dense_normal(void *h, const double *x, const double *y_obs, const dou
     for (int64_t k = 0; k < o.N; ++k)
       for (int a = 0; a < 2; ++a)
         for (int64_t c = 0; c < nx; ++c)
-          WJ[(2 * k + a) * nx + c] = key_w(key_blocks, k, a, 0) * Jd[(2 * k) * nx + c] +
-                                     key_w(key_blocks, k, a, 1) * Jd[(2 * k + 1) * nx + c];
+          WJ[(2 * k + a) * nx + c] = kw[k] * (key_w(key_blocks, k, a, 0) * Jd[(2 * k) * nx + c] +
+                                              key_w(key_blocks, k, a, 1) * Jd[(2 * k + 1) * nx + c]);
     for (size_t q = 0; q < o.cons.size(); ++q)
       for (int64_t c = 0; c < nx; ++c) {
         double s = 0.0;

@@ -58,6 +58,7 @@ def lib():
                                 C.POINTER(hsba_lm_stats), C.POINTER(LMTrace)]
         L.oracle_dense_normal.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_set_point_blocks.argtypes = [P, _dp]
+        L.oracle_set_loss.argtypes = [P, C.c_int, C.c_double]
         _lib = L
     return _lib
 
@@ -65,7 +66,8 @@ def lib():
 class Oracle:
     """FP64 CPU restatement of the camera_shared LM path for one scene."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, loss=0, loss_scale=1.0):
+        """loss: 0 none, 1 Huber with scale loss_scale on each key's r^T W r."""
         self.scene = scene
         self.d = Desc(scene)
         rc = C.c_int(0)
@@ -79,6 +81,10 @@ class Oracle:
             rc = lib().oracle_set_point_blocks(self.h, dptr(self.d.pb))
             if rc:
                 raise ValueError(f"oracle_set_point_blocks failed: {rc}")
+        if loss:
+            rc = lib().oracle_set_loss(self.h, int(loss), float(loss_scale))
+            if rc:
+                raise ValueError(f"oracle_set_loss failed: {rc}")
 
     def __del__(self):
         if getattr(self, "h", None):
