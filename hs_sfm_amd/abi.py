@@ -104,6 +104,18 @@ class hsba_gcp_desc(C.Structure):
                 ("feature_maps", C.POINTER(C.c_int32)), ("num_gcps", C.c_int64)]
 
 
+class hsba_sfm_files(C.Structure):
+    _fields_ = [("tracks", C.c_char_p), ("keysets", C.POINTER(C.c_char_p)), ("num_keysets", C.c_int64),
+                ("intrinsics", C.c_char_p), ("extrinsics", C.c_char_p), ("points", C.c_char_p),
+                ("image_intrinsic_map", C.c_char_p), ("image_extrinsic_map", C.c_char_p),
+                ("track_point_map", C.c_char_p)]
+
+
+class hsba_scene_view(C.Structure):
+    _fields_ = [("desc", hsba_problem_desc), ("x0", _dp), ("y_obs", _dp), ("x_size", C.c_int64),
+                ("y_size", C.c_int64)]
+
+
 class hsba_job_result(C.Structure):
     _fields_ = [("job_id", C.c_int64), ("status", C.c_int32), ("reserved_", C.c_int32),
                 ("submit_time_s", C.c_double), ("admit_time_s", C.c_double),
@@ -235,6 +247,12 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.hsba_gcp_mahalanobis.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
     lib.hsba_gcp_lm_solve.argtypes = [P, _dp, _dp, _dp, _dp, C.POINTER(hsba_lm_options), _dp,
                                       C.POINTER(hsba_lm_stats)]
+    lib.hsba_scene_load_sfm.argtypes = [C.POINTER(hsba_sfm_files), C.c_uint32, C.c_uint32, C.POINTER(P)]
+    lib.hsba_scene_load_bal.argtypes = [C.c_char_p, C.POINTER(P)]
+    lib.hsba_scene_get.argtypes = [P, C.POINTER(hsba_scene_view)]
+    lib.hsba_scene_store_sfm.argtypes = [P, _dp, C.c_char_p, C.c_char_p, C.c_char_p]
+    lib.hsba_scene_store_bal.argtypes = [P, _dp, C.c_char_p]
+    lib.hsba_scene_destroy.argtypes = [P]
     _LIB = lib
     return lib
 
@@ -251,7 +269,8 @@ EXPORTED_SYMBOLS = [
     "hsba_naive_evaluate", "hsba_naive_jacobian", "hsba_naive_build_normal", "hsba_naive_solve",
     "hsba_naive_lm_solve", "hsba_gcp_problem_create", "hsba_gcp_problem_destroy", "hsba_gcp_evaluate",
     "hsba_gcp_jacobian", "hsba_gcp_build_normal", "hsba_gcp_solve", "hsba_gcp_mahalanobis",
-    "hsba_gcp_lm_solve",
+    "hsba_gcp_lm_solve", "hsba_scene_load_sfm", "hsba_scene_load_bal", "hsba_scene_get",
+    "hsba_scene_store_sfm", "hsba_scene_store_bal", "hsba_scene_destroy",
 ]
 
 

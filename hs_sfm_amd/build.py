@@ -24,6 +24,7 @@ SOURCES = [
     "hsba_engine.cpp",
     "hsba_server.cpp",
     "hsba_naive.cpp",
+    "hsba_io.cpp",
     "hsba_api.cpp",
 ]
 ARCH = os.environ.get("HSBA_OFFLOAD_ARCH", "gfx950")

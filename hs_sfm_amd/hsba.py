@@ -340,6 +340,70 @@ class GCPProblem:
         return rc, xo, st
 
 
+class SceneFile:
+    """A problem read from files through the C ABI (hsba_scene_*): the hs_sfm
+    text formats (sfm_file_io/*_loader.hpp, assembled as
+    BundleAdjustmentOptimizor does) or a BAL file.  `scene` is a Scene ready
+    for Problem / Batch / Server; store_*() write an optimised x back."""
+
+    def __init__(self, handle, lib):
+        self.lib, self.h = lib, handle
+        v = abi.hsba_scene_view()
+        abi.check(lib.hsba_scene_get(self.h, C.byref(v)), "hsba_scene_get")
+        d = v.desc
+        N, I, Cc = d.num_keys, d.num_images, d.num_cameras
+        fm = np.ctypeslib.as_array(d.feature_maps, (max(N, 1), 2))[:N].copy() if N else np.zeros((0, 2), np.int32)
+        icm = np.ctypeslib.as_array(d.image_camera_map, (max(I, 1),))[:I].copy()
+        intr = np.ctypeslib.as_array(d.intrinsic_defaults, (max(Cc, 1), 10))[:Cc].copy()
+        self.scene = Scene(num_points=int(d.num_points), num_images=int(I), num_cameras=int(Cc),
+                           feature_maps=fm.astype(np.int32), image_camera_map=icm.astype(np.int32),
+                           comp_mask=int(d.intrinsic_computations_mask), free_mask=int(d.intrinsic_free_mask),
+                           intrinsic_defaults=intr, fix_mask=0, name="file")
+        self.scene.x0 = np.ctypeslib.as_array(v.x0, (max(v.x_size, 1),))[:v.x_size].copy()
+        self.scene.y_obs = np.ctypeslib.as_array(v.y_obs, (max(v.y_size, 1),))[:v.y_size].copy()
+
+    @classmethod
+    def load_sfm(cls, tracks, keysets, intrinsics, extrinsics, points, image_intrinsic_map,
+                 image_extrinsic_map, track_point_map, comp_mask=7, free_mask=0):
+        lib = abi.load_library()
+        enc = [None if k is None else str(k).encode() for k in keysets]
+        ks = (C.c_char_p * max(1, len(enc)))(*enc)
+        f = abi.hsba_sfm_files(str(tracks).encode(), ks, len(enc), str(intrinsics).encode(),
+                               str(extrinsics).encode(), str(points).encode(),
+                               str(image_intrinsic_map).encode(), str(image_extrinsic_map).encode(),
+                               str(track_point_map).encode())
+        h = C.c_void_p()
+        abi.check(lib.hsba_scene_load_sfm(C.byref(f), comp_mask, free_mask, C.byref(h)), "hsba_scene_load_sfm")
+        return cls(h, lib)
+
+    @classmethod
+    def load_bal(cls, path):
+        lib = abi.load_library()
+        h = C.c_void_p()
+        abi.check(lib.hsba_scene_load_bal(str(path).encode(), C.byref(h)), "hsba_scene_load_bal")
+        return cls(h, lib)
+
+    def store_sfm(self, x, intrinsics=None, extrinsics=None, points=None):
+        enc = [None if p is None else str(p).encode() for p in (intrinsics, extrinsics, points)]
+        abi.check(self.lib.hsba_scene_store_sfm(self.h, abi.dptr(np.ascontiguousarray(x, np.float64)), *enc),
+                  "hsba_scene_store_sfm")
+
+    def store_bal(self, x, path):
+        abi.check(self.lib.hsba_scene_store_bal(self.h, abi.dptr(np.ascontiguousarray(x, np.float64)),
+                                                str(path).encode()), "hsba_scene_store_bal")
+
+    def close(self):
+        if self.h:
+            self.lib.hsba_scene_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """Serving loop over the C ABI (continuous batching of independent jobs).
 

@@ -368,6 +368,61 @@ int hsba_gcp_lm_solve(hsba_gcp_problem *problem, const double *x0, const double 
                       const double *feature_cov_inverse, const double *gcp_blocks,
                       const hsba_lm_options *lm, double *x_out, hsba_lm_stats *stats);
 
+/* ---- problem I/O --------------------------------------------------------- */
+/* A host-side problem read from files: the description, x0 and observations,
+ * ready for hsba_problem_create / hsba_lm_solve, plus what is needed to write
+ * an optimised x back.  Host memory only; no device call. */
+typedef struct hsba_scene hsba_scene;
+
+/* The hs_sfm text files (sfm_file_io/ *_loader.hpp), whitespace separated. */
+typedef struct {
+  const char *tracks;               /* tracks_loader.hpp:24-67: n, then per track n_views (image key blunder)* */
+  const char *const *keysets;       /* [num_keysets], keyset_loader.hpp:23-40: one file per image */
+  int64_t num_keysets;
+  const char *intrinsics;           /* intrinsic_params_set_loader.hpp:24-56: f skew px py pr */
+  const char *extrinsics;           /* extrinsic_params_set_loader.hpp:24-47: angle-axis, centre */
+  const char *points;               /* points_loader.hpp:24-41 */
+  const char *image_intrinsic_map;  /* object_index_loader.hpp:21-44: 0 = none, else id + 1 */
+  const char *image_extrinsic_map;
+  const char *track_point_map;
+} hsba_sfm_files;
+
+/* Assembles the camera_shared problem exactly as
+ * BundleAdjustmentOptimizor::operator() does (bundle_adjustment_optimizor.hpp:59-203):
+ * tracks with a point, views on images with an extrinsic and not flagged as
+ * blunders, keys in track order, x = [points | (angle-axis, -R C) | intrinsics
+ * [k1 k2 k3 d1 d2 f skew px py pr] restricted to the free mask], y = the keys.
+ * intrinsic_free_mask as in hsba_problem_desc (0 = every parameter of the
+ * enabled groups). */
+int hsba_scene_load_sfm(const hsba_sfm_files *files, uint32_t intrinsic_computations_mask,
+                        uint32_t intrinsic_free_mask, hsba_scene **scene);
+/* "Bundle Adjustment in the Large" text format (Agarwal et al. 2010): one
+ * image and one camera group per BAL camera [angle-axis, t, f, k1, k2],
+ * projection -P / P_z, observations centred on the principal point.  Mapped
+ * onto camera_shared by the camera-frame flip diag(1, -1, -1) (so that
+ * P_z > 0) and a negated v observation; free intrinsics {f, k1, k2}. */
+int hsba_scene_load_bal(const char *path, hsba_scene **scene);
+
+typedef struct {
+  hsba_problem_desc desc;           /* pointers stay valid while the scene lives */
+  const double *x0;
+  const double *y_obs;
+  int64_t x_size;
+  int64_t y_size;
+} hsba_scene_view;
+int hsba_scene_get(const hsba_scene *scene, hsba_scene_view *view);
+/* Writes x back into the loaded entities and saves them in the formats of
+ * points_saver.hpp, extrinsic_params_set_saver.hpp and
+ * intrinsic_params_set_saver.hpp (all points / images / cameras of the input
+ * files; entities outside the bundle keep their input values).  Values are
+ * written with 17 significant digits (the reference streams with the default
+ * 6).  Any path may be NULL to skip that file. */
+int hsba_scene_store_sfm(hsba_scene *scene, const double *x, const char *intrinsics,
+                         const char *extrinsics, const char *points);
+/* Writes the BAL file of a scene loaded with hsba_scene_load_bal at x. */
+int hsba_scene_store_bal(hsba_scene *scene, const double *x, const char *path);
+int hsba_scene_destroy(hsba_scene *scene);
+
 /* ---- batched independent problems (local-BA windows) ------------------- */
 /* Many independent problems solved together: one set of launches per LM
  * iteration covers every window.  No reference counterpart (the reference

@@ -1,0 +1,212 @@
+"""Problem I/O (hsba_scene_*): host-only calls into the library.
+
+hs_sfm text formats: a synthetic world with images lacking an extrinsic,
+blunder views, tracks without a point and unused keys is written in the
+formats of sfm_file_io/*_saver.hpp; the loaded problem must equal the
+restated assembly of BundleAdjustmentOptimizor (oracle/sfm_io.py), and —
+independently of that restatement — the FP64 oracle's projections at the
+loaded x0 must reproduce the loaded keys exactly (the keys were made by
+projecting the true points, so any index mix-up breaks it).  Writing x back
+and reading the files again returns x.
+
+BAL: observations made with the BAL camera model in numpy; the loaded
+camera_shared problem's projections must equal them, and storing / reloading
+returns the same cameras."""
+import os
+
+import numpy as np
+import pytest
+
+from hs_sfm_amd import synthetic as syn
+from hs_sfm_amd.hsba import SceneFile
+from oracle import sfm_io
+from oracle.oracle import Oracle
+
+
+def project(X, r, c, q):
+    """camera_shared projection with no distortion: R(r)(X - c), f skew px py pr."""
+    P = (X - c) @ sfm_io.rodrigues(r).T
+    n = P[:, :2] / P[:, 2:3]
+    f, skew, px, py, pr = q
+    return np.stack([f * n[:, 0] + skew * n[:, 1] + px, f * pr * n[:, 1] + py], 1)
+
+
+def world(seed=3):
+    rng = np.random.default_rng(seed)
+    I, Pf = 7, 40
+    img_extr = [0, 1, 2, 3, 4, -1, 5]            # image 5 has no extrinsic
+    img_intr = [i % 2 for i in range(I)]
+    intr = np.array([[900.0, 0.3, 400.0, 300.0, 1.01], [1100.0, -0.2, 410.0, 290.0, 0.99]])
+    ang = 2 * np.pi * np.arange(I) / I
+    centres = np.stack([9 * np.cos(ang), 9 * np.sin(ang), 1.5 + 0.2 * np.sin(ang)], 1)
+    rot, _ = syn.look_at(centres, np.zeros(3))
+    pts = rng.uniform(-1.5, 1.5, (Pf, 3))
+    keys = [list(map(tuple, rng.uniform(0, 800, (3, 2)))) for _ in range(I)]  # unused keys first
+    tracks, track_pt = [], []
+    order = rng.permutation(Pf)
+    for p in order[:34]:                          # six points in no track
+        views = []
+        for i in rng.choice(I, int(rng.integers(2, 6)), replace=False):
+            uv = project(pts[p:p + 1], rot[i], centres[i], intr[img_intr[i]])[0]
+            bl = int(rng.random() < 0.15)
+            keys[i].append((1e4, -1e4) if bl else tuple(uv))  # blunder keys are garbage
+            views.append((int(i), len(keys[i]) - 1, bl))
+        tracks.append(views)
+        track_pt.append(int(p))
+    tracks.append([(5, 0, 0), (1, 0, 1)])         # only a no-extrinsic view and a blunder: dropped
+    track_pt.append(int(order[35]))
+    tracks.append([(0, 1, 0), (2, 1, 0)])         # no point: dropped
+    track_pt.append(-1)
+    extr = np.array([np.concatenate([rot[i], centres[i]]) for i in range(I) if img_extr[i] >= 0])
+    return dict(tracks=tracks, keys=keys, intr=intr, extr=extr, pts=pts, img_intr=img_intr,
+                img_extr=img_extr, track_pt=track_pt)
+
+
+def write_world(w, d):
+    paths = dict(tracks=os.path.join(d, "tracks.txt"), intrinsics=os.path.join(d, "intrinsics.txt"),
+                 extrinsics=os.path.join(d, "extrinsics.txt"), points=os.path.join(d, "points.txt"),
+                 image_intrinsic_map=os.path.join(d, "image_intrinsic.txt"),
+                 image_extrinsic_map=os.path.join(d, "image_extrinsic.txt"),
+                 track_point_map=os.path.join(d, "track_point.txt"))
+    sfm_io.write_tracks(paths["tracks"], w["tracks"])
+    sfm_io.write_rows(paths["intrinsics"], w["intr"], one_per_line=True)
+    sfm_io.write_rows(paths["extrinsics"], w["extr"])
+    sfm_io.write_rows(paths["points"], w["pts"])
+    sfm_io.write_index_map(paths["image_intrinsic_map"], w["img_intr"])
+    sfm_io.write_index_map(paths["image_extrinsic_map"], w["img_extr"])
+    sfm_io.write_index_map(paths["track_point_map"], w["track_pt"])
+    keysets = []
+    for i, k in enumerate(w["keys"]):
+        keysets.append(os.path.join(d, f"keys_{i}.txt"))
+        sfm_io.write_keyset(keysets[-1], k)
+    return paths, keysets
+
+
+def test_sfm_load_matches_assembly_and_projects(tmp_path):
+    w = world()
+    paths, keysets = write_world(w, str(tmp_path))
+    sf = SceneFile.load_sfm(keysets=keysets, **paths)
+    s = sf.scene
+    fm, icm, x, y, bundle = sfm_io.assemble(w["tracks"], [np.asarray(k) for k in w["keys"]], w["intr"],
+                                            w["extr"], w["pts"], w["img_intr"], w["img_extr"], w["track_pt"])
+    assert s.num_points == len(bundle) == 34 and s.num_images == 6 and s.num_cameras == 2
+    assert np.array_equal(s.feature_maps, fm) and np.array_equal(s.image_camera_map, icm)
+    assert np.allclose(s.x0, x, rtol=0, atol=1e-12) and np.array_equal(s.y_obs, y)
+    # independent of the restatement: the loaded problem reproduces the keys
+    uv = Oracle(s).evaluate(s.x0)
+    assert np.abs(uv - s.y_obs).max() < 1e-9
+
+
+def test_sfm_store_round_trip(tmp_path):
+    w = world()
+    paths, keysets = write_world(w, str(tmp_path))
+    sf = SceneFile.load_sfm(keysets=keysets, **paths)
+    s = sf.scene
+    x = s.x0.copy()
+    x[:3 * s.num_points] += 0.01  # move the bundle points
+    out = {k: str(tmp_path / f"out_{k}.txt") for k in ("intrinsics", "extrinsics", "points")}
+    sf.store_sfm(x, **out)
+    pts = sfm_io.read_rows(out["points"], 3)
+    moved = set(sf_bundle_points(w))
+    for p in range(len(w["pts"])):
+        if p in moved:
+            assert np.allclose(pts[p], w["pts"][p] + 0.01, rtol=0, atol=1e-12)
+        else:
+            assert np.array_equal(pts[p], w["pts"][p])  # outside the bundle: untouched
+    assert np.allclose(sfm_io.read_rows(out["extrinsics"], 6), w["extr"], rtol=0, atol=1e-12)
+    assert np.allclose(sfm_io.read_rows(out["intrinsics"], 5), w["intr"], rtol=0, atol=0)
+    paths2 = dict(paths, intrinsics=out["intrinsics"], extrinsics=out["extrinsics"], points=out["points"])
+    s2 = SceneFile.load_sfm(keysets=keysets, **paths2).scene
+    assert np.allclose(s2.x0, x, rtol=0, atol=1e-12)
+
+
+def sf_bundle_points(w):
+    return [p for views, p in zip(w["tracks"], w["track_pt"])
+            if p >= 0 and any(w["img_extr"][im] >= 0 and bl != 1 for im, _, bl in views)]
+
+
+def test_sfm_rejects_bad_files(tmp_path):
+    w = world()
+    paths, keysets = write_world(w, str(tmp_path))
+    with open(paths["tracks"], "w") as f:
+        f.write("1\n2 0 999 0 1 0 0\n")  # key 999 does not exist
+    with pytest.raises(RuntimeError):
+        SceneFile.load_sfm(keysets=keysets, **paths)
+    with pytest.raises(RuntimeError):
+        SceneFile.load_sfm(keysets=keysets, **dict(paths, points=str(tmp_path / "missing.txt")))
+
+
+def bal_world(seed=4):
+    rng = np.random.default_rng(seed)
+    C, P = 5, 60
+    cams = []
+    for c in range(C):
+        ax = rng.normal(size=3)
+        ax /= np.linalg.norm(ax)
+        ang = [0.3, 1.2, np.pi - 1e-9, 2.9, 1e-12][c]  # includes angles at pi and near 0
+        r = ax * ang
+        t = np.array([0.0, 0.0, -10.0])  # the origin sits 10 units down the camera's -z axis
+        cams.append(np.concatenate([r, t, [800 + 40 * c, rng.normal(0, 1e-2), rng.normal(0, 1e-3)]]))
+    cams = np.array(cams)
+    pts = rng.uniform(-1, 1, (P, 3))
+    obs = []
+    for c in range(C):
+        uv = sfm_io.bal_project(cams[c], pts)
+        Pc = pts @ sfm_io.rodrigues(cams[c][:3]).T + cams[c][3:6]
+        for p in range(P):
+            if Pc[p, 2] < 0:  # in front of a BAL camera
+                obs.append((c, p, uv[p, 0], uv[p, 1]))
+    return cams, pts, obs
+
+
+def test_bal_load_projects_like_bal(tmp_path):
+    cams, pts, obs = bal_world()
+    assert len(obs) > 200
+    path = str(tmp_path / "problem.bal")
+    sfm_io.write_bal(path, cams, pts, obs)
+    s = SceneFile.load_bal(path).scene
+    assert s.num_images == s.num_cameras == 5 and s.num_points == 60 and s.num_keys == len(obs)
+    uv = Oracle(s).evaluate(s.x0)
+    ref = np.array([[u, -v] for _, _, u, v in obs]).reshape(-1)  # v flips with the camera frame
+    assert np.abs(uv - s.y_obs).max() < 1e-8 * np.abs(ref).max()
+    assert np.array_equal(s.y_obs, ref)
+
+
+def test_bal_store_round_trip(tmp_path):
+    cams, pts, obs = bal_world()
+    path, out = str(tmp_path / "in.bal"), str(tmp_path / "out.bal")
+    sfm_io.write_bal(path, cams, pts, obs)
+    sf = SceneFile.load_bal(path)
+    sf.store_bal(sf.scene.x0, out)
+    cams2, pts2, obs2 = sfm_io.read_bal(out)
+    assert obs2 == obs and np.array_equal(pts2, pts)
+    for a, b in zip(cams, cams2):
+        assert np.allclose(sfm_io.rodrigues(a[:3]), sfm_io.rodrigues(b[:3]), rtol=0, atol=1e-12)
+        assert np.allclose(a[3:], b[3:], rtol=0, atol=1e-12)
+
+
+@pytest.mark.gpu
+def test_bal_lm_end_to_end(gpu, tmp_path):
+    """A loaded BAL problem (9-parameter cameras) solved on the device, mixed
+    precision, against the FP64 oracle: same reprojection RMSE within 1e-3 px."""
+    from hs_sfm_amd import abi
+    from hs_sfm_amd.hsba import Problem
+    cams, pts, obs = bal_world()
+    rng = np.random.default_rng(9)
+    obs = [(c, p, u + rng.normal(), v + rng.normal()) for c, p, u, v in obs]
+    noisy = cams.copy()
+    noisy[:, :3] += rng.normal(0, 2e-3, (len(cams), 3))
+    noisy[:, 3:6] += rng.normal(0, 2e-2, (len(cams), 3))
+    path = str(tmp_path / "noisy.bal")
+    sfm_io.write_bal(path, noisy, pts + rng.normal(0, 1e-2, pts.shape), obs)
+    sf = SceneFile.load_bal(path)
+    s = sf.scene
+    print("bal lm", s.num_keys, flush=True)
+    rc, x, st = Problem(s, precision=abi.PRECISION_MIXED).lm(max_iterations=20)
+    rc_o, xo, st_o, _ = Oracle(s).lm(max_iterations=20)
+    assert rc == 0 and rc_o == 0
+    assert abs(syn.reprojection_rmse(s, x) - syn.reprojection_rmse(s, xo)) < 1e-3
+    assert syn.reprojection_rmse(s, x) < 1.6  # 1 px noise per coordinate: about sqrt(2) per key
+    sf.store_bal(x, str(tmp_path / "solved.bal"))
+    cams2, _, _ = sfm_io.read_bal(str(tmp_path / "solved.bal"))
+    assert np.isfinite(cams2).all()
