@@ -247,6 +247,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.hsba_gcp_mahalanobis.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
     lib.hsba_gcp_lm_solve.argtypes = [P, _dp, _dp, _dp, _dp, C.POINTER(hsba_lm_options), _dp,
                                       C.POINTER(hsba_lm_stats)]
+    lib.hsba_covariance.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
     lib.hsba_scene_load_sfm.argtypes = [C.POINTER(hsba_sfm_files), C.c_uint32, C.c_uint32, C.POINTER(P)]
     lib.hsba_scene_load_bal.argtypes = [C.c_char_p, C.POINTER(P)]
     lib.hsba_scene_get.argtypes = [P, C.POINTER(hsba_scene_view)]
@@ -260,7 +261,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 EXPORTED_SYMBOLS = [
     "hsba_version", "hsba_device_count", "hsba_problem_create", "hsba_problem_destroy",
     "hsba_problem_sizes", "hsba_evaluate", "hsba_jacobian", "hsba_build_normal",
-    "hsba_solve", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
+    "hsba_solve", "hsba_covariance", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
     "hsba_gradient_dot", "hsba_mahalanobis", "hsba_lm_solve", "hsba_lm_begin",
     "hsba_lm_iterate", "hsba_lm_end", "hsba_batch_create",
     "hsba_batch_destroy", "hsba_batch_lm_solve", "hsba_server_create",

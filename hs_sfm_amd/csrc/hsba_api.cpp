@@ -185,6 +185,23 @@ int hsba_build_normal(hsba_problem *p, const double *x, const double *y_obs, con
   return p->eng->build_normal(point_blocks, image_blocks, camera_blocks, gradient);
 }
 
+int hsba_covariance(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
+                    double *point_blocks, double *image_blocks, double *camera_blocks, double *reduced) {
+  if (!p || !x) return HSBA_ERR_INVALID;
+  int rc = set_x(p, x);
+  if (rc) return rc;
+  // N does not depend on y except through robust-loss weights
+  std::vector<double> zeros;
+  if (!y_obs) {
+    zeros.assign(p->eng->L.hw[0].y_size, 0.0);
+    y_obs = zeros.data();
+  }
+  const double *ys[1] = {y_obs};
+  if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
+  if ((rc = p->eng->build_normal(nullptr, nullptr, nullptr, nullptr))) return rc;
+  return p->eng->covariance(point_blocks, image_blocks, camera_blocks, reduced);
+}
+
 int hsba_solve(hsba_problem *p, double mu, double *delta_x) {
   if (!p || !delta_x) return HSBA_ERR_INVALID;
   return p->eng->solve(mu, delta_x, nullptr, nullptr);

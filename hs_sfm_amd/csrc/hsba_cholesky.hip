@@ -533,6 +533,129 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
   }
 }
 
+// ---------------------------------------------------------------- inverse
+// S^-1 = L^-T L^-1 from the factor, for the posterior covariance.
+// Y = L^-T is upper block-triangular: Y_ii = Linv_i^T and, for j < i,
+//   Y_ji = -(sum_{k=j..i-1} Y_jk L_ik^T) Linv_i^T,
+// one launch per block column i (its workgroups j = 0..i are independent);
+// then S^-1_ab = sum_{i >= max(a, b)} Y_ai Y_bi^T, one workgroup per tile.
+
+// acc += a * b^T (LDS tiles, 256 threads, MFMA result layout of mfma_nt)
+__device__ __forceinline__ void mfma_nt_acc(const double (*a)[LDT], const double (*b)[LDT], dbl4 acc[2][2]) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r0 = 32 * (w >> 1), c0 = 32 * (w & 1);
+  const int li = l & 15, lk = l >> 4;
+#pragma unroll 4
+  for (int k = 0; k < NB; k += 4) {
+    const double a0 = a[r0 + li][k + lk], a1 = a[r0 + 16 + li][k + lk];
+    const double b0 = b[c0 + li][k + lk], b1 = b[c0 + 16 + li][k + lk];
+    acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+    acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+    acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+    acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+  }
+}
+
+// element (r, c) of the 64 x 64 result held by this thread in the mfma_nt layout
+#define HSBA_ACC_FOR(BODY)                                                            \
+  {                                                                                   \
+    const int w_ = threadIdx.x >> 6, l_ = threadIdx.x & 63;                           \
+    const int rb_ = 32 * (w_ >> 1) + (l_ >> 4), cb_ = 32 * (w_ & 1) + (l_ & 15);      \
+    _Pragma("unroll") for (int u = 0; u < 2; ++u)                                     \
+    _Pragma("unroll") for (int v = 0; v < 2; ++v)                                     \
+    _Pragma("unroll") for (int e = 0; e < 4; ++e) {                                   \
+      const int r = rb_ + 16 * u + 4 * e, c = cb_ + 16 * v;                           \
+      BODY;                                                                           \
+    }                                                                                 \
+  }
+
+__global__ void __launch_bounds__(256) k_chol_inv_col(WinInfo wi, const double *Lsrc, const double *Dinv, double *Y,
+                                                      int i) {
+  __shared__ double ta[NB][LDT];
+  __shared__ double tb[NB][LDT];
+  const int j = blockIdx.x;  // 0..i
+  const int64_t ld = wi.ld;
+  const double *L = Lsrc + wi.s_off;
+  double *Yw = Y + wi.s_off;
+  const double *Di = Dinv + wi.dinv_off + (int64_t)i * NB * NB;
+  if (j == i) {
+    load_tile(ta, Di, NB, 0, 0);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      Yw[(int64_t)(i * NB + (e >> 6)) * ld + i * NB + (e & 63)] = ta[e & 63][e >> 6];
+    }
+    return;
+  }
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) acc[u][v] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int k = j; k < i; ++k) {
+    {
+      double v0[16], v1[16];
+      fetch_tile(v0, Yw, ld, j * NB, k * NB);
+      fetch_tile(v1, L, ld, i * NB, k * NB);
+      put_tile(ta, v0);
+      put_tile(tb, v1);
+    }
+    __syncthreads();
+    mfma_nt_acc(ta, tb, acc);  // += Y_jk L_ik^T
+    __syncthreads();
+  }
+  HSBA_ACC_FOR(ta[r][c] = acc[u][v][e]);
+  load_tile(tb, Di, NB, 0, 0);
+  __syncthreads();
+  mfma_nt(ta, tb, acc);  // T Linv_i^T
+  HSBA_ACC_FOR(Yw[(int64_t)(j * NB + r) * ld + i * NB + c] = -acc[u][v][e]);
+}
+
+__global__ void __launch_bounds__(256) k_chol_inv_gram(WinInfo wi, const double *Y, double *Sinv) {
+  __shared__ double ta[NB][LDT];
+  __shared__ double tb[NB][LDT];
+  const int t = blockIdx.x;
+  int a = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (a * (a + 1) / 2 > t) --a;
+  while ((a + 1) * (a + 2) / 2 <= t) ++a;
+  const int b = t - a * (a + 1) / 2;  // b <= a
+  const int64_t ld = wi.ld;
+  const double *Yw = Y + wi.s_off;
+  double *Sw = Sinv + wi.s_off;
+  dbl4 acc[2][2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v) acc[u][v] = dbl4{0.0, 0.0, 0.0, 0.0};
+  for (int i = a; i < wi.nb; ++i) {
+    {
+      double v0[16], v1[16];
+      fetch_tile(v0, Yw, ld, a * NB, i * NB);
+      fetch_tile(v1, Yw, ld, b * NB, i * NB);
+      put_tile(ta, v0);
+      put_tile(tb, v1);
+    }
+    __syncthreads();
+    mfma_nt_acc(ta, tb, acc);  // += Y_ai Y_bi^T
+    __syncthreads();
+  }
+  // both triangles from the same element (the diagonal tile from its lower half)
+  HSBA_ACC_FOR(if (a != b || r >= c) {
+    Sw[(int64_t)(a * NB + r) * ld + b * NB + c] = acc[u][v][e];
+    Sw[(int64_t)(b * NB + c) * ld + a * NB + r] = acc[u][v][e];
+  });
+}
+#undef HSBA_ACC_FOR
+
+void chol_inverse_impl(const CholPlan &plan, int w, bool large, double *Y, double *Sinv, hipStream_t st) {
+  const WinInfo wi = plan.wi_host[w];
+  const double *Lsrc = large ? plan.Lf : plan.S;
+  for (int i = 0; i < wi.nb; ++i)
+    hipLaunchKernelGGL(k_chol_inv_col, dim3(i + 1), dim3(256), 0, st, wi, Lsrc, plan.Dinv, Y, i);
+  hipLaunchKernelGGL(k_chol_inv_gram, dim3(wi.nb * (wi.nb + 1) / 2), dim3(256), 0, st, wi, Y, Sinv);
+}
+
 // Identity on the padding rows / columns (once, after allocation).
 __global__ void k_pad_identity(const WinInfo *wis, int nwin, double *S) {
   const int w = blockIdx.y;

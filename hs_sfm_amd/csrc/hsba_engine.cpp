@@ -710,6 +710,82 @@ struct Engine final : EngineBase {
   // One damped Newton step on the resident normal equations (window 0):
   // CameraSharedNormalEquationSolver::operator().  S / rhs, when given,
   // receive the reduced system instead of (or besides) the step.
+  // Diagonal blocks of N^-1 at the last build (no damping) and the dense
+  // inverse of the reduced system: the Schur complement S = U - W^T V^-1 W is
+  // factored, S^-1 formed from the factor on the device, and each point's
+  // block from S^-1 and its Z rows.
+  int covariance(double *Vp, double *Ui, double *Uc, double *red) override {
+    if (!built) return HSBA_ERR_GENERIC;
+    if (L.pcg || L.nwin != 1) return HSBA_ERR_UNSUPPORTED;
+    if (L.hw[0].unobserved) return HSBA_ERR_NOT_SPD;
+    int rc = set_state(nullptr, 0.0, 1, 0, 1);
+    if (rc) return rc;
+    const WinInfo &wi = L.wi[0];
+    ks->point_factor(d, st);
+    ks->key_z(d, st);
+    ks->pairs(d, st);
+    ks->assemble(d, st, L.lower_total, (int)L.rows_total);
+    chol_factor_solve_impl(plan, small_list_dev, (int)L.small_list.size(), L.large_list.data(),
+                           (int)L.large_list.size(), st);
+    HCHK(hipGetLastError());
+    if ((rc = get_state())) return rc;
+    if (ws_host[0].chol_fail) return HSBA_ERR_NOT_SPD;
+    const int64_t ld = wi.ld, n = wi.n;
+    double *Y = nullptr, *Sinv = nullptr, *pc = nullptr;
+    auto release = [&]() {
+      if (Y) (void)hipFree(Y);
+      if (Sinv) (void)hipFree(Sinv);
+      if (pc) (void)hipFree(pc);
+    };
+    std::vector<double> hS;
+    if (n > 0) {
+      const size_t bytes = sizeof(double) * (size_t)(wi.s_off + ld * ld);
+      if (hipMalloc((void **)&Y, bytes) != hipSuccess || hipMalloc((void **)&Sinv, bytes) != hipSuccess) {
+        release();
+        return HSBA_ERR_OOM;
+      }
+      const bool large = !L.large_list.empty();
+      chol_inverse_impl(plan, 0, large, Y, Sinv, st);
+    }
+    if (L.pts_free && L.P && Vp) {
+      if (hipMalloc((void **)&pc, sizeof(double) * 9 * L.P) != hipSuccess) {
+        release();
+        return HSBA_ERR_OOM;
+      }
+      ks->point_cov(d, st, Sinv, ld, wi.r_off, pc);
+    }
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(st) != hipSuccess) {
+      release();
+      return HSBA_ERR_DEVICE;
+    }
+    if (pc && hipMemcpy(Vp, pc, sizeof(double) * 9 * L.P, hipMemcpyDeviceToHost) != hipSuccess) {
+      release();
+      return HSBA_ERR_DEVICE;
+    }
+    if (n > 0 && (Ui || Uc || red)) {
+      hS.resize((size_t)ld * ld);
+      if (hipMemcpy(hS.data(), Sinv + wi.s_off, sizeof(double) * hS.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+        release();
+        return HSBA_ERR_DEVICE;
+      }
+    }
+    release();
+    auto at = [&](int64_t i, int64_t j) { return hS[(size_t)(i - wi.r_off) * ld + (j - wi.r_off)]; };
+    if (Ui && L.IMG)
+      for (int64_t i = 0; i < L.I; ++i)
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < 6; ++b) Ui[36 * i + 6 * a + b] = at(L.img_row[i] + a, L.img_row[i] + b);
+    if (Uc && L.D)
+      for (int64_t c = 0; c < L.C; ++c)
+        for (int a = 0; a < L.D; ++a)
+          for (int b = 0; b < L.D; ++b)
+            Uc[(c * L.D + a) * L.D + b] = at(L.cam_row[c] + a, L.cam_row[c] + b);
+    if (red)
+      for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = 0; j < n; ++j) red[i * n + j] = hS[(size_t)i * ld + j];
+    return HSBA_OK;
+  }
+
   int solve(double mu, double *delta, double *S, double *rhs) override {
     if (!built) return HSBA_ERR_GENERIC;
     if (L.hw[0].unobserved) return HSBA_ERR_GENERIC;

@@ -102,6 +102,7 @@ struct EngineBase {
   virtual int build_normal(double *V, double *Ui, double *Uc, double *g) = 0;
   virtual int download_normal(double *V, double *Ui, double *Uc, double *g) = 0;
   virtual int solve(double mu, double *delta, double *S, double *rhs) = 0;
+  virtual int covariance(double *Vp, double *Ui, double *Uc, double *reduced) = 0;
   virtual int evaluate(double *y) = 0;
   virtual int jacobian(double *jp, double *ji, double *jc) = 0;
   virtual int scalars(double *maxdiag, double *ginf) = 0;

@@ -31,6 +31,9 @@ struct KernelSet {
   virtual void pcg_iteration(const Dev<T> &d, hipStream_t st, int glog2) = 0;
   virtual void pcg_finish(const Dev<T> &d, hipStream_t st) = 0;
   virtual void pcg_params(double rtol, int max_it) = 0;
+  // posterior covariance of every point, [P][3][3]: L_p^-T (I + Z_p S^-1 Z_p^T) L_p^-1
+  virtual void point_cov(const Dev<T> &d, hipStream_t st, const double *Sinv, int64_t ld, int r_off,
+                         double *out) = 0;
 };
 
 template <typename T>
@@ -49,5 +52,8 @@ struct CholPlan {
 void chol_init_padding(const CholPlan &plan, hipStream_t st);
 void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int num_small,
                             const int *large_host, int num_large, hipStream_t st);
+// S^-1 of window w from its factor (after chol_factor_solve_impl): Y (ld x ld
+// scratch at the window's offset) receives L^-T, Sinv the full symmetric inverse.
+void chol_inverse_impl(const CholPlan &plan, int w, bool large, double *Y, double *Sinv, hipStream_t st);
 
 }  // namespace hsba

@@ -875,7 +875,15 @@ __global__ void __launch_bounds__(64) k_lm_control(Dev<T> d, int phase) {
   } else if (phase == 2) {
     if (!(s.active && s.trial)) return;
     if (s.chol_fail) {
-      s.status = -3; s.stop_reason = 3; s.active = 0; s.trial = 0; return;
+      // (N + mu I) not factorable (in mixed precision the FP32 Schur
+      // complement of an ill-conditioned problem loses definiteness as mu
+      // shrinks): a rejected step, more damping next time
+      s.mu *= s.nu;
+      s.nu *= 2.0;
+      s.need_lin = 0;
+      s.trial = 0;
+      if (!(s.mu < 1.0e300)) { s.status = -3; s.stop_reason = 3; s.active = 0; }
+      return;
     }
     s.dnorm2 = dn; s.xnorm2 = xn; s.gain_den = gd;
     if (!(dn == dn)) { s.status = -4; s.stop_reason = 3; s.active = 0; s.trial = 0; return; }
@@ -959,6 +967,85 @@ __global__ void k_commit(Dev<T> d) {
       if (q < D) d.cam_free[(int64_t)c * D + q] = d.cam_free_new[(int64_t)c * D + q];
     }
   }
+}
+
+// ------------------------------------------------------------ posterior covariance
+// Point block of N^-1 (N undamped): with V_p = L L^T and Z_a = L^-1 W_a,
+//   (N^-1)_pp = V^-1 + V^-1 W S^-1 W^T V^-1 = L^-T (I + sum_{a,b} Z_a S^-1_ab Z_b^T) L^-1
+// over the point's keys a, b (S^-1_ab = the E x E block of S^-1 at the rows of
+// key a's image / camera and key b's).  One wavefront per point: lane a forms
+// U_a = sum_b S^-1_ab Z_b^T, then the lanes sum Z_a U_a.  Replaces the point
+// blocks of the dense pseudo-inverse of
+// camera_shared_analytical_x_covariance_calculator.hpp:64-87 (equal when N is
+// nonsingular).
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(256) k_point_cov(Dev<T> d, const double *Sinv, int64_t ld, int r_off, double *out) {
+  constexpr int E = IMG + D;
+  constexpr int ZS = z_stride(E > 0 ? E : 1, sizeof(T));
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (p >= d.P) return;  // the whole wavefront
+  double m[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) m[q] = 0.0;
+  if (E > 0) {
+    const int kb = d.pt_ptr[p], ke = d.pt_ptr[p + 1];
+    for (int a = kb + lane; a < ke; a += 64) {
+      const int ia = d.key_img[a];
+      int ra[E > 0 ? E : 1];
+#pragma unroll
+      for (int j = 0; j < E; ++j) ra[j] = (j < IMG ? d.img_row[ia] + j : d.cam_row[d.img_cam[ia]] + (j - IMG)) - r_off;
+      double u[E > 0 ? E : 1][3];
+#pragma unroll
+      for (int i = 0; i < E; ++i) u[i][0] = u[i][1] = u[i][2] = 0.0;
+      for (int b = kb; b < ke; ++b) {
+        const int ib = d.key_img[b];
+        const T *zb = d.Z + (int64_t)b * ZS;
+        double z[3][E > 0 ? E : 1];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int j = 0; j < E; ++j) z[c][j] = (double)zb[c * E + j];
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+          const int rb = (j < IMG ? d.img_row[ib] + j : d.cam_row[d.img_cam[ib]] + (j - IMG)) - r_off;
+#pragma unroll
+          for (int i = 0; i < E; ++i) {
+            const double g = Sinv[(int64_t)ra[i] * ld + rb];
+            u[i][0] += g * z[0][j];
+            u[i][1] += g * z[1][j];
+            u[i][2] += g * z[2][j];
+          }
+        }
+      }
+      const T *za = d.Z + (int64_t)a * ZS;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          const double zv = (double)za[r * E + i];
+          m[3 * r] += zv * u[i][0];
+          m[3 * r + 1] += zv * u[i][1];
+          m[3 * r + 2] += zv * u[i][2];
+        }
+    }
+  }
+  for (int s = 1; s < 64; s <<= 1) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) m[q] += __shfl_xor(m[q], s, 64);
+  }
+  if (lane != 0) return;
+  m[0] += 1.0; m[4] += 1.0; m[8] += 1.0;
+  // C = Li^T M Li, Li = L^-1 lower
+  const T *Lp = d.Linv + 6 * p;
+  const double Li[3][3] = {{(double)Lp[0], 0.0, 0.0}, {(double)Lp[1], (double)Lp[2], 0.0},
+                           {(double)Lp[3], (double)Lp[4], (double)Lp[5]}};
+  double t[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) t[r][c] = m[3 * r] * Li[0][c] + m[3 * r + 1] * Li[1][c] + m[3 * r + 2] * Li[2][c];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      out[9 * (int64_t)p + 3 * r + c] = Li[0][r] * t[0][c] + Li[1][r] * t[1][c] + Li[2][r] * t[2][c];
 }
 
 // ------------------------------------------------------------ export kernels
@@ -1372,6 +1459,10 @@ struct KernelSetImpl final : KernelSet<T> {
   void pcg_params(double rtol, int max_it) override {
     pcg_rtol = rtol;
     pcg_max_it = max_it;
+  }
+  void point_cov(const Dev<T> &d, hipStream_t st, const double *Sinv, int64_t ld, int r_off, double *out) override {
+    if (d.P == 0) return;
+    hipLaunchKernelGGL((k_point_cov<T, IMG, D>), dim3((d.P + 3) / 4), dim3(256), 0, st, d, Sinv, ld, r_off, out);
   }
   void export_keys(const Dev<T> &d, hipStream_t st, double *y, double *jp, double *ji, double *jc,
                    int want_j) override {

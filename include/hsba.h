@@ -231,6 +231,22 @@ int hsba_solve(hsba_problem *problem, double mu, double *delta_x);
 int hsba_reduced_system(hsba_problem *problem, double mu,
                         double *reduced_matrix, double *reduced_rhs);
 
+/* Posterior covariance at x: CameraSharedAnalyticalXCovarianceCalculator
+ * (camera_shared_analytical_x_covariance_calculator.hpp:45-90) forms the dense
+ * N = J^T Sigma^-1 J and its pseudo-inverse by SVD.  Here the diagonal blocks
+ * of N^-1 come from the Schur complement on the device (S = U - W^T V^-1 W is
+ * factored and inverted, each point block is L_p^-T (I + Z_p S^-1 Z_p^T) L_p^-1),
+ * which equals the reference's pseudo-inverse when N is nonsingular, i.e. when
+ * the gauge is fixed (constraints, GCPs or fixed blocks).  A singular N (free
+ * gauge) fails with HSBA_ERR_NOT_SPD.  Outputs (any may be NULL): point blocks
+ * [P][3][3], image blocks [I][6][6], camera blocks [C][d][d], and the dense
+ * inverse of the reduced system [reduced_size][reduced_size] (image and camera
+ * block of N^-1).  y_obs may be NULL (N depends on y only through a robust
+ * loss).  Dense-solver problems only (HSBA_ERR_UNSUPPORTED under PCG). */
+int hsba_covariance(hsba_problem *problem, const double *x, const double *y_obs,
+                    const hsba_ycov_inverse *ycov, double *point_blocks, double *image_blocks,
+                    double *camera_blocks, double *reduced);
+
 /* LM scalar helpers on the last build:
  *   max diag(N)            camera_shared_normal_matrix_max_diagonal_value_calculator.hpp:26-38
  *   ||g||_inf              camera_shared_gradient_infinite_norm_calculator.hpp:27-39

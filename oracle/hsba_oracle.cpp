@@ -1064,8 +1064,14 @@ int oracle_lm(void *h, const double *x0, const double *y_obs, const double *key_
   while (!found && k < opt->max_iterations) {
     ++k;
     if (solve(o, mu, delta.data(), literal)) {
-      st.status = HSBA_ERR_NOT_SPD; st.stop_reason = HSBA_STOP_FAILED;
-      break;
+      // a damped system that does not factor is a rejected step (the
+      // hs_optimizor rule on solver failure is not available; this is the
+      // device's rule)
+      oracle_lm_trace tf = {mu, F, 0.0, 0.0, 0.0, 0, 0};
+      if (trace) trace[k - 1] = tf;
+      mu *= nu; nu *= 2.0;
+      if (!(mu < 1.0e300)) { st.status = HSBA_ERR_NOT_SPD; st.stop_reason = HSBA_STOP_FAILED; break; }
+      continue;
     }
     double dn = 0.0, xn2 = 0.0;
     for (int64_t j = 0; j < o.x_size; ++j) { dn += delta[j] * delta[j]; xn2 += x[j] * x[j]; }

@@ -145,7 +145,7 @@ def bal_world(seed=4):
         ax /= np.linalg.norm(ax)
         ang = [0.3, 1.2, np.pi - 1e-9, 2.9, 1e-12][c]  # includes angles at pi and near 0
         r = ax * ang
-        t = np.array([0.0, 0.0, -10.0])  # the origin sits 10 units down the camera's -z axis
+        t = np.array([0.0, 0.0, -3.0])  # the origin 3 units down the camera's -z axis: a wide field of view
         cams.append(np.concatenate([r, t, [800 + 40 * c, rng.normal(0, 1e-2), rng.normal(0, 1e-3)]]))
     cams = np.array(cams)
     pts = rng.uniform(-1, 1, (P, 3))
