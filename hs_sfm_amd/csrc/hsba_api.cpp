@@ -56,7 +56,17 @@ bool auto_pcg(const hsba_problem_desc &desc) {
   const int dcam = __builtin_popcount(fb);
   const int64_t n = ((desc.fix_mask & HSBA_FIX_IMAGES) ? 0 : 6 * desc.num_images) +
                     ((desc.fix_mask & HSBA_FIX_CAMERAS) ? 0 : dcam * desc.num_cameras);
-  return n > 4096;
+  if (n <= 4096) return false;
+  // the PCG preconditioner pairs each image with its own camera block: with
+  // free intrinsics shared between images the dense path is taken instead
+  if (!(desc.fix_mask & HSBA_FIX_CAMERAS) && dcam > 0 && desc.image_camera_map) {
+    std::vector<int32_t> per_cam(std::max<int64_t>(0, desc.num_cameras), 0);
+    for (int64_t i = 0; i < desc.num_images; ++i) {
+      const int32_t c = desc.image_camera_map[i];
+      if (c < 0 || c >= desc.num_cameras || ++per_cam[c] > 1) return false;
+    }
+  }
+  return true;
 }
 
 int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_options *opt, hipStream_t stream,

@@ -91,15 +91,19 @@ struct FactorScratch {
 // The whole workgroup (256 threads, one element each): factor the 16 x 16
 // diagonal block of wa at (c0, c0) (lower triangle, SPD) in place into its
 // Cholesky factor L_p, and write L_p^-1 (lower, zero upper) to the same block
-// of wx.  Elimination of [A | I] in LDL^T form, two columns per barrier:
-// columns j, j+1 and rows j, j+1 of x are published as they stand, and every
-// thread eliminates both itself,
-//   d_j = a[j][j], l = a[j+1][j] / d_j, d_j+1 = a[j+1][j+1] - l a[j+1][j]
-//   f_i = a[i][j] / d_j, u_i = a[i][j+1] - f_i a[j+1][j], g_i = u_i / d_j+1
-//   a[i][c] -= f_i a[c][j] + g_i u_c              (c > j+1, i >= c)
-//   x[i][:] -= f_i x[j][:] + g_i (x[j+1][:] - l x[j][:])   (i > j+1)
-// which leaves a[i][j] = Lt[i][j] d_j and x = Lt^-1, so L = Lt D^1/2 and
-// L^-1 = D^-1/2 x.  Thread t owns element (t / 16, t % 16).
+// of wx.  Elimination of [A | I] by 2 x 2 pivot blocks, one barrier per pair
+// of columns: columns j, j+1 and rows j, j+1 of x are published as they
+// stand, and every thread eliminates both at once with the pivot block
+// P = [[d0, e], [e, d1r]] (one reciprocal, of det P, on the critical path):
+//   [f_i, g_i] = [a[i][j], a[i][j+1]] P^-1
+//   a[i][c] -= f_i a[c][j] + g_i a[c][j+1]            (c > j+1, i >= c)
+//   x[i][:] -= f_i x[j][:] + g_i x[j+1][:]             (i > j+1)
+// which is the two single-column steps of LDL^T at once.  Column j keeps
+// Lt[i][j] d0; column j+1 and row j+1 of x take their single-step values
+//   a[i][j+1] = a[i][j+1] - a[i][j] e / d0,  d1 = d1r - e^2 / d0,
+//   x[j+1][:] -= (e / d0) x[j][:]
+// (off the critical path).  At the end a = Lt D and x = Lt^-1, so
+// L = Lt D^1/2 and L^-1 = D^-1/2 x.  Thread t owns element (t / 16, t % 16).
 __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int c0, FactorScratch &fs,
                                        bool &bad) {
   const int tid = threadIdx.x;
@@ -118,21 +122,22 @@ __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int
     const double acj = fs.col[b][0][c], ac1 = fs.col[b][1][c];
     const double d0 = fs.col[b][0][j], e = fs.col[b][0][j + 1], d1r = fs.col[b][1][j + 1];
     const double x0c = fs.xrow[b][0][c], x1c = fs.xrow[b][1][c];
-    const bool ok0 = d0 > 0.0;
-    const double inv0 = ok0 ? recip_nr(d0) : 0.0;
-    const double l = e * inv0;
-    const double d1 = d1r - e * l;
-    const bool ok1 = d1 > 0.0;
-    const double inv1 = ok1 ? recip_nr(d1) : 0.0;
-    bad = bad || !ok0 || !ok1;
-    if (tid == 0) { fs.d[j] = d0; fs.d[j + 1] = d1; }
-    const double fi = aij * inv0, ui = ai1 - fi * e, gi = ui * inv1;
-    const double fc = acj * inv0, uc = ac1 - fc * e;
-    const double xh = x1c - l * x0c;
+    const double det = d0 * d1r - e * e;
+    const bool ok = d0 > 0.0 && det > 0.0;
+    const double idet = ok ? recip_nr(det) : 0.0;
+    const double fi = (aij * d1r - ai1 * e) * idet, gi = (ai1 * d0 - aij * e) * idet;
     const bool lower = i >= c;
-    a = (lower && c > j + 1) ? a - (fi * acj + gi * uc) : a;
-    a = (lower && c == j + 1) ? (i == j + 1 ? d1 : ui) : a;
-    x = (i > j + 1) ? x - (fi * x0c + gi * xh) : (i == j + 1 ? xh : x);
+    const double a_upd = a - (fi * acj + gi * ac1);
+    const double x_upd = x - (fi * x0c + gi * x1c);
+    // single-step values of column j+1, pivot d1 and row j+1 of x
+    const double inv0 = d0 > 0.0 ? recip_nr(d0) : 0.0;
+    const double l = e * inv0;
+    const double d1 = det * inv0;
+    bad = bad || !ok;
+    if (tid == 0) { fs.d[j] = d0; fs.d[j + 1] = d1; }
+    a = (lower && c > j + 1) ? a_upd : a;
+    a = (lower && c == j + 1) ? (i == j + 1 ? d1 : ai1 - aij * l) : a;
+    x = (i > j + 1) ? x_upd : (i == j + 1 ? x1c - l * x0c : x);
   }
   __syncthreads();
   const double dc = fs.d[c] > 0.0 ? fs.d[c] : 1.0, di = fs.d[i] > 0.0 ? fs.d[i] : 1.0;
