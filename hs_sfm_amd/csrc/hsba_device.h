@@ -71,6 +71,13 @@ __host__ __device__ constexpr int z_stride(int e, int tbytes) {
   return (3 * e * tbytes + 127) / 128 * (128 / tbytes);
 }
 
+// One incidence of a camera pair: the keys a (first image) and b (second image)
+// of one point, as a << 32 | b.  64 bits, so neither the track length nor the
+// number of keys of a problem is limited by the code.
+__host__ __device__ constexpr uint64_t inc_code(int64_t a, int64_t b) { return ((uint64_t)a << 32) | (uint64_t)b; }
+__host__ __device__ constexpr int inc_a(uint64_t c) { return (int)(c >> 32); }
+__host__ __device__ constexpr int inc_b(uint64_t c) { return (int)(c & 0xffffffffu); }
+
 // Huber loss on a key's squared whitened residual s (scale a):
 // rho(s) = s for s <= a^2, 2 a sqrt(s) - a^2 beyond; rho'(s) = min(1, a / sqrt(s)).
 // rho'' <= 0 everywhere, so the robustified Gauss-Newton model is plain
@@ -117,7 +124,7 @@ struct Dev {
   const int32_t *key_ipos, *ikey_pt;
   // camera pairs
   const int64_t *pair_ptr;
-  const uint32_t *inc;
+  const uint64_t *inc;      // pair incidences: key a << 32 | key b
   const int32_t *pair_win, *pair_img;  // pair_img >= 0: diagonal pair of that image
   T *pair_out;
   // windows
@@ -139,6 +146,7 @@ struct Dev {
   double *pv, *pr, *pz, *pp, *pAp, *Minv;
   T *tpt;                   // unused
   double *tpd;              // [P][3] point-space intermediate of the implicit matvec
+  double *pdot;             // [I][3] per-image partials of p.Ap, r.z, r.r (summed per window in image order)
   PcgState *pcg;
 };
 

@@ -133,7 +133,7 @@ struct Engine final : EngineBase {
     int32_t *cam_ptr, *cam_imgs, *cam_win, *cam_row, *pair_win, *pair_img, *row_blk, *row_sub;
     int32_t *ck, *cbg, *cen, *cw;
     int64_t *pair_ptr, *blk_ptr;
-    uint32_t *inc;
+    uint64_t *inc;
     Contrib *contrib;
     ConstraintRow *cons;
     WinInfo *wi;
@@ -169,7 +169,7 @@ struct Engine final : EngineBase {
     A(d.partial, 4LL * d.nchunks);
     A(d.pv, L.rows_total); A(d.pr, L.rows_total); A(d.pz, L.rows_total); A(d.pp, L.rows_total);
     A(d.pAp, L.rows_total); A(d.Minv, (int64_t)E * E * L.I);
-    A(d.tpd, 3 * L.P); A(d.pcg, L.nwin);
+    A(d.tpd, 3 * L.P); A(d.pcg, L.nwin); A(d.pdot, 3 * L.I);
     A(small_list_dev, (int64_t)L.small_list.size());
 #undef A
     if ((rc = upload(key_img, L.key_img, st)) || (rc = upload(key_pt, L.key_pt, st)) ||

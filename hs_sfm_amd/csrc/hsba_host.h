@@ -34,7 +34,7 @@ struct Layout {
   std::vector<int32_t> key_ipos, ikey_pt;  // PCG: key -> image-major row, row -> point
   std::vector<int32_t> cam_ptr, cam_imgs, cam_win, cam_row;
   std::vector<int64_t> pair_ptr;
-  std::vector<uint32_t> inc;
+  std::vector<uint64_t> inc;
   std::vector<int32_t> pair_win, pair_img;
   std::vector<int64_t> blk_ptr;
   std::vector<Contrib> contrib;

@@ -374,9 +374,9 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
 #pragma unroll
   for (int j = 0; j < E; ++j) racc[j] = 0;
   const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
-  auto fetch = [&](uint32_t code, vec za[NV], vec zb[NV], T qv[3]) {
-    const int a = int(code >> 7);
-    const int b = a + int(code & 127u);
+  auto fetch = [&](uint64_t code, vec za[NV], vec zb[NV], T qv[3]) {
+    const int a = inc_a(code);
+    const int b = inc_b(code);
     const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
     const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
 #pragma unroll
@@ -402,12 +402,12 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
   T qc[3] = {0, 0, 0};
   int64_t j = jb + lane;
   if (j < je) fetch(d.inc[j], va, vb, qc);
-  uint32_t cn = (j + 64 < je) ? d.inc[j + 64] : 0u;
+  uint64_t cn = (j + 64 < je) ? d.inc[j + 64] : 0u;
   for (; j < je; j += 64) {
     vec na[NV], nb[NV];
     T qn[3] = {0, 0, 0};
     const bool more = j + 64 < je;
-    const uint32_t cnn = (j + 128 < je) ? d.inc[j + 128] : 0u;
+    const uint64_t cnn = (j + 128 < je) ? d.inc[j + 128] : 0u;
     if (more) fetch(cn, na, nb, qn);
     mult(va, vb, qc);
     if (more) {
@@ -491,8 +491,8 @@ __global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d) {
   for (int64_t base = jb; base < je; base += SI) {
     const int n = je - base < SI ? (int)(je - base) : SI;
     if (ls < n) {
-      const uint32_t code = d.inc[base + ls];
-      const int a = int(code >> 7), b = a + int(code & 127u);
+      const uint64_t code = d.inc[base + ls];
+      const int a = inc_a(code), b = inc_b(code);
       const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
       const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
       vec v[2 * NV / 4];
@@ -1144,19 +1144,37 @@ __global__ void k_pcg_precond(Dev<T> d) {
     for (int b = 0; b < E; ++b) out[a * E + b] = b <= a ? M[a][b] : 0.0;
 }
 
-// z = M^-1 r per super-block (init = 1 also sets x = 0 and p = z).
+// Per super-block (image and its own camera, one thread per image):
+//   init = 1:  r = rhs, x = 0, z = M^-1 r, p = z
+//   init = 0:  x += alpha p, r -= alpha Ap, z = M^-1 r
+// and the block's partials of r.z and r.r for the window's dot product.
 template <typename T, int IMG, int D>
 __global__ void k_pcg_apply(Dev<T> d, int init) {
   constexpr int E = IMG + D;
   const int im = blockIdx.x * blockDim.x + threadIdx.x;
   if (im >= d.I) return;
   const int w = d.img_win[im];
-  if (!pcg_live(d.ws[w], d.pcg[w])) return;
+  const PcgState &g = d.pcg[w];
+  if (!pcg_live(d.ws[w], g)) return;
   const int c = d.img_cam[im];
   const double *L = d.Minv + (int64_t)im * E * E;
-  double v[E > 0 ? E : 1];
+  const double alpha = g.alpha;
+  double v[E > 0 ? E : 1], rv[E > 0 ? E : 1];
 #pragma unroll
-  for (int j = 0; j < E; ++j) v[j] = d.pr[sup_row<IMG, D>(d.img_row, d.cam_row, c, im, j)];
+  for (int j = 0; j < E; ++j) {
+    const int R = sup_row<IMG, D>(d.img_row, d.cam_row, c, im, j);
+    double r;
+    if (init) {
+      r = d.rhs[R];
+      d.dred[R] = 0.0;
+    } else {
+      d.dred[R] += alpha * d.pp[R];
+      r = d.pr[R] - alpha * d.pAp[R];
+    }
+    d.pr[R] = r;
+    rv[j] = r;
+    v[j] = r;
+  }
 #pragma unroll
   for (int i = 0; i < E; ++i) {
 #pragma unroll
@@ -1169,12 +1187,17 @@ __global__ void k_pcg_apply(Dev<T> d, int init) {
     for (int m = i + 1; m < E; ++m) v[i] -= L[m * E + i] * v[m];
     v[i] /= L[i * E + i];
   }
+  double rz = 0.0, rr = 0.0;
 #pragma unroll
   for (int j = 0; j < E; ++j) {
-    const int r = sup_row<IMG, D>(d.img_row, d.cam_row, c, im, j);
-    d.pz[r] = v[j];
-    if (init) { d.pp[r] = v[j]; d.dred[r] = 0.0; }
+    const int R = sup_row<IMG, D>(d.img_row, d.cam_row, c, im, j);
+    d.pz[R] = v[j];
+    if (init) d.pp[R] = v[j];
+    rz += rv[j] * v[j];
+    rr += rv[j] * rv[j];
   }
+  d.pdot[3 * im + 1] = rz;
+  d.pdot[3 * im + 2] = rr;
 }
 
 // t_p = sum_{b in p} Z_b v_sup(b), one lane group per point.
@@ -1255,19 +1278,26 @@ __global__ void __launch_bounds__(128) k_pcg_images(Dev<T> d) {
 #pragma unroll
     for (int j = 0; j < E; ++j) red[wave][j] = acc[j];
   __syncthreads();
-  if (threadIdx.x < E) {
-    const int a = threadIdx.x;
-    const T *Di = d.Dimg + (int64_t)im * E * E;
-    double y = 0.0;
-    for (int b = 0; b < E; ++b) {
-      double u;
-      if (a >= IMG && b >= IMG) u = (double)d.Uc[(int64_t)c * (D > 0 ? D : 1) * D + (a - IMG) * D + (b - IMG)];
-      else u = (double)Di[a * E + b];
-      y += u * d.pp[sup_row<IMG, D>(d.img_row, d.cam_row, c, im, b)];
+  if (threadIdx.x < 64) {  // wave 0: rows of the super-block, then p.Ap over them
+    double prod = 0.0;
+    if (threadIdx.x < E) {
+      const int a = threadIdx.x;
+      const T *Di = d.Dimg + (int64_t)im * E * E;
+      double y = 0.0;
+      for (int b = 0; b < E; ++b) {
+        double u;
+        if (a >= IMG && b >= IMG) u = (double)d.Uc[(int64_t)c * (D > 0 ? D : 1) * D + (a - IMG) * D + (b - IMG)];
+        else u = (double)Di[a * E + b];
+        y += u * d.pp[sup_row<IMG, D>(d.img_row, d.cam_row, c, im, b)];
+      }
+      const int r = sup_row<IMG, D>(d.img_row, d.cam_row, c, im, a);
+      y += d.ws[w].mu * d.pp[r];
+      const double ap = y - (d.pts_free ? (red[0][a] + red[1][a]) : 0.0);
+      d.pAp[r] = ap;
+      prod = d.pp[r] * ap;
     }
-    const int r = sup_row<IMG, D>(d.img_row, d.cam_row, c, im, a);
-    y += d.ws[w].mu * d.pp[r];
-    d.pAp[r] = y - (d.pts_free ? (red[0][a] + red[1][a]) : 0.0);
+    for (int m = 1; m < 64; m <<= 1) prod += __shfl_xor(prod, m, 64);
+    if (threadIdx.x == 0) d.pdot[3 * im] = prod;
   }
 }
 
@@ -1280,11 +1310,13 @@ __global__ void __launch_bounds__(256) k_pcg_dot(Dev<T> d, int mode) {
   const WinInfo &wi = d.wi[w];
   PcgState &g = d.pcg[w];
   if (!pcg_live(d.ws[w], g)) return;
+  // every row of the window lies in exactly one image's super-block (one
+  // image per camera), so the per-image partials add up to the dot products
   double a0 = 0.0, a1 = 0.0;
-  for (int i = threadIdx.x; i < wi.n; i += blockDim.x) {
-    const int r = wi.r_off + i;
-    if (mode == 1) a0 += d.pp[r] * d.pAp[r];
-    else { a0 += d.pr[r] * d.pz[r]; a1 += d.pr[r] * d.pr[r]; }
+  for (int i = threadIdx.x; i < wi.num_imgs; i += blockDim.x) {
+    const double *pd = d.pdot + 3 * (int64_t)(wi.img_off + i);
+    if (mode == 1) a0 += pd[0];
+    else { a0 += pd[1]; a1 += pd[2]; }
   }
   __shared__ double red[2][256];
   red[0][threadIdx.x] = a0; red[1][threadIdx.x] = a1;
@@ -1436,7 +1468,6 @@ struct KernelSetImpl final : KernelSet<T> {
     hipLaunchKernelGGL((k_pcg_precond<T, IMG, D>), dim3((d.I + 63) / 64), dim3(64), 0, st, d);
     hipLaunchKernelGGL((k_pcg_reset<T>), dim3((d.nwin + 63) / 64), dim3(64), 0, st, d, pcg_rtol * pcg_rtol,
                        pcg_max_it);
-    hipLaunchKernelGGL((k_pcg_vec<T>), dim3((rows + 255) / 256), dim3(256), 0, st, d, 2, rows);
     hipLaunchKernelGGL((k_pcg_apply<T, IMG, D>), dim3((d.I + 63) / 64), dim3(64), 0, st, d, 1);
     hipLaunchKernelGGL((k_pcg_dot<T>), dim3(d.nwin), dim3(256), 0, st, d, 0);
   }
@@ -1448,7 +1479,6 @@ struct KernelSetImpl final : KernelSet<T> {
       hipLaunchKernelGGL((k_pcg_points<T, IMG, D>), dim3((threads + 255) / 256), dim3(256), 0, st, d, glog2);
     hipLaunchKernelGGL((k_pcg_images<T, IMG, D>), dim3(d.I), dim3(128), 0, st, d);
     hipLaunchKernelGGL((k_pcg_dot<T>), dim3(d.nwin), dim3(256), 0, st, d, 1);
-    hipLaunchKernelGGL((k_pcg_vec<T>), dim3((rows + 255) / 256), dim3(256), 0, st, d, 0, rows);
     hipLaunchKernelGGL((k_pcg_apply<T, IMG, D>), dim3((d.I + 63) / 64), dim3(64), 0, st, d, 0);
     hipLaunchKernelGGL((k_pcg_dot<T>), dim3(d.nwin), dim3(256), 0, st, d, 2);
     hipLaunchKernelGGL((k_pcg_vec<T>), dim3((rows + 255) / 256), dim3(256), 0, st, d, 1, rows);

@@ -124,8 +124,6 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
     }
     L.P += h.P; L.I += h.I; L.C += h.C; L.N += h.N;
   }
-  if (L.N >= (1LL << 25) && L.pts_free && L.E > 0)
-    return fail(why, "more than 2^25 keys in one dense-Schur problem", HSBA_ERR_UNSUPPORTED);
   if (L.P >= (1LL << 31) || L.N >= (1LL << 31)) return fail(why, "too large", HSBA_ERR_UNSUPPORTED);
 
   L.key_img.resize(L.N); L.key_pt.resize(L.N); L.key_orig.resize(L.N);
@@ -202,8 +200,6 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
       }
     }
     if (h.P == 0) L.pt_ptr[pt_off] = (int32_t)key_off;
-    if (L.pts_free && L.E > 0 && max_track > 127)
-      return fail(why, "a point seen by more than 127 images", HSBA_ERR_UNSUPPORTED);
     // ---- images and cameras
     for (int64_t i = 0; i < h.I; ++i) {
       L.img_win[img_off + i] = (int32_t)w;
@@ -344,7 +340,7 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
             const int64_t row = (L.key_img[u] - img_off) * h.I - img_off;
             for (int64_t v = u; v < (pcg ? u + 1 : e); ++v) {
               const int32_t q = pid[(size_t)(row + L.key_img[v])];
-              L.inc[cur[q - first_pair]++] = (uint32_t)((u << 7) | (v - u));
+              L.inc[cur[q - first_pair]++] = inc_code(u, v);
             }
           }
         }
@@ -457,7 +453,7 @@ int64_t layout_device_bytes(const Layout &L, int precision) {
   b += L.P * (8 + 3 * 8 * 3 + t * 18);                              // points
   b += L.I * (4 * 5 + 6 * 8 * 2 + t * (E * E + 2 * E));             // images
   b += L.C * (4 * 3 + 8 * (2 * D + 20) + t * (D * D + D));           // cameras
-  b += L.npairs * (8 + 8 + t * E * E) + L.incidences * 4;           // pairs
+  b += L.npairs * (8 + 8 + t * E * E) + L.incidences * 8;           // pairs
   b += (L.S_total + L.dinv_total) * 8 + L.rows_total * (8 * 8 + 8); // reduced systems
   b += (int64_t)L.contrib.size() * sizeof(Contrib) + (int64_t)L.blk_ptr.size() * 8;
   b += (int64_t)L.chunk_kind.size() * (16 + 32) + (int64_t)L.nwin * (sizeof(WinInfo) + sizeof(WinState));
@@ -520,7 +516,7 @@ int64_t estimate_device_bytes(const hsba_problem_desc &ds, int precision, bool p
   b += P * (8 + 3 * 8 * 3 + t * 18 + 24);
   b += I * (4 * 5 + 6 * 8 * 2 + t * (2 * E * E + 2 * E) + 8);
   b += C * (4 * 4 + 8 * (2 * D + 20) + t * (D * D + D));
-  b += npairs * (8 + 8 + t * E * E) + inc * 4;
+  b += npairs * (8 + 8 + t * E * E) + inc * 8;
   b += (2 * S + dinv) * 8 + ld * (8 * 10 + 8);                       // S, factor, reduced vectors
   b += (4 * npairs + 2 * nblk) * (int64_t)sizeof(Contrib) + (nblk * (nblk + 1) / 2 + 1) * 8;
   b += chunks * (16 + 32) + 4096;

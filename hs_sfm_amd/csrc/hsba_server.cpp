@@ -136,8 +136,7 @@ struct hsba_server {
 
   // Admission (caller holds mu): the oldest job and the later queued jobs of
   // its class that fit the budget and the active-job limit, in arrival order.
-  // Keys per batch: a batch's keys share one index space (below 2^25 for the
-  // dense path), and a bounded batch bounds the latency of its first job.
+  // Keys per batch: a bounded batch bounds the latency of its first job.
   static constexpr int64_t kMaxBatchKeys = int64_t(1) << 23;
 
   bool take_group(std::vector<std::unique_ptr<Job>> &g) {

@@ -188,3 +188,29 @@ def test_pcg_fp64_fixed_cameras(gpu):
     _, xo, sto, _ = o.lm(max_iterations=10)
     assert rc == 0
     assert abs(st.final_cost - sto.final_cost) < 1e-6 * sto.final_cost
+
+
+def test_auto_solver_keeps_shared_cameras_dense(gpu):
+    """Above 4096 reduced unknowns the automatic choice is PCG, whose
+    preconditioner needs one image per camera; with one camera group shared by
+    700 images (free intrinsics) the dense path is taken instead."""
+    s = syn.tiny_scene(num_images=700, num_points=300, num_cameras=1, kmin=2)
+    p = Problem(s, precision=abi.PRECISION_MIXED)
+    print(f"auto solver n_reduced={p.n_reduced}", flush=True)
+    assert p.n_reduced == 6 * 700 + p.cam_d
+    p.build_normal(s.x0)
+    rc, dx = p.solve(1e-3 * p.max_diagonal())
+    assert rc == 0 and np.isfinite(dx).all()
+
+
+def test_long_tracks(gpu):
+    """Points seen by 150..200 images (pair incidences index keys with 32 bits
+    each, so track length is not limited); mixed precision LM vs the oracle."""
+    s = syn.tiny_scene(num_images=200, num_points=40, num_cameras=2, kmin=150, constraints=True)
+    assert np.bincount(s.feature_maps[:, 1]).max() > 128
+    o, p = Oracle(s), Problem(s, precision=abi.PRECISION_MIXED)
+    print(f"long tracks keys={s.num_keys}", flush=True)
+    rc_o, xo, st_o, _ = o.lm(max_iterations=5)
+    rc_g, xg, st_g = p.lm(max_iterations=5)
+    assert rc_o == 0 and rc_g == 0
+    assert abs(syn.reprojection_rmse(s, xg) - syn.reprojection_rmse(s, xo)) < 1e-3
