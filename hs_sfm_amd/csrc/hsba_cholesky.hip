@@ -156,12 +156,11 @@ __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int
 // then the off-diagonal blocks of L^-1 by block diagonals s = i - j = 1..3:
 //   T_ij = sum_{k=j}^{i-1} L_ik X_kj,  X_ij = -M_i T_ij
 // (T is staged in the unused upper triangle of wa).  256 threads.
-__device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dstL, int64_t ldl,
-                                                    double *dstI, int64_t ldi, int *fail, double (*wa)[LDT],
-                                                    double (*wx)[LDT], FactorScratch &fs) {
+// tile_factor_inverse_lds: the same with the tile already in wa (synchronised).
+__device__ __forceinline__ void tile_factor_inverse_lds(double *dstL, int64_t ldl, double *dstI, int64_t ldi,
+                                                        int *fail, double (*wa)[LDT], double (*wx)[LDT],
+                                                        FactorScratch &fs) {
   const int tid = threadIdx.x;
-  load_tile(wa, src, lds, 0, 0);
-  __syncthreads();
   bool bad = false;
   const int rr = tid >> 4, cc = tid & 15;
 #pragma unroll
@@ -248,6 +247,14 @@ __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t l
   }
 }
 
+__device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dstL, int64_t ldl,
+                                                    double *dstI, int64_t ldi, int *fail, double (*wa)[LDT],
+                                                    double (*wx)[LDT], FactorScratch &fs) {
+  load_tile(wa, src, lds, 0, 0);
+  __syncthreads();
+  tile_factor_inverse_lds(dstL, ldl, dstI, ldi, fail, wa, wx, fs);
+}
+
 // 64 x 64 x 64 products on the matrix cores (v_mfma_f64_16x16x4_f64).  Wave
 // w owns the 32 x 32 quadrant (rows 32 (w / 2), columns 32 (w % 2)) as 2 x 2
 // blocks of 16 x 16.  Operand lanes: A[i][k] from lane i + 16 k, B[k][j] from
@@ -311,6 +318,28 @@ __device__ __forceinline__ void tile_sub_nt_global(double *C, int64_t ld, int r0
     for (int v = 0; v < 2; ++v)
 #pragma unroll
       for (int e = 0; e < 4; ++e) C[(int64_t)(rr + 16 * u + 4 * e) * ld + cc + 16 * v] = cv[u][v][e] - acc[u][v][e];
+}
+
+// out (LDS) = C (global 64 x 64 at r0, c0) - a b^T; out must not alias a or b.
+__device__ __forceinline__ void tile_sub_nt_to_lds(double (*out)[LDT], const double *C, int64_t ld, int r0, int c0,
+                                                   const double (*a)[LDT], const double (*b)[LDT]) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int rl = 32 * (w >> 1) + (l >> 4), cl = 32 * (w & 1) + (l & 15);
+  double cv[2][2][4];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) cv[u][v][e] = C[(int64_t)(r0 + rl + 16 * u + 4 * e) * ld + c0 + cl + 16 * v];
+  dbl4 acc[2][2];
+  mfma_nt(a, b, acc);
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int v = 0; v < 2; ++v)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) out[rl + 16 * u + 4 * e][cl + 16 * v] = cv[u][v][e] - acc[u][v][e];
 }
 
 // y = M v (transposed = false) or M^T v for a 64 x 64 global tile M with row
@@ -402,16 +431,18 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
   tile_mul_nt_lds(pa, pa, li);                          // L_ik = A_ik Linv^T
   if (jb != ib) tile_mul_nt_lds(pb, pb, li);            // L_jk
   if (jb == k + 1) store_tile(pa, L, ld, ib * NB, k * NB);
-  tile_sub_nt_global(A, ld, ib * NB, jb * NB, pa, jb != ib ? pb : pa);
   if (ib == k + 1 && jb == k + 1) {
-    // look-ahead: factor the next diagonal tile now that it is final
-    __threadfence_block();
+    // look-ahead: the next diagonal tile is final after this update, so it is
+    // formed in LDS (pb is free: its products used pa) and factored there
+    tile_sub_nt_to_lds(pb, A, ld, ib * NB, jb * NB, pa, pa);
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
-    tile_factor_inverse(A + (int64_t)ib * NB * ld + ib * NB, ld, L + (int64_t)ib * NB * ld + ib * NB, ld,
-                        Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB, &fail, pa, pb, fs);
+    tile_factor_inverse_lds(L + (int64_t)ib * NB * ld + ib * NB, ld, Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB,
+                            &fail, pb, pa, fs);
     __syncthreads();
     if (threadIdx.x == 0 && fail) s.chol_fail = 1;
+  } else {
+    tile_sub_nt_global(A, ld, ib * NB, jb * NB, pa, jb != ib ? pb : pa);
   }
 }
 
