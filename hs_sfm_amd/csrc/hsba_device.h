@@ -71,6 +71,9 @@ __host__ __device__ constexpr int z_stride(int e, int tbytes) {
   return (3 * e * tbytes + 127) / 128 * (128 / tbytes);
 }
 
+// Keys per image chunk of the image-block reduction.
+constexpr int kImgChunk = 512;
+
 // One incidence of a camera pair: the keys a (first image) and b (second image)
 // of one point, as a << 32 | b.  64 bits, so neither the track length nor the
 // number of keys of a problem is limited by the code.
@@ -110,6 +113,9 @@ struct Dev {
   T *V, *gp, *Linv, *q;
   // images
   const int32_t *img_ptr, *img_keys, *img_win, *img_cam, *img_row;
+  const int32_t *ich_img, *ich_begin, *ich_end, *img_ch_ptr;  // image-key chunks
+  int32_t nich, pad_ich_;
+  T *ipart;               // [chunks][E(E+1)/2 + E] partial image blocks
   double *imgs, *imgs_new;
   T *Dimg, *gr, *rp;
   // cameras
@@ -258,21 +264,32 @@ __device__ __forceinline__ void key_linearize(const double X[3], const double im
   }
   if (D > 0) {
     // camera block: [k1 k2 k3 d1 d2] through the intrinsic layer (:526-544),
-    // [f skew px py pr] from IntrinsicIntrinsicDerivation (:546-557)
-    double f0[kFull], f1[kFull];
-    const double nd0[5] = {rr2 * n0, rr4 * n0, rr6 * n0, 2.0 * n0 * n1, rr2 + 2.0 * n0 * n0};
-    const double nd1[5] = {rr2 * n1, rr4 * n1, rr6 * n1, rr2 + 2.0 * n1 * n1, 2.0 * n0 * n1};
-#pragma unroll
-    for (int j = 0; j < 5; ++j) {
-      f0[j] = F * nd0[j] + SK * nd1[j];
-      f1[j] = F * PR * nd1[j];
-    }
-    f0[5] = m0; f0[6] = m1; f0[7] = 1.0; f0[8] = 0.0; f0[9] = 0.0;
-    f1[5] = PR * m1; f1[6] = 0.0; f1[7] = 0.0; f1[8] = 1.0; f1[9] = F * m1;
+    // [f skew px py pr] from IntrinsicIntrinsicDerivation (:546-557).  Only
+    // the D free columns are formed; the column index is the same for every
+    // thread, so the switch is a uniform branch (a table indexed by it would
+    // be placed in scratch memory).
 #pragma unroll
     for (int q2 = 0; q2 < (D > 0 ? D : 1); ++q2) {
-      Jc[0][q2] = sel10(f0, fidx[q2]);
-      Jc[1][q2] = sel10(f1, fidx[q2]);
+      double nd0 = 0.0, nd1 = 0.0, c0 = 0.0, c1 = 0.0;
+      bool distortion = true;
+      switch (fidx[q2]) {
+        case IK1: nd0 = rr2 * n0; nd1 = rr2 * n1; break;
+        case IK2: nd0 = rr4 * n0; nd1 = rr4 * n1; break;
+        case IK3: nd0 = rr6 * n0; nd1 = rr6 * n1; break;
+        case ID1: nd0 = 2.0 * n0 * n1; nd1 = rr2 + 2.0 * n1 * n1; break;
+        case ID2: nd0 = rr2 + 2.0 * n0 * n0; nd1 = 2.0 * n0 * n1; break;
+        case IF: distortion = false; c0 = m0; c1 = PR * m1; break;
+        case ISKEW: distortion = false; c0 = m1; c1 = 0.0; break;
+        case IPX: distortion = false; c0 = 1.0; c1 = 0.0; break;
+        case IPY: distortion = false; c0 = 0.0; c1 = 1.0; break;
+        default: distortion = false; c0 = 0.0; c1 = F * m1; break;  // IPR
+      }
+      if (distortion) {
+        c0 = F * nd0 + SK * nd1;
+        c1 = F * PR * nd1;
+      }
+      Jc[0][q2] = c0;
+      Jc[1][q2] = c1;
     }
   }
 }

@@ -170,6 +170,17 @@ struct Engine final : EngineBase {
     A(d.pv, L.rows_total); A(d.pr, L.rows_total); A(d.pz, L.rows_total); A(d.pp, L.rows_total);
     A(d.pAp, L.rows_total); A(d.Minv, (int64_t)E * E * L.I);
     A(d.tpd, 3 * L.P); A(d.pcg, L.nwin); A(d.pdot, 3 * L.I);
+    d.nich = (int32_t)L.ich_img.size();
+    {
+      int32_t *ich_img, *ich_begin, *ich_end, *img_ch_ptr;
+      A(ich_img, L.ich_img.size()); A(ich_begin, L.ich_begin.size()); A(ich_end, L.ich_end.size());
+      A(img_ch_ptr, L.img_ch_ptr.size());
+      A(d.ipart, (int64_t)(E * (E + 1) / 2 + E) * std::max<int64_t>(1, L.ich_img.size()));
+      if ((rc = upload(ich_img, L.ich_img, st)) || (rc = upload(ich_begin, L.ich_begin, st)) ||
+          (rc = upload(ich_end, L.ich_end, st)) || (rc = upload(img_ch_ptr, L.img_ch_ptr, st)))
+        return rc;
+      d.ich_img = ich_img; d.ich_begin = ich_begin; d.ich_end = ich_end; d.img_ch_ptr = img_ch_ptr;
+    }
     A(small_list_dev, (int64_t)L.small_list.size());
 #undef A
     if ((rc = upload(key_img, L.key_img, st)) || (rc = upload(key_pt, L.key_pt, st)) ||

@@ -31,6 +31,9 @@ struct Layout {
   std::vector<int32_t> key_img, key_pt, key_orig;
   std::vector<int32_t> pt_ptr, pt_win;
   std::vector<int32_t> img_ptr, img_keys, img_win, img_cam, img_row;
+  // image-key chunks of at most kImgChunk keys: chunk -> image, begin / end in
+  // img_keys; img_ch_ptr[i] .. img_ch_ptr[i+1] are image i's chunks in order
+  std::vector<int32_t> ich_img, ich_begin, ich_end, img_ch_ptr;
   std::vector<int32_t> key_ipos, ikey_pt;  // PCG: key -> image-major row, row -> point
   std::vector<int32_t> cam_ptr, cam_imgs, cam_win, cam_row;
   std::vector<int64_t> pair_ptr;

@@ -116,16 +116,19 @@ __global__ void __launch_bounds__(256) k_linearize(Dev<T> d, int glog2) {
 }
 
 // ------------------------------------------------------------ image blocks
-// One workgroup per image: D_i = sum Jr^T L Jr (E x E, upper triangle) and
-// gr_i = sum Jr^T L r over the image's keys (image-major list), recomputing the
-// Jacobian instead of reading a stored one.  Replaces
+// D_i = sum Jr^T L Jr (E x E, upper triangle) and gr_i = sum Jr^T L r over the
+// image's keys (image-major list), recomputing the Jacobian instead of reading
+// a stored one.  One workgroup per chunk of at most kImgChunk of an image's
+// keys writes the chunk's partial sums; k_image_combine adds an image's chunks
+// in chunk order (deterministic).  Replaces
 // camera_shared_normal_equation_builder.hpp:221-248, 288-307, 392-415.
 template <typename T, int IMG, int D>
 __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
   constexpr int E = IMG + D;
   constexpr int NU = E * (E + 1) / 2;
   constexpr int NV = NU + E;
-  const int im = blockIdx.x;
+  const int ch = blockIdx.x;
+  const int im = d.ich_img[ch];
   const WinState &s = d.ws[d.img_win[im]];
   if (!(s.active && s.need_lin)) return;
   T acc[NV > 0 ? NV : 1];
@@ -135,7 +138,7 @@ __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
 #pragma unroll
   for (int j = 0; j < 6; ++j) img[j] = d.imgs[6 * im + j];
   load_cam(d, d.img_cam[im], false, cam);
-  const int kb = d.img_ptr[im], ke = d.img_ptr[im + 1];
+  const int kb = d.ich_begin[ch], ke = d.ich_end[ch];
   for (int t = kb + threadIdx.x; t < ke; t += blockDim.x) {
     const int k = d.img_keys[t];
     const int p = d.key_pt[k];
@@ -161,20 +164,23 @@ __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
         for (int m = 0; m < (D > 0 ? D : 1); ++m) Jc[r][m] *= sw;
       }
     }
-    double jr[2][E > 0 ? E : 1];
+    // products in the storage type (FP32 in mixed precision: the sums are
+    // FP32 anyway, and the narrower operands keep the kernel out of scratch)
+    T jr[2][E > 0 ? E : 1];
 #pragma unroll
     for (int j = 0; j < E; ++j) {
-      jr[0][j] = j < IMG ? Ji[0][j] : Jc[0][j - IMG];
-      jr[1][j] = j < IMG ? Ji[1][j] : Jc[1][j - IMG];
+      jr[0][j] = T(j < IMG ? Ji[0][j] : Jc[0][j - IMG]);
+      jr[1][j] = T(j < IMG ? Ji[1][j] : Jc[1][j - IMG]);
     }
+    const T r0 = T(res[0]), r1 = T(res[1]);
     int u = 0;
 #pragma unroll
     for (int a = 0; a < E; ++a) {
 #pragma unroll
-      for (int b = a; b < E; ++b) acc[u++] += T(jr[0][a] * jr[0][b] + jr[1][a] * jr[1][b]);
+      for (int b = a; b < E; ++b) acc[u++] += jr[0][a] * jr[0][b] + jr[1][a] * jr[1][b];
     }
 #pragma unroll
-    for (int a = 0; a < E; ++a) acc[NU + a] += T(jr[0][a] * res[0] + jr[1][a] * res[1]);
+    for (int a = 0; a < E; ++a) acc[NU + a] += jr[0][a] * r0 + jr[1][a] * r1;
   }
   // wave butterfly then fixed-order sum of the two waves
   for (int m = 1; m < 64; m <<= 1) {
@@ -188,18 +194,32 @@ __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
     for (int j = 0; j < NV; ++j) red[wave][j] = acc[j];
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < NV; j += blockDim.x) {
-    const T v = red[0][j] + red[1][j];
-    if (j < NU) {
-      // unpack upper-triangle index j -> (a, b) and write both halves
-      int a = 0, rem = j;
-      while (rem >= E - a) { rem -= E - a; ++a; }
-      const int b = a + rem;
-      d.Dimg[(int64_t)im * E * E + a * E + b] = v;
-      d.Dimg[(int64_t)im * E * E + b * E + a] = v;
-    } else {
-      d.gr[(int64_t)im * E + (j - NU)] = v;
-    }
+  for (int j = threadIdx.x; j < NV; j += blockDim.x) d.ipart[(int64_t)ch * NV + j] = red[0][j] + red[1][j];
+}
+
+// Sum of an image's chunk partials in chunk order into D_i (both halves) and
+// gr_i; one thread per (image, entry).
+template <typename T, int IMG, int D>
+__global__ void k_image_combine(Dev<T> d) {
+  constexpr int E = IMG + D;
+  constexpr int NU = E * (E + 1) / 2;
+  constexpr int NV = NU + E;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (NV == 0 || gid >= (int64_t)d.I * NV) return;
+  const int im = (int)(gid / NV), j = (int)(gid - (int64_t)im * NV);
+  const WinState &s = d.ws[d.img_win[im]];
+  if (!(s.active && s.need_lin)) return;
+  T v = 0;
+  for (int ch = d.img_ch_ptr[im]; ch < d.img_ch_ptr[im + 1]; ++ch) v += d.ipart[(int64_t)ch * NV + j];
+  if (j < NU) {
+    // unpack upper-triangle index j -> (a, b) and write both halves
+    int a = 0, rem = j;
+    while (rem >= E - a) { rem -= E - a; ++a; }
+    const int b = a + rem;
+    d.Dimg[(int64_t)im * E * E + a * E + b] = v;
+    d.Dimg[(int64_t)im * E * E + b * E + a] = v;
+  } else {
+    d.gr[(int64_t)im * E + (j - NU)] = v;
   }
 }
 
@@ -208,7 +228,7 @@ __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
 // in image order (the reference accumulates U_c per key in key order,
 // camera_shared_normal_equation_builder.hpp:235-248, 403-415).
 template <typename T, int IMG, int D>
-__global__ void k_camera_reduce(Dev<T> d) {
+__global__ void __launch_bounds__(128) k_camera_reduce(Dev<T> d) {
   constexpr int E = IMG + D;
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (D == 0 || c >= d.C) return;
@@ -1052,7 +1072,8 @@ __global__ void __launch_bounds__(256) k_point_cov(Dev<T> d, const double *Sinv,
 // Per-key residuals and Jacobian blocks in the caller's key order (FP64),
 // used by hsba_evaluate / hsba_jacobian.
 template <typename T, int IMG, int D>
-__global__ void k_export_keys(Dev<T> d, double *y, double *jp, double *ji, double *jc, int want_j) {
+__global__ void __launch_bounds__(256) k_export_keys(Dev<T> d, double *y, double *jp, double *ji, double *jc,
+                                                     int want_j) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.N) return;
   const int p = d.key_pt[k], im = d.key_img[k];
@@ -1100,7 +1121,7 @@ __device__ __forceinline__ int sup_row(const int *img_row, const int *cam_row, i
 // Preconditioner blocks M_i = [D_i with the camera block of U_c] + mu I - P_ii,
 // factored M_i = L L^T in FP64 registers (one thread per image).
 template <typename T, int IMG, int D>
-__global__ void k_pcg_precond(Dev<T> d) {
+__global__ void __launch_bounds__(64) k_pcg_precond(Dev<T> d) {
   constexpr int E = IMG + D;
   const int im = blockIdx.x * blockDim.x + threadIdx.x;
   if (im >= d.I) return;
@@ -1149,7 +1170,7 @@ __global__ void k_pcg_precond(Dev<T> d) {
 //   init = 0:  x += alpha p, r -= alpha Ap, z = M^-1 r
 // and the block's partials of r.z and r.r for the window's dot product.
 template <typename T, int IMG, int D>
-__global__ void k_pcg_apply(Dev<T> d, int init) {
+__global__ void __launch_bounds__(64) k_pcg_apply(Dev<T> d, int init) {
   constexpr int E = IMG + D;
   const int im = blockIdx.x * blockDim.x + threadIdx.x;
   if (im >= d.I) return;
@@ -1402,7 +1423,12 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void image_reduce(const Dev<T> &d, hipStream_t st) override {
     if (E == 0 || d.I == 0) return;
-    hipLaunchKernelGGL((k_image_reduce<T, IMG, D>), dim3(d.I), dim3(128), 0, st, d);
+    if (d.nich > 0) hipLaunchKernelGGL((k_image_reduce<T, IMG, D>), dim3(d.nich), dim3(128), 0, st, d);
+    {
+      constexpr int NV = E * (E + 1) / 2 + E;
+      const int64_t n = (int64_t)d.I * NV;
+      if (n > 0) hipLaunchKernelGGL((k_image_combine<T, IMG, D>), dim3((n + 255) / 256), dim3(256), 0, st, d);
+    }
   }
   void camera_reduce(const Dev<T> &d, hipStream_t st) override {
     if (D == 0 || d.C == 0) return;

@@ -39,7 +39,9 @@ int fail(std::string *why, const std::string &msg, int code = HSBA_ERR_INVALID) 
   return code;
 }
 
-constexpr int kChunk = 8192;
+// Entries per reduction chunk (one workgroup of 256 threads each): small
+// enough that the cost and step reductions of a 1M-key window fill the GPU.
+constexpr int kChunk = 1024;
 
 // Host threads for the O(keys x track) passes: one per 64k keys, at most 16.
 int layout_threads(int64_t keys) {
@@ -417,6 +419,16 @@ int build_layout(const hsba_problem_desc *descs, int64_t count, Layout &L, std::
     std::vector<int32_t> fill(L.img_ptr.begin(), L.img_ptr.end() - 1);
     for (int64_t k = 0; k < L.N; ++k) L.img_keys[fill[L.key_img[k]]++] = (int32_t)k;
   }
+  L.img_ch_ptr.assign(L.I + 1, 0);
+  L.ich_img.clear(); L.ich_begin.clear(); L.ich_end.clear();
+  for (int64_t i = 0; i < L.I; ++i) {
+    for (int32_t b = L.img_ptr[i]; b < L.img_ptr[i + 1]; b += kImgChunk) {
+      L.ich_img.push_back((int32_t)i);
+      L.ich_begin.push_back(b);
+      L.ich_end.push_back(std::min<int32_t>(b + kImgChunk, L.img_ptr[i + 1]));
+    }
+    L.img_ch_ptr[i + 1] = (int32_t)L.ich_img.size();
+  }
   if (pcg) {
     L.key_ipos.resize(L.N);
     L.ikey_pt.resize(L.N);
@@ -457,6 +469,7 @@ int64_t layout_device_bytes(const Layout &L, int precision) {
   b += (L.S_total + L.dinv_total) * 8 + L.rows_total * (8 * 8 + 8); // reduced systems
   b += (int64_t)L.contrib.size() * sizeof(Contrib) + (int64_t)L.blk_ptr.size() * 8;
   b += (int64_t)L.chunk_kind.size() * (16 + 32) + (int64_t)L.nwin * (sizeof(WinInfo) + sizeof(WinState));
+  b += (int64_t)L.ich_img.size() * (12 + t * (E * (E + 1) / 2 + E)) + (L.I + 1) * 4;  // image-key chunks
   return b;
 }
 
@@ -508,7 +521,8 @@ int64_t estimate_device_bytes(const hsba_problem_desc &ds, int precision, bool p
   const int64_t n = IMG * I + D * C, ld = (n + 63) / 64 * 64, nb = ld / 64;
   const int64_t S = pcg ? 0 : ld * ld, dinv = pcg ? 0 : nb * 64 * 64;
   const int64_t nblk = pcg ? 0 : I + C;
-  const int64_t chunks = (N + 8191) / 8192 + (6 * I + 8191) / 8192 + (D * C + 8191) / 8192 + 3;
+  const int64_t chunks = (N + kChunk - 1) / kChunk + (3 * P + kChunk - 1) / kChunk + (6 * I + kChunk - 1) / kChunk +
+                         (D * C + kChunk - 1) / kChunk + 4;
   int64_t b = 0;
   b += N * (4 * 3 + 16) + (has_cov ? N * 24 : 0);
   b += N * (3 * E * t + z_stride((int)E, (int)t) * t) + N * 4;     // W, Z, image key lists
@@ -520,6 +534,7 @@ int64_t estimate_device_bytes(const hsba_problem_desc &ds, int precision, bool p
   b += (2 * S + dinv) * 8 + ld * (8 * 10 + 8);                       // S, factor, reduced vectors
   b += (4 * npairs + 2 * nblk) * (int64_t)sizeof(Contrib) + (nblk * (nblk + 1) / 2 + 1) * 8;
   b += chunks * (16 + 32) + 4096;
+  b += (N / kImgChunk + I) * (12 + t * (E * (E + 1) / 2 + E)) + (I + 1) * 4;  // image-key chunks
   // allocation granules (about 70 buffers) and slack
   return b + b / 8 + 70 * 256 + (1 << 20);
 }
