@@ -82,10 +82,14 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int PB = 16;  // panel width inside a 64 x 64 tile
 
+struct alignas(16) DPair {
+  double x, y;
+};
+
 struct FactorScratch {
-  double col[2][2][PB];   // columns j, j+1 of the diagonal block being eliminated
-  double xrow[2][2][PB];  // rows j, j+1 of its inverse
-  double d[PB];           // pivots
+  DPair col[2][PB];   // {a[i][j], a[i][j+1]}: columns j, j+1 of the diagonal block being eliminated
+  DPair xrow[2][PB];    // {x[j][c], x[j+1][c]}: rows j, j+1 of its inverse
+  double d[PB];         // pivots
 };
 
 // The whole workgroup (256 threads, one element each): factor the 16 x 16
@@ -113,15 +117,17 @@ __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int
 #pragma unroll
   for (int j = 0; j < PB; j += 2) {
     const int b = (j >> 1) & 1;
-    if (c == j) fs.col[b][0][i] = a;
-    if (c == j + 1) fs.col[b][1][i] = a;
-    if (i == j) fs.xrow[b][0][c] = x;
-    if (i == j + 1) fs.xrow[b][1][c] = x;
+    if (c == j) fs.col[b][i].x = a;
+    if (c == j + 1) fs.col[b][i].y = a;
+    if (i == j) fs.xrow[b][c].x = x;
+    if (i == j + 1) fs.xrow[b][c].y = x;
     __syncthreads();
-    const double aij = fs.col[b][0][i], ai1 = fs.col[b][1][i];
-    const double acj = fs.col[b][0][c], ac1 = fs.col[b][1][c];
-    const double d0 = fs.col[b][0][j], e = fs.col[b][0][j + 1], d1r = fs.col[b][1][j + 1];
-    const double x0c = fs.xrow[b][0][c], x1c = fs.xrow[b][1][c];
+    // pairs: one 16-byte LDS read each
+    const DPair ci = fs.col[b][i], cc = fs.col[b][c], p0 = fs.col[b][j], p1 = fs.col[b][j + 1];
+    const DPair xr = fs.xrow[b][c];
+    const double aij = ci.x, ai1 = ci.y, acj = cc.x, ac1 = cc.y;
+    const double d0 = p0.x, e = p1.x, d1r = p1.y;
+    const double x0c = xr.x, x1c = xr.y;
     const double det = d0 * d1r - e * e;
     const bool ok = d0 > 0.0 && det > 0.0;
     const double idet = ok ? recip_nr(det) : 0.0;
@@ -146,23 +152,61 @@ __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int
   wx[c0 + i][c0 + c] = (c <= i) ? x / sqrt(di) : 0.0;
 }
 
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// 16 x 16 x 16 block products of one wave on the matrix cores
+// (v_mfma_f64_16x16x4_f64), operands read from LDS tiles by block offset:
+//   blk_nt: acc += A B^T,  blk_nn: acc += A B,
+// A the 16 x 16 block of a at (ar, ac), B that of b at (br, bc).  Operand
+// lanes: A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k; the result
+// D[i][j] is in lane j + 16 (i % 4), element i / 4 (blk_put / blk_sub).
+__device__ __forceinline__ dbl4 blk_nt(const double (*a)[LDT], int ar, int ac, const double (*b)[LDT], int br,
+                                       int bc, dbl4 acc) {
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int k = 0; k < 16; k += 4)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ar + li][ac + k + lk], b[br + li][bc + k + lk], acc, 0, 0, 0);
+  return acc;
+}
+__device__ __forceinline__ dbl4 blk_nn(const double (*a)[LDT], int ar, int ac, const double (*b)[LDT], int br,
+                                       int bc, dbl4 acc) {
+  const int l = threadIdx.x & 63, li = l & 15, lk = l >> 4;
+#pragma unroll
+  for (int k = 0; k < 16; k += 4)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ar + li][ac + k + lk], b[br + k + lk][bc + li], acc, 0, 0, 0);
+  return acc;
+}
+// out block at (r, c) = s * acc  (s = +1 or -1)
+__device__ __forceinline__ void blk_put(double (*out)[LDT], int r, int c, dbl4 acc, double s) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) out[r + (l >> 4) + 4 * e][c + (l & 15)] = s * acc[e];
+}
+// out block at (r, c) -= acc
+__device__ __forceinline__ void blk_sub(double (*out)[LDT], int r, int c, dbl4 acc) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) out[r + (l >> 4) + 4 * e][c + (l & 15)] -= acc[e];
+}
+
 // Factor a 64 x 64 SPD tile (lower triangle of src, row pitch lds) and invert
 // the factor: writes L (zero upper) to dstL and L^-1 (zero upper) to dstI; a
 // non-positive pivot sets *fail.  wa / wx are two LDS tiles of scratch.
 // Right-looking blocked Cholesky with 16-wide panels p = 0..3:
-//   L_pp = chol(A_pp), M_p = L_pp^-1      (one wave, diag16)
-//   L_rp = A_rp M_p^T                     (rows below the panel)
-//   A_ij -= L_ip L_jp^T                   (trailing lower triangle)
-// then the off-diagonal blocks of L^-1 by block diagonals s = i - j = 1..3:
-//   T_ij = sum_{k=j}^{i-1} L_ik X_kj,  X_ij = -M_i T_ij
-// (T is staged in the unused upper triangle of wa).  256 threads.
-// tile_factor_inverse_lds: the same with the tile already in wa (synchronised).
+//   L_pp = chol(A_pp), M_p = L_pp^-1      (whole workgroup, diag16)
+//   L_rp = A_rp M_p^T                     (rows below the panel; wave w: block row p+1+w)
+//   A_ij -= L_ip L_jp^T                   (trailing lower triangle, blocks dealt to the waves)
+// then the off-diagonal blocks of L^-1 by block diagonals s = i - j = 1..3,
+// wave q forming block (q + s, q):
+//   T = sum_{k=j}^{i-1} L_ik X_kj,  X_ij = -M_i T
+// (T is staged in the unused upper block (j, i) of wa and read back by the
+// same wave).  Every block product is a 16 x 16 x 16 matrix-core product.
+// 256 threads.  tile_factor_inverse_lds: the tile already in wa (synchronised).
 __device__ __forceinline__ void tile_factor_inverse_lds(double *dstL, int64_t ldl, double *dstI, int64_t ldi,
                                                         int *fail, double (*wa)[LDT], double (*wx)[LDT],
                                                         FactorScratch &fs) {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, w = tid >> 6;
   bool bad = false;
-  const int rr = tid >> 4, cc = tid & 15;
 #pragma unroll
   for (int p = 0; p < 4; ++p) {
     const int c0 = PB * p;
@@ -170,71 +214,36 @@ __device__ __forceinline__ void tile_factor_inverse_lds(double *dstL, int64_t ld
     __syncthreads();
     const int nu = 3 - p;  // 16-row blocks below the panel
     if (nu > 0) {
-      // panel: L_rp = A_rp M_p^T (M_p zero above its diagonal)
-      double out[3];
-#pragma unroll
-      for (int u = 0; u < 3; ++u) {
-        if (u < nu) {
-          const int r = c0 + PB + rr + 16 * u;
-          double s = 0.0;
-#pragma unroll
-          for (int t = 0; t < PB; ++t) s += wa[r][c0 + t] * wx[c0 + cc][c0 + t];
-          out[u] = s;
-        }
+      // panel: L_rp = A_rp M_p^T, in place (the wave's operand reads precede its writes)
+      if (w < nu) {
+        const int r = c0 + PB * (1 + w);
+        const dbl4 acc = blk_nt(wa, r, c0, wx, c0, c0, dbl4{0.0, 0.0, 0.0, 0.0});
+        blk_put(wa, r, c0, acc, 1.0);
       }
-      wave_sync();  // the 16 lanes reading row r are this wave's
-#pragma unroll
-      for (int u = 0; u < 3; ++u)
-        if (u < nu) wa[c0 + PB + rr + 16 * u][c0 + cc] = out[u];
       __syncthreads();
-      // trailing lower triangle: rows base + rr + 16u, columns base + cc + 16v
-      const int base = c0 + PB;
-      double acc[3][3];
-#pragma unroll
-      for (int u = 0; u < 3; ++u)
-#pragma unroll
-        for (int v = 0; v < 3; ++v) acc[u][v] = 0.0;
-#pragma unroll 4
-      for (int t = 0; t < PB; ++t) {
-        double li[3], lj[3];
-#pragma unroll
-        for (int u = 0; u < 3; ++u) {
-          li[u] = u < nu ? wa[base + rr + 16 * u][c0 + t] : 0.0;
-          lj[u] = u < nu ? wa[base + cc + 16 * u][c0 + t] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 3; ++u)
-#pragma unroll
-          for (int v = 0; v <= u; ++v) acc[u][v] += li[u] * lj[v];
+      // trailing blocks (u, v), v <= u < nu, numbered row by row
+      const int nblk = nu * (nu + 1) / 2;
+      for (int bk = w; bk < nblk; bk += 4) {
+        const int u = bk >= 3 ? 2 : (bk >= 1 ? 1 : 0);
+        const int v = bk - u * (u + 1) / 2;
+        const int ri = c0 + PB * (1 + u), cj = c0 + PB * (1 + v);
+        const dbl4 acc = blk_nt(wa, ri, c0, wa, cj, c0, dbl4{0.0, 0.0, 0.0, 0.0});
+        blk_sub(wa, ri, cj, acc);
       }
-#pragma unroll
-      for (int u = 0; u < 3; ++u)
-#pragma unroll
-        for (int v = 0; v <= u; ++v)
-          if (u < nu && (u > v || rr >= cc)) wa[base + rr + 16 * u][base + cc + 16 * v] -= acc[u][v];
       __syncthreads();
     }
   }
   // off-diagonal blocks of X = L^-1, one block diagonal at a time
 #pragma unroll
   for (int s = 1; s < 4; ++s) {
-#pragma unroll
-    for (int q = 0; q + s < 4; ++q) {
-      const int bj = q, bi = q + s;
-      double t_acc = 0.0;
-      for (int k = bj; k < bi; ++k)
-#pragma unroll 4
-        for (int t = 0; t < PB; ++t) t_acc += wa[PB * bi + rr][PB * k + t] * wx[PB * k + t][PB * bj + cc];
-      wa[PB * bj + rr][PB * bi + cc] = t_acc;  // upper block (bj, bi) is free
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q + s < 4; ++q) {
-      const int bj = q, bi = q + s;
-      double v = 0.0;
-#pragma unroll 4
-      for (int t = 0; t < PB; ++t) v += wx[PB * bi + rr][PB * bi + t] * wa[PB * bj + t][PB * bi + cc];
-      wx[PB * bi + rr][PB * bj + cc] = -v;
+    if (w + s < 4) {
+      const int bj = w, bi = w + s;
+      dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+      for (int k = bj; k < bi; ++k) acc = blk_nn(wa, PB * bi, PB * k, wx, PB * k, PB * bj, acc);
+      blk_put(wa, PB * bj, PB * bi, acc, 1.0);  // upper block (bj, bi) is free
+      wave_sync();
+      const dbl4 xv = blk_nn(wx, PB * bi, PB * bi, wa, PB * bj, PB * bi, dbl4{0.0, 0.0, 0.0, 0.0});
+      blk_put(wx, PB * bi, PB * bj, xv, -1.0);
     }
     __syncthreads();
   }
@@ -260,7 +269,6 @@ __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t l
 // blocks of 16 x 16.  Operand lanes: A[i][k] from lane i + 16 k, B[k][j] from
 // lane j + 16 k; the result D[i][j] lands in lane j + 16 (i % 4), element
 // i / 4.  acc = a * b^T with a, b in LDS.
-typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void mfma_nt(const double (*a)[LDT], const double (*b)[LDT], dbl4 acc[2][2]) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -353,10 +361,10 @@ __device__ __forceinline__ void tile_gemv(const double *M, int64_t ld, const dou
     const int k = sl * 16 + q;
     mv[q] = transposed ? M[(int64_t)k * ld + r] : M[(int64_t)r * ld + k];
   }
-  double acc = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-  for (int q = 0; q < 16; ++q) acc += mv[q] * v[sl * 16 + q];
-  scratch[sl][r] = acc;
+  for (int q = 0; q < 16; ++q) acc[q & 3] += mv[q] * v[sl * 16 + q];
+  scratch[sl][r] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (threadIdx.x < NB) out[threadIdx.x] = scratch[0][threadIdx.x] + scratch[1][threadIdx.x] +
                                            scratch[2][threadIdx.x] + scratch[3][threadIdx.x];
