@@ -328,18 +328,10 @@ __device__ __forceinline__ void tile_sub_nt_global(double *C, int64_t ld, int r0
       for (int e = 0; e < 4; ++e) C[(int64_t)(rr + 16 * u + 4 * e) * ld + cc + 16 * v] = cv[u][v][e] - acc[u][v][e];
 }
 
-// out (LDS) = C (global 64 x 64 at r0, c0) - a b^T; out must not alias a or b.
-__device__ __forceinline__ void tile_sub_nt_to_lds(double (*out)[LDT], const double *C, int64_t ld, int r0, int c0,
-                                                   const double (*a)[LDT], const double (*b)[LDT]) {
+// c (LDS) -= a b^T; c must not alias a or b.
+__device__ __forceinline__ void tile_sub_nt_lds(double (*c)[LDT], const double (*a)[LDT], const double (*b)[LDT]) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int rl = 32 * (w >> 1) + (l >> 4), cl = 32 * (w & 1) + (l & 15);
-  double cv[2][2][4];
-#pragma unroll
-  for (int u = 0; u < 2; ++u)
-#pragma unroll
-    for (int v = 0; v < 2; ++v)
-#pragma unroll
-      for (int e = 0; e < 4; ++e) cv[u][v][e] = C[(int64_t)(r0 + rl + 16 * u + 4 * e) * ld + c0 + cl + 16 * v];
   dbl4 acc[2][2];
   mfma_nt(a, b, acc);
 #pragma unroll
@@ -347,7 +339,7 @@ __device__ __forceinline__ void tile_sub_nt_to_lds(double (*out)[LDT], const dou
 #pragma unroll
     for (int v = 0; v < 2; ++v)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) out[rl + 16 * u + 4 * e][cl + 16 * v] = cv[u][v][e] - acc[u][v][e];
+      for (int e = 0; e < 4; ++e) c[rl + 16 * u + 4 * e][cl + 16 * v] -= acc[u][v][e];
 }
 
 // y = M v (transposed = false) or M^T v for a 64 x 64 global tile M with row
@@ -425,24 +417,27 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
   while (r * (r + 1) / 2 > t) --r;
   while ((r + 1) * (r + 2) / 2 <= t) ++r;
   const int ib = k + 1 + r, jb = k + 1 + (t - r * (r + 1) / 2);
+  // look-ahead: tile (k+1, k+1) is final after this update, so it is formed
+  // and factored in LDS by its workgroup
+  const bool look = ib == k + 1 && jb == k + 1;
   {
-    // all three tiles' loads in flight together
+    // all three tiles' loads in flight together (the look-ahead workgroup's
+    // third tile is the diagonal tile it updates)
     double v0[16], v1[16], v2[16];
     fetch_tile(v0, Li, NB, 0, 0);
     fetch_tile(v1, A, ld, ib * NB, k * NB);
     if (jb != ib) fetch_tile(v2, A, ld, jb * NB, k * NB);
+    else if (look) fetch_tile(v2, A, ld, ib * NB, jb * NB);
     put_tile(li, v0);
     put_tile(pa, v1);
-    if (jb != ib) put_tile(pb, v2);
+    if (jb != ib || look) put_tile(pb, v2);
   }
   __syncthreads();
   tile_mul_nt_lds(pa, pa, li);                          // L_ik = A_ik Linv^T
   if (jb != ib) tile_mul_nt_lds(pb, pb, li);            // L_jk
   if (jb == k + 1) store_tile(pa, L, ld, ib * NB, k * NB);
-  if (ib == k + 1 && jb == k + 1) {
-    // look-ahead: the next diagonal tile is final after this update, so it is
-    // formed in LDS (pb is free: its products used pa) and factored there
-    tile_sub_nt_to_lds(pb, A, ld, ib * NB, jb * NB, pa, pa);
+  if (look) {
+    tile_sub_nt_lds(pb, pa, pa);
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
     tile_factor_inverse_lds(L + (int64_t)ib * NB * ld + ib * NB, ld, Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB,
@@ -454,49 +449,111 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
   }
 }
 
+// Products of a 64 x 64 global tile with a 64-vector by 1024 threads: thread
+// t holds the four entries M'[r][4 s + q] (q = 0..3) of M' = M^T
+// (transposed) or M, r = t % 64, s = t / 64 (16 slices of four columns).
+constexpr int BT = 1024;
+__device__ __forceinline__ void fetch4(double m[4], const double *M, int64_t ld, bool transposed) {
+  const int r = threadIdx.x & 63, sl = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    m[q] = transposed ? M[(int64_t)(4 * sl + q) * ld + r] : M[(int64_t)r * ld + 4 * sl + q];
+}
+// this thread's share of sum_k M'[r][k] v[k] (v in LDS)
+__device__ __forceinline__ double part4(const double m[4], const double *v) {
+  const int sl = threadIdx.x >> 6;
+  return (m[0] * v[4 * sl] + m[1] * v[4 * sl + 1]) + (m[2] * v[4 * sl + 2] + m[3] * v[4 * sl + 3]);
+}
+// sum of the 16 slices' shares; the result is valid in threads 0..63 (row r = t)
+__device__ __forceinline__ double reduce16(double part, double (*red)[NB]) {
+  red[threadIdx.x >> 6][threadIdx.x & 63] = part;
+  __syncthreads();
+  double v = 0.0;
+  if (threadIdx.x < NB) {
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int sl = 0; sl < 16; ++sl) q[sl & 3] += red[sl][threadIdx.x];
+    v = (q[0] + q[1]) + (q[2] + q[3]);
+  }
+  return v;
+}
+
 // Backward solve of the block rows k_hi, k_hi - 1, ..., k_lo (at most four)
-// in one launch.  Every workgroup forms that short chain itself,
+// in one launch of 1024-thread workgroups.  Every workgroup forms that short
+// chain itself,
 //   v = y_k - sum_{m = k+1..k_hi} L_mk^T x_m   (y_k = Linv_k b_k for the last block),
 //   x_k = Linv_k^T v,
-// then workgroup ib < k_lo applies y_ib -= sum_{m = k_lo..k_hi} L_m,ib^T x_m.
-__global__ void __launch_bounds__(256) k_chol_back(WinInfo wi, WinState *ws, int w, const double *Lf,
-                                                   const double *Dinv, const double *b, double *y, double *x,
-                                                   int k_hi, int k_lo) {
+// with every tile of the chain fetched into registers before the first
+// product (the chain is then bound by its reductions, not by memory latency);
+// workgroup ib < k_lo then applies y_ib -= sum_{m = k_lo..k_hi} L_m,ib^T x_m.
+__global__ void __launch_bounds__(BT) k_chol_back(WinInfo wi, WinState *ws, int w, const double *Lf,
+                                                  const double *Dinv, const double *b, double *y, double *x,
+                                                  int k_hi, int k_lo) {
   __shared__ double xs[4][NB];
-  __shared__ double vsh[3][NB];
-  __shared__ double red[4][NB];
+  __shared__ double vs[NB];
+  __shared__ double red[16][NB];
   const WinState &s = ws[w];
   if (!chol_active(s)) return;
   const int64_t ld = wi.ld;
   const double *L = Lf + wi.s_off;
+  const double *Dw = Dinv + wi.dinv_off;
   const int t = threadIdx.x;
-  for (int k = k_hi; k >= k_lo; --k) {
-    const double *Li = Dinv + wi.dinv_off + (int64_t)k * NB * NB;
-    const bool last = k == wi.nb - 1;
-    if (t < NB) vsh[0][t] = last ? b[wi.r_off + k * NB + t] : y[wi.r_off + k * NB + t];
-    __syncthreads();
-    if (last) {
-      tile_gemv(Li, NB, vsh[0], vsh[1], false, red);
-      if (t < NB) vsh[0][t] = vsh[1][t];
+  const int nk = k_hi - k_lo + 1;  // 1..4; step a is block row k_hi - a
+  const bool last = k_hi == wi.nb - 1;
+  double di[4][4], lt[6][4], dl[4], yv[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    if (a < nk) fetch4(di[a], Dw + (int64_t)(k_hi - a) * NB * NB, NB, true);
+  // L_{k_hi - c, k_hi - a} for c < a, at lt[a (a - 1) / 2 + c]
+#pragma unroll
+  for (int a = 1; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < a; ++c)
+      if (a < nk) fetch4(lt[a * (a - 1) / 2 + c], L + (int64_t)(k_hi - c) * NB * ld + (k_hi - a) * NB, ld, true);
+  if (last) fetch4(dl, Dw + (int64_t)k_hi * NB * NB, NB, false);
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+    yv[a] = (t < NB && a < nk) ? ((last && a == 0) ? b[wi.r_off + k_hi * NB + t] : y[wi.r_off + (k_hi - a) * NB + t])
+                               : 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (a < nk) {
+      double v;
+      if (a == 0 && last) {
+        if (t < NB) vs[t] = yv[0];
+        __syncthreads();
+        v = reduce16(part4(dl, vs), red);  // Linv_k b_k
+      } else if (a == 0) {
+        v = yv[0];
+      } else {
+        double part = 0.0;
+#pragma unroll
+        for (int c = 0; c < a; ++c) part += part4(lt[a * (a - 1) / 2 + c], xs[c]);
+        v = yv[a] - reduce16(part, red);
+      }
+      __syncthreads();
+      if (t < NB) vs[t] = v;
+      __syncthreads();
+      const double xk = reduce16(part4(di[a], vs), red);
+      if (t < NB) xs[a][t] = xk;
       __syncthreads();
     }
-    for (int m = k_hi; m > k; --m) {
-      tile_gemv(L + (int64_t)m * NB * ld + k * NB, ld, xs[k_hi - m], vsh[2], true, red);
-      if (t < NB) vsh[0][t] -= vsh[2][t];
-      __syncthreads();
-    }
-    tile_gemv(Li, NB, vsh[0], xs[k_hi - k], true, red);  // x_k
   }
   if (blockIdx.x == 0 && t < NB)
-    for (int k = k_lo; k <= k_hi; ++k) x[wi.r_off + k * NB + t] = xs[k_hi - k][t];
+    for (int a = 0; a < nk; ++a) x[wi.r_off + (k_hi - a) * NB + t] = xs[a][t];
   if (k_lo == 0) return;
   const int ib = blockIdx.x;  // < k_lo
-  double acc = 0.0;
-  for (int m = k_lo; m <= k_hi; ++m) {
-    tile_gemv(L + (int64_t)m * NB * ld + ib * NB, ld, xs[k_hi - m], vsh[2], true, red);
-    if (t < NB) acc += vsh[2][t];
+  double part = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    if (a < nk) {
+      double m[4];
+      fetch4(m, L + (int64_t)(k_hi - a) * NB * ld + ib * NB, ld, true);
+      part += part4(m, xs[a]);
+    }
   }
-  if (t < NB) y[wi.r_off + ib * NB + t] -= acc;
+  const double u = reduce16(part, red);
+  if (t < NB) y[wi.r_off + ib * NB + t] -= u;
 }
 
 // ---------------------------------------------------------------- small path
@@ -744,7 +801,7 @@ void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int
     }
     for (int k_hi = wi.nb - 1; k_hi >= 0; k_hi -= 4) {
       const int k_lo = k_hi >= 3 ? k_hi - 3 : 0;
-      hipLaunchKernelGGL(k_chol_back, dim3(k_lo > 0 ? k_lo : 1), dim3(256), 0, st, wi, plan.ws, w, plan.Lf,
+      hipLaunchKernelGGL(k_chol_back, dim3(k_lo > 0 ? k_lo : 1), dim3(BT), 0, st, wi, plan.ws, w, plan.Lf,
                          plan.Dinv, plan.bwork, plan.ywork, plan.dred, k_hi, k_lo);
     }
   }
