@@ -82,32 +82,33 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int PB = 16;  // panel width inside a 64 x 64 tile
 
-struct alignas(16) DPair {
-  double x, y;
+struct alignas(32) DQuad {
+  double v[4];
 };
 
 struct FactorScratch {
-  DPair col[2][PB];   // {a[i][j], a[i][j+1]}: columns j, j+1 of the diagonal block being eliminated
-  DPair xrow[2][PB];    // {x[j][c], x[j+1][c]}: rows j, j+1 of its inverse
-  double d[PB];         // pivots
+  DQuad col[2][PB];   // a[i][j .. j+3]: the four columns of the pivot block being eliminated
+  DQuad xrow[2][PB];  // x[j .. j+3][c]: the four rows of the inverse at that block
+  double d[PB];       // pivots
 };
 
 // The whole workgroup (256 threads, one element each): factor the 16 x 16
 // diagonal block of wa at (c0, c0) (lower triangle, SPD) in place into its
 // Cholesky factor L_p, and write L_p^-1 (lower, zero upper) to the same block
-// of wx.  Elimination of [A | I] by 2 x 2 pivot blocks, one barrier per pair
-// of columns: columns j, j+1 and rows j, j+1 of x are published as they
-// stand, and every thread eliminates both at once with the pivot block
-// P = [[d0, e], [e, d1r]] (one reciprocal, of det P, on the critical path):
-//   [f_i, g_i] = [a[i][j], a[i][j+1]] P^-1
-//   a[i][c] -= f_i a[c][j] + g_i a[c][j+1]            (c > j+1, i >= c)
-//   x[i][:] -= f_i x[j][:] + g_i x[j+1][:]             (i > j+1)
-// which is the two single-column steps of LDL^T at once.  Column j keeps
-// Lt[i][j] d0; column j+1 and row j+1 of x take their single-step values
-//   a[i][j+1] = a[i][j+1] - a[i][j] e / d0,  d1 = d1r - e^2 / d0,
-//   x[j+1][:] -= (e / d0) x[j][:]
-// (off the critical path).  At the end a = Lt D and x = Lt^-1, so
-// L = Lt D^1/2 and L^-1 = D^-1/2 x.  Thread t owns element (t / 16, t % 16).
+// of wx.  Elimination of [A | I] by 4 x 4 pivot blocks, one barrier per
+// block: the block's four columns of a and four rows of x are published as
+// they stand; every thread forms P = L_P D_P L_P^T (unit lower L_P; four
+// reciprocals, computed redundantly in registers) and, by forward
+// substitution with L_P,
+//   y_i = a_iJ L_P^-T,  y_c = a_cJ L_P^-T,  w_c = L_P^-1 x_J[:, c],  z_i = y_i D_P^-1
+// (a_iJ: row i of the block's columns, x_J: the block's rows of x), then
+//   a[i][c] = y_i[c - j]               (c in the block, i >= c)
+//   a[i][c] -= z_i . y_c               (c beyond the block, i >= c)
+//   x[i][c] = w_c[i - j]               (i in the block)
+//   x[i][c] -= z_i . w_c               (i beyond the block)
+// which is four single-column steps of LDL^T at once.  At the end a = Lt D
+// and x = Lt^-1 (Lt unit lower), so L = Lt D^1/2 and L^-1 = D^-1/2 x.
+// Thread t owns element (t / 16, t % 16).
 __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int c0, FactorScratch &fs,
                                        bool &bad) {
   const int tid = threadIdx.x;
@@ -115,35 +116,51 @@ __device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int
   double a = (i >= c) ? wa[c0 + i][c0 + c] : 0.0;
   double x = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-  for (int j = 0; j < PB; j += 2) {
-    const int b = (j >> 1) & 1;
-    if (c == j) fs.col[b][i].x = a;
-    if (c == j + 1) fs.col[b][i].y = a;
-    if (i == j) fs.xrow[b][c].x = x;
-    if (i == j + 1) fs.xrow[b][c].y = x;
+  for (int j = 0; j < PB; j += 4) {
+    const int b = (j >> 2) & 1;
+    if (c >= j && c < j + 4) fs.col[b][i].v[c - j] = a;
+    if (i >= j && i < j + 4) fs.xrow[b][c].v[i - j] = x;
     __syncthreads();
-    // pairs: one 16-byte LDS read each
-    const DPair ci = fs.col[b][i], cc = fs.col[b][c], p0 = fs.col[b][j], p1 = fs.col[b][j + 1];
-    const DPair xr = fs.xrow[b][c];
-    const double aij = ci.x, ai1 = ci.y, acj = cc.x, ac1 = cc.y;
-    const double d0 = p0.x, e = p1.x, d1r = p1.y;
-    const double x0c = xr.x, x1c = xr.y;
-    const double det = d0 * d1r - e * e;
-    const bool ok = d0 > 0.0 && det > 0.0;
-    const double idet = ok ? recip_nr(det) : 0.0;
-    const double fi = (aij * d1r - ai1 * e) * idet, gi = (ai1 * d0 - aij * e) * idet;
-    const bool lower = i >= c;
-    const double a_upd = a - (fi * acj + gi * ac1);
-    const double x_upd = x - (fi * x0c + gi * x1c);
-    // single-step values of column j+1, pivot d1 and row j+1 of x
-    const double inv0 = d0 > 0.0 ? recip_nr(d0) : 0.0;
-    const double l = e * inv0;
-    const double d1 = det * inv0;
-    bad = bad || !ok;
-    if (tid == 0) { fs.d[j] = d0; fs.d[j + 1] = d1; }
-    a = (lower && c > j + 1) ? a_upd : a;
-    a = (lower && c == j + 1) ? (i == j + 1 ? d1 : ai1 - aij * l) : a;
-    x = (i > j + 1) ? x_upd : (i == j + 1 ? x1c - l * x0c : x);
+    const DQuad ai = fs.col[b][i], ac = fs.col[b][c], xr = fs.xrow[b][c];
+    const DQuad p1 = fs.col[b][j + 1], p2 = fs.col[b][j + 2], p3 = fs.col[b][j + 3];
+    const double d0 = fs.col[b][j].v[0];
+    // P = L_P D_P L_P^T (t_rs = l_rs d_s)
+    const double r0 = d0 > 0.0 ? recip_nr(d0) : 0.0;
+    const double l10 = p1.v[0] * r0, l20 = p2.v[0] * r0, l30 = p3.v[0] * r0;
+    const double d1 = p1.v[1] - l10 * p1.v[0];
+    const double r1 = d1 > 0.0 ? recip_nr(d1) : 0.0;
+    const double t21 = p2.v[1] - l20 * p1.v[0], t31 = p3.v[1] - l30 * p1.v[0];
+    const double l21 = t21 * r1, l31 = t31 * r1;
+    const double d2 = p2.v[2] - l20 * p2.v[0] - l21 * t21;
+    const double r2 = d2 > 0.0 ? recip_nr(d2) : 0.0;
+    const double t32 = p3.v[2] - l30 * p2.v[0] - l31 * t21;
+    const double l32 = t32 * r2;
+    const double d3 = p3.v[3] - l30 * p3.v[0] - l31 * t31 - l32 * t32;
+    const double r3 = d3 > 0.0 ? recip_nr(d3) : 0.0;
+    bad = bad || !(d0 > 0.0 && d1 > 0.0 && d2 > 0.0 && d3 > 0.0);
+    // forward substitutions with L_P
+    double yi[4], yc[4], wc[4];
+    yi[0] = ai.v[0];
+    yi[1] = ai.v[1] - l10 * yi[0];
+    yi[2] = ai.v[2] - l20 * yi[0] - l21 * yi[1];
+    yi[3] = ai.v[3] - l30 * yi[0] - l31 * yi[1] - l32 * yi[2];
+    yc[0] = ac.v[0];
+    yc[1] = ac.v[1] - l10 * yc[0];
+    yc[2] = ac.v[2] - l20 * yc[0] - l21 * yc[1];
+    yc[3] = ac.v[3] - l30 * yc[0] - l31 * yc[1] - l32 * yc[2];
+    wc[0] = xr.v[0];
+    wc[1] = xr.v[1] - l10 * wc[0];
+    wc[2] = xr.v[2] - l20 * wc[0] - l21 * wc[1];
+    wc[3] = xr.v[3] - l30 * wc[0] - l31 * wc[1] - l32 * wc[2];
+    const double z0 = yi[0] * r0, z1 = yi[1] * r1, z2 = yi[2] * r2, z3 = yi[3] * r3;
+    if (tid == 0) { fs.d[j] = d0; fs.d[j + 1] = d1; fs.d[j + 2] = d2; fs.d[j + 3] = d3; }
+    const double a_upd = a - ((z0 * yc[0] + z1 * yc[1]) + (z2 * yc[2] + z3 * yc[3]));
+    const double x_upd = x - ((z0 * wc[0] + z1 * wc[1]) + (z2 * wc[2] + z3 * wc[3]));
+    const int cq = c - j, iq = i - j;
+    const double a_blk = cq == 0 ? yi[0] : (cq == 1 ? yi[1] : (cq == 2 ? yi[2] : yi[3]));
+    const double x_blk = iq == 0 ? wc[0] : (iq == 1 ? wc[1] : (iq == 2 ? wc[2] : wc[3]));
+    if (i >= c) a = (c >= j + 4) ? a_upd : (c >= j ? a_blk : a);
+    x = (i >= j + 4) ? x_upd : (i >= j ? x_blk : x);
   }
   __syncthreads();
   const double dc = fs.d[c] > 0.0 ? fs.d[c] : 1.0, di = fs.d[i] > 0.0 ? fs.d[i] : 1.0;
