@@ -508,30 +508,43 @@ __global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d) {
   const int ls = t >> 2, lq = t & 3;  // loader: incidence, quarter of its 2 NV vectors
   const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
   acc_t acc = {0, 0, 0, 0};
+  // software pipeline: while stage s is multiplied, the Z rows of stage s + 1
+  // are in flight (registers) and so is the incidence code of stage s + 2
+  vec v[2 * NV / 4];
+  T qv[3] = {0, 0, 0};
+  auto fetch = [&](uint64_t code) {
+    const int a = inc_a(code), b = inc_b(code);
+    const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
+    const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
+#pragma unroll
+    for (int u = 0; u < 2 * NV / 4; ++u) {
+      const int q = lq + 4 * u;
+      v[u] = q < NV ? ra[q] : rb[q - NV];
+    }
+    if (diag && lq == 0) {
+      const int p = d.key_pt[a];
+      qv[0] = d.q[3 * p]; qv[1] = d.q[3 * p + 1]; qv[2] = d.q[3 * p + 2];
+    }
+  };
+  uint64_t code_next = 0;
+  {
+    const int n0 = je - jb < SI ? (int)(je - jb) : SI;
+    if (ls < n0) fetch(d.inc[jb + ls]);
+    if (jb + SI + ls < je) code_next = d.inc[jb + SI + ls];
+  }
   for (int64_t base = jb; base < je; base += SI) {
     const int n = je - base < SI ? (int)(je - base) : SI;
     if (ls < n) {
-      const uint64_t code = d.inc[base + ls];
-      const int a = inc_a(code), b = inc_b(code);
-      const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
-      const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
-      vec v[2 * NV / 4];
-#pragma unroll
-      for (int u = 0; u < 2 * NV / 4; ++u) {
-        const int q = lq + 4 * u;
-        v[u] = q < NV ? ra[q] : rb[q - NV];
-      }
 #pragma unroll
       for (int u = 0; u < 2 * NV / 4; ++u) {
         const int q = lq + 4 * u;
         sz[q < NV ? 0 : 1][ls][q < NV ? q : q - NV] = v[u];
       }
-      if (diag && lq == 0) {
-        const int p = d.key_pt[a];
-        sq[ls][0] = d.q[3 * p]; sq[ls][1] = d.q[3 * p + 1]; sq[ls][2] = d.q[3 * p + 2];
-      }
+      if (diag && lq == 0) { sq[ls][0] = qv[0]; sq[ls][1] = qv[1]; sq[ls][2] = qv[2]; }
     }
     __syncthreads();
+    if (base + SI + ls < je) fetch(code_next);
+    code_next = (base + 2 * SI + ls < je) ? d.inc[base + 2 * SI + ls] : 0;
     const int rows = 3 * max(0, min(16, n - 16 * wave));
     for (int g0 = 0; g0 < rows; g0 += 4) {
       const int g = g0 + kr;
