@@ -345,18 +345,59 @@ __device__ __forceinline__ void tile_sub_nt_global(double *C, int64_t ld, int r0
       for (int e = 0; e < 4; ++e) C[(int64_t)(rr + 16 * u + 4 * e) * ld + cc + 16 * v] = cv[u][v][e] - acc[u][v][e];
 }
 
-// c (LDS) -= a b^T; c must not alias a or b.
-__device__ __forceinline__ void tile_sub_nt_lds(double (*c)[LDT], const double (*a)[LDT], const double (*b)[LDT]) {
-  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-  const int rl = 32 * (w >> 1) + (l >> 4), cl = 32 * (w & 1) + (l & 15);
+// out = a b^T for LDS tiles with b lower triangular (b[j][k] = 0 for k > j,
+// as every stored L_kk^-1 is); out may alias a.  Wave w forms row blocks
+// 2 (w / 2), 2 (w / 2) + 1 of column blocks {0, 3} (w even) or {1, 2} (w odd)
+// and skips the zero part of b: 40 block products per wave instead of 64,
+// with the same result (only exact zeros are left out).
+__device__ __forceinline__ void tile_mul_nt_tril_lds(double (*out)[LDT], const double (*a)[LDT],
+                                                     const double (*b)[LDT]) {
+  const int w = threadIdx.x >> 6;
+  const int r0 = 32 * (w >> 1);
+  const int cb0 = (w & 1) ? 1 : 0, cb1 = (w & 1) ? 2 : 3;
   dbl4 acc[2][2];
-  mfma_nt(a, b, acc);
 #pragma unroll
   for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 2; ++v)
+    for (int v = 0; v < 2; ++v) acc[u][v] = dbl4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int e = 0; e < 4; ++e) c[rl + 16 * u + 4 * e][cl + 16 * v] -= acc[u][v][e];
+  for (int kb = 0; kb < 4; ++kb) {
+    if (kb <= cb0) {
+      acc[0][0] = blk_nt(a, r0, PB * kb, b, PB * cb0, PB * kb, acc[0][0]);
+      acc[1][0] = blk_nt(a, r0 + PB, PB * kb, b, PB * cb0, PB * kb, acc[1][0]);
+    }
+    if (kb <= cb1) {
+      acc[0][1] = blk_nt(a, r0, PB * kb, b, PB * cb1, PB * kb, acc[0][1]);
+      acc[1][1] = blk_nt(a, r0 + PB, PB * kb, b, PB * cb1, PB * kb, acc[1][1]);
+    }
+  }
+  __syncthreads();  // out may alias a
+  blk_put(out, r0, PB * cb0, acc[0][0], 1.0);
+  blk_put(out, r0 + PB, PB * cb0, acc[1][0], 1.0);
+  blk_put(out, r0, PB * cb1, acc[0][1], 1.0);
+  blk_put(out, r0 + PB, PB * cb1, acc[1][1], 1.0);
+  __syncthreads();
+}
+
+// c (LDS) -= a a^T on the ten lower 16 x 16 blocks (i >= j) only, dealt
+// 3 / 3 / 2 / 2 to the waves; the upper blocks of c are left as they are.
+// c must not alias a.
+__device__ __forceinline__ void tile_sub_aat_lower_lds(double (*c)[LDT], const double (*a)[LDT]) {
+  const int w = threadIdx.x >> 6;
+  // wave 0: (0,0) (1,0) (3,3); wave 1: (1,1) (2,0) (3,2); wave 2: (2,1) (3,0); wave 3: (2,2) (3,1)
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    int bi, bj;
+    if (q == 0) { bi = w == 0 ? 0 : (w == 1 ? 1 : 2); bj = w == 0 ? 0 : (w == 3 ? 2 : 1); }
+    else if (q == 1) { bi = w == 0 ? 1 : (w == 1 ? 2 : 3); bj = w == 3 ? 1 : 0; }
+    else { bi = w < 2 ? 3 : -1; bj = w == 0 ? 3 : 2; }
+    if (bi >= 0) {
+      dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb) acc = blk_nt(a, PB * bi, PB * kb, a, PB * bj, PB * kb, acc);
+      blk_sub(c, PB * bi, PB * bj, acc);
+    }
+  }
 }
 
 // y = M v (transposed = false) or M^T v for a 64 x 64 global tile M with row
@@ -450,11 +491,11 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
     if (jb != ib || look) put_tile(pb, v2);
   }
   __syncthreads();
-  tile_mul_nt_lds(pa, pa, li);                          // L_ik = A_ik Linv^T
-  if (jb != ib) tile_mul_nt_lds(pb, pb, li);            // L_jk
+  tile_mul_nt_tril_lds(pa, pa, li);                     // L_ik = A_ik Linv^T
+  if (jb != ib) tile_mul_nt_tril_lds(pb, pb, li);       // L_jk
   if (jb == k + 1) store_tile(pa, L, ld, ib * NB, k * NB);
   if (look) {
-    tile_sub_nt_lds(pb, pa, pa);
+    tile_sub_aat_lower_lds(pb, pa);
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
     tile_factor_inverse_lds(L + (int64_t)ib * NB * ld + ib * NB, ld, Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB,
@@ -605,7 +646,7 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
     for (int ib = kb + 1; ib < wi.nb; ++ib) {
       load_tile(a, A, ld, ib * NB, kb * NB);
       __syncthreads();
-      tile_mul_nt_lds(a, a, c);
+      tile_mul_nt_tril_lds(a, a, c);
       store_tile(a, A, ld, ib * NB, kb * NB);
     }
     __syncthreads();
