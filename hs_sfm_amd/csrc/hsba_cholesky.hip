@@ -207,8 +207,10 @@ __device__ __forceinline__ void blk_sub(double (*out)[LDT], int r, int c, dbl4 a
 }
 
 // Factor a 64 x 64 SPD tile (lower triangle of src, row pitch lds) and invert
-// the factor: writes L (zero upper) to dstL and L^-1 (zero upper) to dstI; a
-// non-positive pivot sets *fail.  wa / wx are two LDS tiles of scratch.
+// the factor: writes L (zero upper) to dstL unless it is null and L^-1 (zero
+// upper) to dstI; a non-positive pivot sets *fail.  The factorizations below
+// pass a null dstL: the solves and the inverse use L_kk^-1 for diagonal tiles
+// and never read L_kk.  wa / wx are two LDS tiles of scratch.
 // Right-looking blocked Cholesky with 16-wide panels p = 0..3:
 //   L_pp = chol(A_pp), M_p = L_pp^-1      (whole workgroup, diag16)
 //   L_rp = A_rp M_p^T                     (rows below the panel; wave w: block row p+1+w)
@@ -268,7 +270,7 @@ __device__ __forceinline__ void tile_factor_inverse_lds(double *dstL, int64_t ld
 #pragma unroll 4
   for (int q = 0; q < 16; ++q) {
     const int e = tid + 256 * q, i = e >> 6, j = e & 63;
-    dstL[(int64_t)i * ldl + j] = (j <= i) ? wa[i][j] : 0.0;
+    if (dstL != nullptr) dstL[(int64_t)i * ldl + j] = (j <= i) ? wa[i][j] : 0.0;
     dstI[(int64_t)i * ldi + j] = (j <= i) ? wx[i][j] : 0.0;
   }
 }
@@ -435,7 +437,7 @@ __global__ void __launch_bounds__(256) k_chol_first(WinInfo wi, WinState *ws, in
   if (!chol_active(s)) return;
   if (threadIdx.x == 0) fail = 0;
   __syncthreads();
-  tile_factor_inverse(S + wi.s_off, wi.ld, Lf + wi.s_off, wi.ld, Dinv + wi.dinv_off, NB, &fail, wa, wx, fs);
+  tile_factor_inverse(S + wi.s_off, wi.ld, nullptr, wi.ld, Dinv + wi.dinv_off, NB, &fail, wa, wx, fs);
   __syncthreads();
   if (threadIdx.x == 0 && fail) s.chol_fail = 1;
 }
@@ -498,7 +500,7 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
     tile_sub_aat_lower_lds(pb, pa);
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
-    tile_factor_inverse_lds(L + (int64_t)ib * NB * ld + ib * NB, ld, Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB,
+    tile_factor_inverse_lds(nullptr, ld, Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB,
                             &fail, pb, pa, fs);
     __syncthreads();
     if (threadIdx.x == 0 && fail) s.chol_fail = 1;
@@ -636,7 +638,7 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
   for (int kb = 0; kb < wi.nb; ++kb) {
     __syncthreads();
     __threadfence_block();
-    tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, A + (int64_t)kb * NB * ld + kb * NB, ld,
+    tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, nullptr, ld,
                         Dw + (int64_t)kb * NB * NB, NB, &fail, a, c, fs);
     __threadfence_block();
     __syncthreads();
