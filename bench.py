@@ -180,7 +180,7 @@ def run_mid(args):
 
 
 def run_large(args):
-    """2000 cameras x 9, 600k points, ~5.1M keys, banded; block-Jacobi
+    """Large banded strip: 2000 cameras x 9, 600k points, ~5.1M keys, banded; block-Jacobi
     PCG on the implicit Schur complement (FP32 matvec, FP64 vectors), rtol 1e-6."""
     scene = syn.large_scene()
     K = args.steps
@@ -209,7 +209,7 @@ def run_large(args):
 
 
 def run_windows(args):
-    """512 independent windows (5..40 cameras, 200..5000 points) in
+    """Batch of 512 independent windows (5..40 cameras, 200..5000 points) in
     one batch (one set of launches per LM iteration) vs the same windows solved
     one at a time through the single-problem path."""
     scenes = syn.windows_batch(count=args.windows)
