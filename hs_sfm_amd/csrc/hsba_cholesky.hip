@@ -8,8 +8,9 @@
 //
 // Diagonal tiles: a blocked factorization with 16-wide panels (each 16 x 16
 // diagonal block eliminated by the whole workgroup, one element per thread,
-// two columns per barrier) yields both L_kk and L_kk^-1; panel tiles are then
-// GEMMs with L_kk^-1.
+// four columns per barrier; the block products on the matrix cores) yields
+// L_kk^-1, the only form of the diagonal factor tiles that the solves and the
+// inverse read; panel tiles are then GEMMs with L_kk^-1.
 //
 // Large windows (more than 4 tiles): one fused launch per panel step k.
 // Every workgroup of step k owns one trailing tile (i, j), k < j <= i: it forms
@@ -17,8 +18,9 @@
 // workgroup of tile (k+1, k+1) then factors that tile (look-ahead), so each
 // step is a single launch.  The right-hand side rides along: step k also
 // forms y_k = L_kk^-1 b_k and updates b_j -= L_jk y_k.  The factor goes to a
-// separate buffer (column k of A is read by every workgroup of step k).  The
-// backward solve L^T x = y is one multi-workgroup launch per four block rows.
+// separate buffer (column k of A is read by every workgroup of step k); its
+// diagonal tiles are not written.  The backward solve L^T x = y is one
+// multi-workgroup launch per four block rows.
 //
 // Small windows: one workgroup (256 threads) per window runs the whole
 // blocked factorization and both solves; a batch of windows is one launch.
