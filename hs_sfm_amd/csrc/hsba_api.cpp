@@ -85,13 +85,10 @@ int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_opti
     return rc;
   }
   // FP64 dense-Schur problems whose camera block is 7 or more wide are switched
-  // off: the FP64 LM on the 200-camera x 9-parameter scene faulted the GPU
-  // once, and no cause was found in this code (the same kernels run on the
-  // CPU emulation under AddressSanitizer agree with the FP64 oracle to 1e-15).
-  // Those instantiations of the pair kernel exceed the VGPR budget (AGPRs plus
-  // ~1350 spilled SGPRs).  Kept off until a cause is shown.
-  if (o.precision == HSBA_PRECISION_FP64 && eng->L.pts_free && eng->L.E >= 7 &&
-      !std::getenv("HSBA_ALLOW_FP64_WIDE"))
+  // off: an FP64 LM on the 200-camera x 9-parameter scene faulted the GPU once
+  // (with an earlier pair kernel that spilled ~1350 SGPRs), and no cause in
+  // this code has been shown.  Kept off, with no override, until one is.
+  if (o.precision == HSBA_PRECISION_FP64 && eng->L.pts_free && eng->L.E >= 7)
     return HSBA_ERR_UNSUPPORTED;
   eng->precision = o.precision;
   eng->pcg_rtol = o.pcg_rtol;
@@ -128,37 +125,45 @@ extern "C" {
 const char *hsba_version(void) { return "hsba-mi355x 0.1 (gfx950)"; }
 
 int hsba_device_count(int *count) {
-  if (!count) return HSBA_ERR_INVALID;
-  int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess) { *count = 0; return HSBA_ERR_DEVICE; }
-  *count = n;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!count) return HSBA_ERR_INVALID;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { *count = 0; return HSBA_ERR_DEVICE; }
+    *count = n;
+    return HSBA_OK;
+  });
 }
 
 int hsba_problem_create(const hsba_problem_desc *desc, const hsba_options *options, hsba_problem **problem) {
-  if (!desc || !problem) return HSBA_ERR_INVALID;
-  *problem = nullptr;
-  std::unique_ptr<hsba_problem> p(new hsba_problem());
-  int rc = make(desc, 1, options, p->eng);
-  if (rc) return rc;
-  *problem = p.release();
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!desc || !problem) return HSBA_ERR_INVALID;
+    *problem = nullptr;
+    std::unique_ptr<hsba_problem> p(new hsba_problem());
+    int rc = make(desc, 1, options, p->eng);
+    if (rc) return rc;
+    *problem = p.release();
+    return HSBA_OK;
+  });
 }
 
 int hsba_problem_destroy(hsba_problem *problem) {
-  delete problem;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    delete problem;
+    return HSBA_OK;
+  });
 }
 
 int hsba_problem_sizes(const hsba_problem *p, int64_t *x_size, int64_t *y_size, int64_t *camera_params,
                        int64_t *reduced_size) {
-  if (!p) return HSBA_ERR_INVALID;
-  const Layout &L = p->eng->L;
-  if (x_size) *x_size = L.hw[0].x_size;
-  if (y_size) *y_size = L.hw[0].y_size;
-  if (camera_params) *camera_params = __builtin_popcount(L.free_bits);
-  if (reduced_size) *reduced_size = L.wi[0].n;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!p) return HSBA_ERR_INVALID;
+    const Layout &L = p->eng->L;
+    if (x_size) *x_size = L.hw[0].x_size;
+    if (y_size) *y_size = L.hw[0].y_size;
+    if (camera_params) *camera_params = __builtin_popcount(L.free_bits);
+    if (reduced_size) *reduced_size = L.wi[0].n;
+    return HSBA_OK;
+  });
 }
 
 static int set_x(hsba_problem *p, const double *x) {
@@ -167,159 +172,193 @@ static int set_x(hsba_problem *p, const double *x) {
 }
 
 int hsba_evaluate(hsba_problem *p, const double *x, double *y) {
-  if (!p || !x || !y) return HSBA_ERR_INVALID;
-  int rc = set_x(p, x);
-  if (rc) return rc;
-  if ((rc = p->eng->evaluate(y))) return rc;
-  // constraint rows copy x entries (camera_shared_vector_function.hpp:485-629)
-  const hsba::HostWindow &h = p->eng->L.hw[0];
-  for (size_t q = 0; q < h.cons_xidx.size(); ++q) y[2 * h.N + q] = x[h.cons_xidx[q]];
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!p || !x || !y) return HSBA_ERR_INVALID;
+    int rc = set_x(p, x);
+    if (rc) return rc;
+    if ((rc = p->eng->evaluate(y))) return rc;
+    // constraint rows copy x entries (camera_shared_vector_function.hpp:485-629)
+    const hsba::HostWindow &h = p->eng->L.hw[0];
+    for (size_t q = 0; q < h.cons_xidx.size(); ++q) y[2 * h.N + q] = x[h.cons_xidx[q]];
+    return HSBA_OK;
+  });
 }
 
 int hsba_jacobian(hsba_problem *p, const double *x, double *point_blocks, double *image_blocks,
                   double *camera_blocks) {
-  if (!p || !x) return HSBA_ERR_INVALID;
-  int rc = set_x(p, x);
-  if (rc) return rc;
-  return p->eng->jacobian(point_blocks, image_blocks, camera_blocks);
+  return hsba::guarded([&]() -> int {
+    if (!p || !x) return HSBA_ERR_INVALID;
+    int rc = set_x(p, x);
+    if (rc) return rc;
+    return p->eng->jacobian(point_blocks, image_blocks, camera_blocks);
+  });
 }
 
 int hsba_build_normal(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
                       double *point_blocks, double *image_blocks, double *camera_blocks, double *gradient) {
-  if (!p || !x || !y_obs) return HSBA_ERR_INVALID;
-  int rc = set_x(p, x);
-  if (rc) return rc;
-  const double *ys[1] = {y_obs};
-  if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
-  return p->eng->build_normal(point_blocks, image_blocks, camera_blocks, gradient);
+  return hsba::guarded([&]() -> int {
+    if (!p || !x || !y_obs) return HSBA_ERR_INVALID;
+    int rc = set_x(p, x);
+    if (rc) return rc;
+    const double *ys[1] = {y_obs};
+    if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
+    return p->eng->build_normal(point_blocks, image_blocks, camera_blocks, gradient);
+  });
 }
 
 int hsba_covariance(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
                     double *point_blocks, double *image_blocks, double *camera_blocks, double *reduced) {
-  if (!p || !x) return HSBA_ERR_INVALID;
-  int rc = set_x(p, x);
-  if (rc) return rc;
-  // N does not depend on y except through robust-loss weights
-  std::vector<double> zeros;
-  if (!y_obs) {
-    zeros.assign(p->eng->L.hw[0].y_size, 0.0);
-    y_obs = zeros.data();
-  }
-  const double *ys[1] = {y_obs};
-  if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
-  if ((rc = p->eng->build_normal(nullptr, nullptr, nullptr, nullptr))) return rc;
-  return p->eng->covariance(point_blocks, image_blocks, camera_blocks, reduced);
+  return hsba::guarded([&]() -> int {
+    if (!p || !x) return HSBA_ERR_INVALID;
+    int rc = set_x(p, x);
+    if (rc) return rc;
+    // N does not depend on y except through robust-loss weights
+    std::vector<double> zeros;
+    if (!y_obs) {
+      zeros.assign(p->eng->L.hw[0].y_size, 0.0);
+      y_obs = zeros.data();
+    }
+    const double *ys[1] = {y_obs};
+    if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
+    if ((rc = p->eng->build_normal(nullptr, nullptr, nullptr, nullptr))) return rc;
+    return p->eng->covariance(point_blocks, image_blocks, camera_blocks, reduced);
+  });
 }
 
 int hsba_solve(hsba_problem *p, double mu, double *delta_x) {
-  if (!p || !delta_x) return HSBA_ERR_INVALID;
-  return p->eng->solve(mu, delta_x, nullptr, nullptr);
+  return hsba::guarded([&]() -> int {
+    if (!p || !delta_x) return HSBA_ERR_INVALID;
+    return p->eng->solve(mu, delta_x, nullptr, nullptr);
+  });
 }
 
 int hsba_reduced_system(hsba_problem *p, double mu, double *reduced_matrix, double *reduced_rhs) {
-  if (!p) return HSBA_ERR_INVALID;
-  return p->eng->solve(mu, nullptr, reduced_matrix, reduced_rhs);
+  return hsba::guarded([&]() -> int {
+    if (!p) return HSBA_ERR_INVALID;
+    return p->eng->solve(mu, nullptr, reduced_matrix, reduced_rhs);
+  });
 }
 
 int hsba_max_diagonal(hsba_problem *p, double *value) {
-  if (!p || !value) return HSBA_ERR_INVALID;
-  return p->eng->scalars(value, nullptr);
+  return hsba::guarded([&]() -> int {
+    if (!p || !value) return HSBA_ERR_INVALID;
+    return p->eng->scalars(value, nullptr);
+  });
 }
 
 int hsba_gradient_inf_norm(hsba_problem *p, double *value) {
-  if (!p || !value) return HSBA_ERR_INVALID;
-  return p->eng->scalars(nullptr, value);
+  return hsba::guarded([&]() -> int {
+    if (!p || !value) return HSBA_ERR_INVALID;
+    return p->eng->scalars(nullptr, value);
+  });
 }
 
 int hsba_gradient_dot(hsba_problem *p, const double *delta_x, double *value) {
-  if (!p || !delta_x || !value) return HSBA_ERR_INVALID;
-  const Layout &L = p->eng->L;
-  std::vector<double> g(L.hw[0].x_size);
-  int rc = p->eng->download_normal(nullptr, nullptr, nullptr, g.data());
-  if (rc) return rc;
-  double s = 0.0;
-  for (size_t j = 0; j < g.size(); ++j) s += g[j] * delta_x[j];
-  *value = s;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!p || !delta_x || !value) return HSBA_ERR_INVALID;
+    const Layout &L = p->eng->L;
+    std::vector<double> g(L.hw[0].x_size);
+    int rc = p->eng->download_normal(nullptr, nullptr, nullptr, g.data());
+    if (rc) return rc;
+    double s = 0.0;
+    for (size_t j = 0; j < g.size(); ++j) s += g[j] * delta_x[j];
+    *value = s;
+    return HSBA_OK;
+  });
 }
 
 int hsba_mahalanobis(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
                      double *value) {
-  if (!p || !x || !y_obs || !value) return HSBA_ERR_INVALID;
-  int rc = set_x(p, x);
-  if (rc) return rc;
-  const double *ys[1] = {y_obs};
-  if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
-  return p->eng->cost(value);
+  return hsba::guarded([&]() -> int {
+    if (!p || !x || !y_obs || !value) return HSBA_ERR_INVALID;
+    int rc = set_x(p, x);
+    if (rc) return rc;
+    const double *ys[1] = {y_obs};
+    if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
+    return p->eng->cost(value);
+  });
 }
 
 int hsba_lm_solve(hsba_problem *p, const double *x0, const double *y_obs, const hsba_ycov_inverse *ycov,
                   const hsba_lm_options *lm, double *x_out, hsba_lm_stats *stats) {
-  if (!p || !x0 || !y_obs || !lm || !x_out) return HSBA_ERR_INVALID;
-  int rc = set_x(p, x0);
-  if (rc) return rc;
-  const double *ys[1] = {y_obs};
-  if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
-  hsba_lm_stats st;
-  rc = p->eng->lm(lm, &st);
-  double *xo[1] = {x_out};
-  const int rc2 = p->eng->download_x(xo);
-  if (stats) *stats = st;
-  return rc ? rc : rc2;
+  return hsba::guarded([&]() -> int {
+    if (!p || !x0 || !y_obs || !lm || !x_out) return HSBA_ERR_INVALID;
+    int rc = set_x(p, x0);
+    if (rc) return rc;
+    const double *ys[1] = {y_obs};
+    if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
+    hsba_lm_stats st;
+    rc = p->eng->lm(lm, &st);
+    double *xo[1] = {x_out};
+    const int rc2 = p->eng->download_x(xo);
+    if (stats) *stats = st;
+    return rc ? rc : rc2;
+  });
 }
 
 int hsba_lm_begin(hsba_problem *p, const double *x0, const double *y_obs, const hsba_ycov_inverse *ycov,
                   const hsba_lm_options *lm) {
-  if (!p || !x0 || !y_obs || !lm) return HSBA_ERR_INVALID;
-  int rc = set_x(p, x0);
-  if (rc) return rc;
-  const double *ys[1] = {y_obs};
-  if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
-  return p->eng->lm_begin(lm);
+  return hsba::guarded([&]() -> int {
+    if (!p || !x0 || !y_obs || !lm) return HSBA_ERR_INVALID;
+    int rc = set_x(p, x0);
+    if (rc) return rc;
+    const double *ys[1] = {y_obs};
+    if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
+    return p->eng->lm_begin(lm);
+  });
 }
 
 int hsba_lm_iterate(hsba_problem *p, int32_t iterations, hsba_stage_times *times) {
-  if (!p) return HSBA_ERR_INVALID;
-  return p->eng->lm_iterate(iterations, times);
+  return hsba::guarded([&]() -> int {
+    if (!p) return HSBA_ERR_INVALID;
+    return p->eng->lm_iterate(iterations, times);
+  });
 }
 
 int hsba_lm_end(hsba_problem *p, double *x_out, hsba_lm_stats *stats) {
-  if (!p || !x_out) return HSBA_ERR_INVALID;
-  hsba_lm_stats st;
-  int rc = p->eng->lm_end(&st);
-  double *xo[1] = {x_out};
-  const int rc2 = p->eng->download_x(xo);
-  if (stats) *stats = st;
-  return rc ? rc : rc2;
+  return hsba::guarded([&]() -> int {
+    if (!p || !x_out) return HSBA_ERR_INVALID;
+    hsba_lm_stats st;
+    int rc = p->eng->lm_end(&st);
+    double *xo[1] = {x_out};
+    const int rc2 = p->eng->download_x(xo);
+    if (stats) *stats = st;
+    return rc ? rc : rc2;
+  });
 }
 
 int hsba_batch_create(const hsba_problem_desc *descs, int64_t count, const hsba_options *options,
                       hsba_batch **batch) {
-  if (!descs || count <= 0 || !batch) return HSBA_ERR_INVALID;
-  *batch = nullptr;
-  std::unique_ptr<hsba_batch> b(new hsba_batch());
-  int rc = make(descs, count, options, b->eng);
-  if (rc) return rc;
-  *batch = b.release();
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!descs || count <= 0 || !batch) return HSBA_ERR_INVALID;
+    *batch = nullptr;
+    std::unique_ptr<hsba_batch> b(new hsba_batch());
+    int rc = make(descs, count, options, b->eng);
+    if (rc) return rc;
+    *batch = b.release();
+    return HSBA_OK;
+  });
 }
 
 int hsba_batch_destroy(hsba_batch *batch) {
-  delete batch;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    delete batch;
+    return HSBA_OK;
+  });
 }
 
 int hsba_batch_lm_solve(hsba_batch *b, const double *const *x0s, const double *const *y_obs,
                         const hsba_ycov_inverse *ycovs, const hsba_lm_options *lm, double *const *x_outs,
                         hsba_lm_stats *stats) {
-  if (!b || !x0s || !y_obs || !lm || !x_outs || !stats) return HSBA_ERR_INVALID;
-  int rc = b->eng->upload_x(x0s);
-  if (rc) return rc;
-  if ((rc = b->eng->upload_obs(y_obs, ycovs))) return rc;
-  rc = b->eng->lm(lm, stats);
-  const int rc2 = b->eng->download_x(x_outs);
-  return rc ? rc : rc2;
+  return hsba::guarded([&]() -> int {
+    if (!b || !x0s || !y_obs || !lm || !x_outs || !stats) return HSBA_ERR_INVALID;
+    int rc = b->eng->upload_x(x0s);
+    if (rc) return rc;
+    if ((rc = b->eng->upload_obs(y_obs, ycovs))) return rc;
+    rc = b->eng->lm(lm, stats);
+    const int rc2 = b->eng->download_x(x_outs);
+    return rc ? rc : rc2;
+  });
 }
 
 }  // extern "C"

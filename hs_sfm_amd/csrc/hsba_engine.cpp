@@ -363,7 +363,7 @@ struct Engine final : EngineBase {
   }
 
   // ---- window state helpers
-  int set_state(const hsba_lm_options *lm, double mu, int active, int need_lin, int trial) {
+  int set_state(const hsba_lm_options *lm, double mu, int active, int need_lin, int trial, bool upload = true) {
     for (int w = 0; w < L.nwin; ++w) {
       WinState &s = ws_host[w];
       std::memset(&s, 0, sizeof(s));
@@ -372,7 +372,7 @@ struct Engine final : EngineBase {
       s.max_iter = lm ? lm->max_iterations : 0;
       s.active = active; s.need_lin = need_lin; s.trial = trial;
     }
-    HCHK(hipMemcpyAsync(d.ws, ws_host.data(), sizeof(WinState) * L.nwin, hipMemcpyHostToDevice, st));
+    if (upload) HCHK(hipMemcpyAsync(d.ws, ws_host.data(), sizeof(WinState) * L.nwin, hipMemcpyHostToDevice, st));
     return HSBA_OK;
   }
   int get_state() {
@@ -512,18 +512,30 @@ struct Engine final : EngineBase {
     return rc ? rc : rc2;
   }
 
-  bool lm_failed_early = false;
+  int lm_run(const hsba_lm_options *lm, hsba_lm_stats *stats) override {
+    int rc = lm_begin(lm);
+    if (rc == HSBA_OK && lm->max_iterations > 0) rc = lm_iterate(lm->max_iterations, nullptr);
+    if (rc) return rc;
+    rc = lm_end(stats);
+    return (rc == HSBA_ERR_DEVICE || rc == HSBA_ERR_OOM) ? rc : HSBA_OK;
+  }
 
   int lm_begin(const hsba_lm_options *lm) override {
-    lm_failed_early = false;
+    int rc = set_state(lm, 0.0, 1, 1, 0, /*upload=*/false);
+    if (rc) return rc;
+    // A free image or camera without any key leaves its diagonal block out of
+    // the reference's reduced system, so SimplicialLDLT fails for that
+    // problem: that window stops at once with the reference's -1, and the
+    // other windows of the batch run as if alone.
     for (int w = 0; w < L.nwin; ++w)
       if (L.hw[w].unobserved) {
-        // the reference's reduced system misses that block: SimplicialLDLT fails
-        lm_failed_early = true;
-        return HSBA_ERR_GENERIC;
+        WinState &s = ws_host[w];
+        s.active = 0;
+        s.need_lin = 0;
+        s.status = HSBA_ERR_GENERIC;
+        s.stop_reason = HSBA_STOP_FAILED;
       }
-    int rc = set_state(lm, 0.0, 1, 1, 0);
-    if (rc) return rc;
+    HCHK(hipMemcpyAsync(d.ws, ws_host.data(), sizeof(WinState) * L.nwin, hipMemcpyHostToDevice, st));
     // initial cost, linearization, mu0 and the gradient test
     ks->cost(d, st, 0);
     enqueue_linearize();
@@ -575,7 +587,6 @@ struct Engine final : EngineBase {
   }
 
   int lm_iterate(int iterations, hsba_stage_times *times) override {
-    if (lm_failed_early) return HSBA_ERR_GENERIC;
     if (iterations <= 0) return HSBA_OK;
     if (L.pcg) return pcg_iterate(iterations, times);
     if (!times) {
@@ -639,14 +650,6 @@ struct Engine final : EngineBase {
   }
 
   int lm_end(hsba_lm_stats *stats) override {
-    if (lm_failed_early) {
-      for (int q = 0; q < L.nwin; ++q) {
-        std::memset(&stats[q], 0, sizeof(hsba_lm_stats));
-        stats[q].status = L.hw[q].unobserved ? HSBA_ERR_GENERIC : HSBA_ERR_UNSUPPORTED;
-        stats[q].stop_reason = HSBA_STOP_FAILED;
-      }
-      return HSBA_ERR_GENERIC;
-    }
     int rc = get_state();
     if (rc) return rc;
     built = true;

@@ -4,6 +4,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -61,6 +62,23 @@ int64_t layout_device_bytes(const Layout &L, int precision);
 int64_t estimate_device_bytes(const hsba_problem_desc &desc, int precision, bool pcg, bool has_cov);
 // x and y lengths of one problem in the reference layouts (before validation).
 int problem_sizes(const hsba_problem_desc &desc, int64_t *x_size, int64_t *y_size);
+// The checks build_layout makes on one problem alone (sizes, id ranges,
+// duplicate keys, constraint records), without building anything: lets the
+// serving loop refuse a malformed job at submit instead of failing its batch.
+int validate_desc(const hsba_problem_desc &desc, std::string *why);
+
+// Runs f() and maps an escaping C++ exception to an error code, so that no
+// exception crosses an extern "C" entry point.
+template <class F>
+int guarded(F &&f) noexcept {
+  try {
+    return f();
+  } catch (const std::bad_alloc &) {
+    return HSBA_ERR_OOM;
+  } catch (...) {
+    return HSBA_ERR_INVALID;
+  }
+}
 
 // First-fit sub-allocator over one device allocation, with coalescing of
 // freed neighbours (256-byte granules).  Thread-safe.  The serving loop
@@ -98,7 +116,11 @@ struct EngineBase {
   virtual int upload_x(const double *const *xs) = 0;
   virtual int upload_obs(const double *const *ys, const hsba_ycov_inverse *ycovs) = 0;
   virtual int download_x(double *const *xs) = 0;
+  // Full LM solve; returns the worst per-window status (or a device error).
   virtual int lm(const hsba_lm_options *lm, hsba_lm_stats *stats) = 0;
+  // The same solve for callers that read each window's status from stats:
+  // returns only device / allocation errors, never a window's own failure.
+  virtual int lm_run(const hsba_lm_options *lm, hsba_lm_stats *stats) = 0;
   virtual int lm_begin(const hsba_lm_options *lm) = 0;
   virtual int lm_iterate(int iterations, hsba_stage_times *times) = 0;
   virtual int lm_end(hsba_lm_stats *stats) = 0;

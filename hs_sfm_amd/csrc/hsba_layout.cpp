@@ -473,6 +473,67 @@ int64_t layout_device_bytes(const Layout &L, int precision) {
   return b;
 }
 
+int validate_desc(const hsba_problem_desc &ds, std::string *why) {
+  if (ds.num_points < 0 || ds.num_images < 0 || ds.num_cameras < 0 || ds.num_keys < 0)
+    return fail(why, "negative size");
+  if (ds.num_points >= (1LL << 31) || ds.num_keys >= (1LL << 31)) return fail(why, "too large", HSBA_ERR_UNSUPPORTED);
+  if (ds.num_keys > 0 && !ds.feature_maps) return fail(why, "feature_maps missing");
+  if (ds.num_images > 0 && !ds.image_camera_map) return fail(why, "image_camera_map missing");
+  const uint32_t fix = ds.fix_mask & 7u;
+  if (fix == 7u) return fail(why, "every block fixed");
+  const uint32_t en = enabled_bits(ds.intrinsic_computations_mask & 7u);
+  const uint32_t fb = ds.intrinsic_free_mask ? ds.intrinsic_free_mask : en;
+  if (fb & ~en) return fail(why, "free mask outside the enabled groups");
+  const int d = __builtin_popcount(fb);
+  if ((fix & HSBA_FIX_POINTS) && !ds.fix_points && ds.num_points) return fail(why, "fix_points missing");
+  if ((fix & HSBA_FIX_IMAGES) && !ds.fix_images && ds.num_images) return fail(why, "fix_images missing");
+  if ((fix & HSBA_FIX_CAMERAS) && !ds.fix_cameras && ds.num_cameras && d) return fail(why, "fix_cameras missing");
+  for (int64_t i = 0; i < ds.num_images; ++i)
+    if (ds.image_camera_map[i] < 0 || ds.image_camera_map[i] >= ds.num_cameras)
+      return fail(why, "image_camera_map out of range");
+  // ranges, then duplicate (image, point) keys: keys bucketed by point, each
+  // bucket checked by sorting its images
+  std::vector<int64_t> ptr(ds.num_points + 1, 0);
+  for (int64_t k = 0; k < ds.num_keys; ++k) {
+    const int32_t im = ds.feature_maps[2 * k], pt = ds.feature_maps[2 * k + 1];
+    if (im < 0 || im >= ds.num_images || pt < 0 || pt >= ds.num_points) return fail(why, "feature_maps out of range");
+    ptr[pt + 1]++;
+  }
+  for (int64_t p = 0; p < ds.num_points; ++p) ptr[p + 1] += ptr[p];
+  {
+    std::vector<int32_t> imgs(ds.num_keys);
+    std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+    for (int64_t k = 0; k < ds.num_keys; ++k) imgs[fill[ds.feature_maps[2 * k + 1]]++] = ds.feature_maps[2 * k];
+    for (int64_t p = 0; p < ds.num_points; ++p) {
+      std::sort(imgs.begin() + ptr[p], imgs.begin() + ptr[p + 1]);
+      for (int64_t a = ptr[p] + 1; a < ptr[p + 1]; ++a)
+        if (imgs[a] == imgs[a - 1]) return fail(why, "duplicate (image, point) key");
+    }
+  }
+  if (!(fix & HSBA_FIX_POINTS) && ds.num_point_constraints > 0) {
+    if (!ds.point_constraints) return fail(why, "point_constraints missing");
+    for (int64_t q = 0; q < ds.num_point_constraints; ++q)
+      if (ds.point_constraints[q].id < 0 || ds.point_constraints[q].id >= ds.num_points)
+        return fail(why, "point constraint id");
+  }
+  if (!(fix & HSBA_FIX_IMAGES) && ds.num_image_constraints > 0) {
+    if (!ds.image_constraints) return fail(why, "image_constraints missing");
+    for (int64_t q = 0; q < ds.num_image_constraints; ++q)
+      if (ds.image_constraints[q].id < 0 || ds.image_constraints[q].id >= ds.num_images)
+        return fail(why, "image constraint id");
+  }
+  if (!(fix & HSBA_FIX_CAMERAS) && ds.num_camera_constraints > 0) {
+    if (!ds.camera_constraints) return fail(why, "camera_constraints missing");
+    for (int64_t q = 0; q < ds.num_camera_constraints; ++q) {
+      const hsba_constraint &c = ds.camera_constraints[q];
+      if (c.id < 0 || c.id >= ds.num_cameras) return fail(why, "camera constraint id");
+      if ((c.mask & 0x3FFu) & ~en) return fail(why, "camera constraint on a disabled group");
+      if ((c.mask & 0x3FFu) & ~fb) return fail(why, "camera constraint on a held parameter");
+    }
+  }
+  return HSBA_OK;
+}
+
 int problem_sizes(const hsba_problem_desc &ds, int64_t *x_size, int64_t *y_size) {
   if (ds.num_points < 0 || ds.num_images < 0 || ds.num_cameras < 0 || ds.num_keys < 0) return HSBA_ERR_INVALID;
   const uint32_t fix = ds.fix_mask & 7u;

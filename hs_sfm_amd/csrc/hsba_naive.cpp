@@ -95,142 +95,170 @@ extern "C" {
 
 int hsba_naive_problem_create(const hsba_naive_desc *desc, const hsba_options *options,
                               hsba_naive_problem **problem) {
-  if (!desc || !problem || !desc->feature_maps || desc->num_cameras <= 0 || desc->num_points < 0 ||
-      desc->num_features < 0)
-    return HSBA_ERR_INVALID;
-  *problem = nullptr;
-  std::unique_ptr<hsba_naive_problem> q(new hsba_naive_problem());
-  q->C = desc->num_cameras;
-  q->P = desc->num_points;
-  q->F = desc->num_features;
-  const int rc = create_shared(*q, desc->feature_maps, options);
-  if (rc) return rc;
-  *problem = q.release();
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!desc || !problem || !desc->feature_maps || desc->num_cameras <= 0 || desc->num_points < 0 ||
+        desc->num_features < 0)
+      return HSBA_ERR_INVALID;
+    *problem = nullptr;
+    std::unique_ptr<hsba_naive_problem> q(new hsba_naive_problem());
+    q->C = desc->num_cameras;
+    q->P = desc->num_points;
+    q->F = desc->num_features;
+    const int rc = create_shared(*q, desc->feature_maps, options);
+    if (rc) return rc;
+    *problem = q.release();
+    return HSBA_OK;
+  });
 }
 
 int hsba_naive_problem_destroy(hsba_naive_problem *problem) {
-  delete problem;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    delete problem;
+    return HSBA_OK;
+  });
 }
 
 int hsba_naive_evaluate(hsba_naive_problem *q, const double *x, double *y) {
-  if (!q || !x || !y) return HSBA_ERR_INVALID;
-  to_shared(*q, x, q->xs.data());
-  return hsba_evaluate(q->p, q->xs.data(), y);
+  return hsba::guarded([&]() -> int {
+    if (!q || !x || !y) return HSBA_ERR_INVALID;
+    to_shared(*q, x, q->xs.data());
+    return hsba_evaluate(q->p, q->xs.data(), y);
+  });
 }
 
 int hsba_naive_jacobian(hsba_naive_problem *q, const double *x, double *camera_blocks, double *point_blocks) {
-  if (!q || !x) return HSBA_ERR_INVALID;
-  if (zero_rotation(*q, x)) return HSBA_ERR_INVALID;
-  to_shared(*q, x, q->xs.data());
-  return hsba_jacobian(q->p, q->xs.data(), point_blocks, camera_blocks, nullptr);
+  return hsba::guarded([&]() -> int {
+    if (!q || !x) return HSBA_ERR_INVALID;
+    if (zero_rotation(*q, x)) return HSBA_ERR_INVALID;
+    to_shared(*q, x, q->xs.data());
+    return hsba_jacobian(q->p, q->xs.data(), point_blocks, camera_blocks, nullptr);
+  });
 }
 
 int hsba_naive_build_normal(hsba_naive_problem *q, const double *x, const double *y_obs,
                             const double *feature_cov_inverse, double *camera_blocks, double *point_blocks,
                             double *gradient) {
-  if (!q || !x || !y_obs) return HSBA_ERR_INVALID;
-  if (zero_rotation(*q, x)) return HSBA_ERR_INVALID;
-  to_shared(*q, x, q->xs.data());
-  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, nullptr};
-  std::vector<double> gs(gradient ? q->xs.size() : 0);
-  const int rc = hsba_build_normal(q->p, q->xs.data(), y_obs, &w, point_blocks, camera_blocks, nullptr,
-                                   gradient ? gs.data() : nullptr);
-  if (!rc && gradient) to_naive(*q, gs.data(), gradient);
-  return rc;
+  return hsba::guarded([&]() -> int {
+    if (!q || !x || !y_obs) return HSBA_ERR_INVALID;
+    if (zero_rotation(*q, x)) return HSBA_ERR_INVALID;
+    to_shared(*q, x, q->xs.data());
+    hsba_ycov_inverse w = {feature_cov_inverse, nullptr, nullptr};
+    std::vector<double> gs(gradient ? q->xs.size() : 0);
+    const int rc = hsba_build_normal(q->p, q->xs.data(), y_obs, &w, point_blocks, camera_blocks, nullptr,
+                                     gradient ? gs.data() : nullptr);
+    if (!rc && gradient) to_naive(*q, gs.data(), gradient);
+    return rc;
+  });
 }
 
 int hsba_naive_solve(hsba_naive_problem *q, double mu, double *delta_x) {
-  if (!q || !delta_x) return HSBA_ERR_INVALID;
-  std::vector<double> ds(q->xs.size());
-  const int rc = hsba_solve(q->p, mu, ds.data());
-  if (!rc) to_naive(*q, ds.data(), delta_x);
-  return rc;
+  return hsba::guarded([&]() -> int {
+    if (!q || !delta_x) return HSBA_ERR_INVALID;
+    std::vector<double> ds(q->xs.size());
+    const int rc = hsba_solve(q->p, mu, ds.data());
+    if (!rc) to_naive(*q, ds.data(), delta_x);
+    return rc;
+  });
 }
 
 int hsba_naive_lm_solve(hsba_naive_problem *q, const double *x0, const double *y_obs,
                         const double *feature_cov_inverse, const hsba_lm_options *lm, double *x_out,
                         hsba_lm_stats *stats) {
-  if (!q || !x0 || !y_obs || !lm || !x_out) return HSBA_ERR_INVALID;
-  if (zero_rotation(*q, x0)) return HSBA_ERR_INVALID;
-  to_shared(*q, x0, q->xs.data());
-  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, nullptr};
-  std::vector<double> xo(q->xs.size());
-  const int rc = hsba_lm_solve(q->p, q->xs.data(), y_obs, &w, lm, xo.data(), stats);
-  to_naive(*q, xo.data(), x_out);
-  return rc;
+  return hsba::guarded([&]() -> int {
+    if (!q || !x0 || !y_obs || !lm || !x_out) return HSBA_ERR_INVALID;
+    if (zero_rotation(*q, x0)) return HSBA_ERR_INVALID;
+    to_shared(*q, x0, q->xs.data());
+    hsba_ycov_inverse w = {feature_cov_inverse, nullptr, nullptr};
+    std::vector<double> xo(q->xs.size());
+    const int rc = hsba_lm_solve(q->p, q->xs.data(), y_obs, &w, lm, xo.data(), stats);
+    to_naive(*q, xo.data(), x_out);
+    return rc;
+  });
 }
 
 // ---- ba_gcp_constrained -------------------------------------------------
 
 int hsba_gcp_problem_create(const hsba_gcp_desc *desc, const hsba_options *options,
                             hsba_gcp_problem **problem) {
-  if (!desc || !problem || !desc->feature_maps || desc->num_cameras <= 0 || desc->num_points < 0 ||
-      desc->num_features < 0 || desc->num_gcps < 0 || desc->num_gcps > desc->num_points)
-    return HSBA_ERR_INVALID;
-  *problem = nullptr;
-  std::unique_ptr<hsba_gcp_problem> q(new hsba_gcp_problem());
-  q->C = desc->num_cameras;
-  q->P = desc->num_points;
-  q->F = desc->num_features;
-  q->G = desc->num_gcps;
-  const int rc = create_shared(*q, desc->feature_maps, options);
-  if (rc) return rc;
-  *problem = q.release();
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!desc || !problem || !desc->feature_maps || desc->num_cameras <= 0 || desc->num_points < 0 ||
+        desc->num_features < 0 || desc->num_gcps < 0 || desc->num_gcps > desc->num_points)
+      return HSBA_ERR_INVALID;
+    *problem = nullptr;
+    std::unique_ptr<hsba_gcp_problem> q(new hsba_gcp_problem());
+    q->C = desc->num_cameras;
+    q->P = desc->num_points;
+    q->F = desc->num_features;
+    q->G = desc->num_gcps;
+    const int rc = create_shared(*q, desc->feature_maps, options);
+    if (rc) return rc;
+    *problem = q.release();
+    return HSBA_OK;
+  });
 }
 
 int hsba_gcp_problem_destroy(hsba_gcp_problem *problem) {
-  delete problem;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    delete problem;
+    return HSBA_OK;
+  });
 }
 
 int hsba_gcp_evaluate(hsba_gcp_problem *q, const double *x, double *y) {
-  // camera_shared y = [keys 2F | point constraint rows], i.e. the GCP
-  // coordinates in GCP order
-  return hsba_naive_evaluate(q, x, y);
+  return hsba::guarded([&]() -> int {
+    // camera_shared y = [keys 2F | point constraint rows], i.e. the GCP
+    // coordinates in GCP order
+    return hsba_naive_evaluate(q, x, y);
+  });
 }
 
 int hsba_gcp_jacobian(hsba_gcp_problem *q, const double *x, double *camera_blocks, double *point_blocks) {
-  return hsba_naive_jacobian(q, x, camera_blocks, point_blocks);
+  return hsba::guarded([&]() -> int {
+    return hsba_naive_jacobian(q, x, camera_blocks, point_blocks);
+  });
 }
 
 int hsba_gcp_build_normal(hsba_gcp_problem *q, const double *x, const double *y_obs,
                           const double *feature_cov_inverse, const double *gcp_blocks, double *camera_blocks,
                           double *point_blocks, double *gradient) {
-  if (!q || !x || !y_obs) return HSBA_ERR_INVALID;
-  if (zero_rotation(*q, x)) return HSBA_ERR_INVALID;
-  to_shared(*q, x, q->xs.data());
-  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
-  std::vector<double> gs(gradient ? q->xs.size() : 0);
-  const int rc = hsba_build_normal(q->p, q->xs.data(), y_obs, &w, point_blocks, camera_blocks, nullptr,
-                                   gradient ? gs.data() : nullptr);
-  if (!rc && gradient) to_naive(*q, gs.data(), gradient);
-  return rc;
+  return hsba::guarded([&]() -> int {
+    if (!q || !x || !y_obs) return HSBA_ERR_INVALID;
+    if (zero_rotation(*q, x)) return HSBA_ERR_INVALID;
+    to_shared(*q, x, q->xs.data());
+    hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
+    std::vector<double> gs(gradient ? q->xs.size() : 0);
+    const int rc = hsba_build_normal(q->p, q->xs.data(), y_obs, &w, point_blocks, camera_blocks, nullptr,
+                                     gradient ? gs.data() : nullptr);
+    if (!rc && gradient) to_naive(*q, gs.data(), gradient);
+    return rc;
+  });
 }
 
 int hsba_gcp_solve(hsba_gcp_problem *q, double mu, double *delta_x) { return hsba_naive_solve(q, mu, delta_x); }
 
 int hsba_gcp_mahalanobis(hsba_gcp_problem *q, const double *x, const double *y_obs,
                          const double *feature_cov_inverse, const double *gcp_blocks, double *value) {
-  if (!q || !x || !y_obs || !value) return HSBA_ERR_INVALID;
-  to_shared(*q, x, q->xs.data());
-  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
-  return hsba_mahalanobis(q->p, q->xs.data(), y_obs, &w, value);
+  return hsba::guarded([&]() -> int {
+    if (!q || !x || !y_obs || !value) return HSBA_ERR_INVALID;
+    to_shared(*q, x, q->xs.data());
+    hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
+    return hsba_mahalanobis(q->p, q->xs.data(), y_obs, &w, value);
+  });
 }
 
 int hsba_gcp_lm_solve(hsba_gcp_problem *q, const double *x0, const double *y_obs,
                       const double *feature_cov_inverse, const double *gcp_blocks, const hsba_lm_options *lm,
                       double *x_out, hsba_lm_stats *stats) {
-  if (!q || !x0 || !y_obs || !lm || !x_out) return HSBA_ERR_INVALID;
-  if (zero_rotation(*q, x0)) return HSBA_ERR_INVALID;
-  to_shared(*q, x0, q->xs.data());
-  hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
-  std::vector<double> xo(q->xs.size());
-  const int rc = hsba_lm_solve(q->p, q->xs.data(), y_obs, &w, lm, xo.data(), stats);
-  to_naive(*q, xo.data(), x_out);
-  return rc;
+  return hsba::guarded([&]() -> int {
+    if (!q || !x0 || !y_obs || !lm || !x_out) return HSBA_ERR_INVALID;
+    if (zero_rotation(*q, x0)) return HSBA_ERR_INVALID;
+    to_shared(*q, x0, q->xs.data());
+    hsba_ycov_inverse w = {feature_cov_inverse, nullptr, gcp_blocks};
+    std::vector<double> xo(q->xs.size());
+    const int rc = hsba_lm_solve(q->p, q->xs.data(), y_obs, &w, lm, xo.data(), stats);
+    to_naive(*q, xo.data(), x_out);
+    return rc;
+  });
 }
 
 }  // extern "C"

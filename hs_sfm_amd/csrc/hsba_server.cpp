@@ -184,17 +184,25 @@ struct hsba_server {
     o.solver = HSBA_SOLVER_AUTO;
     o.loss = opt.loss;
     o.loss_scale = opt.loss_scale;
+    // rc is a failure of the group itself (allocation, device); a job's own
+    // failure (an unobserved image, mu overflow, NaN) is its window's status
+    // and leaves the other jobs of the group as if solved alone
     int rc = st ? HSBA_OK : HSBA_ERR_DEVICE;
     std::unique_ptr<hsba::EngineBase> eng;
     if (!rc) rc = hsba::create_engine(descs.data(), n, &o, st, &arena, eng);
     if (!rc) rc = eng->upload_x(x0s.data());
     if (!rc) rc = eng->upload_obs(ys.data(), ycovs.data());
-    if (!rc) rc = eng->lm(&opt.lm, stats.data());
+    if (!rc) rc = eng->lm_run(&opt.lm, stats.data());
     if (!rc) rc = eng->download_x(xouts.data());
     eng.reset();  // waits for the stream, returns the buffers to the arena
     for (int64_t i = 0; i < n; ++i) {
+      if (rc) {
+        std::memset(&stats[i], 0, sizeof(stats[i]));
+        stats[i].status = rc;
+        stats[i].stop_reason = HSBA_STOP_FAILED;
+      }
       g[i]->res.stats = stats[i];
-      g[i]->res.status = rc ? rc : stats[i].status;
+      g[i]->res.status = stats[i].status;
     }
   }
 
@@ -228,124 +236,135 @@ struct hsba_server {
 extern "C" {
 
 int hsba_server_create(const hsba_server_options *options, hsba_server **server) {
-  if (!server) return HSBA_ERR_INVALID;
-  *server = nullptr;
-  std::unique_ptr<hsba_server> s(new hsba_server());
-  std::memset(&s->opt, 0, sizeof(s->opt));
-  if (options) s->opt = *options;
-  hsba_server_options &o = s->opt;
-  if (o.precision != HSBA_PRECISION_FP64 && o.precision != HSBA_PRECISION_MIXED) return HSBA_ERR_INVALID;
-  if (o.loss != HSBA_LOSS_TRIVIAL && o.loss != HSBA_LOSS_HUBER) return HSBA_ERR_INVALID;
-  if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = int64_t(16) << 30;
-  if (o.max_active_jobs <= 0) o.max_active_jobs = 4096;
-  if (o.streams <= 0) o.streams = 4;
-  if (o.max_batch_jobs <= 0) o.max_batch_jobs = 256;
-  o.streams = std::min(o.streams, 16);
-  int rc = s->arena.init(o.hbm_budget_bytes + o.hbm_budget_bytes / 4);
-  if (rc) return rc;
-  s->t0 = Clock::now();
-  for (int i = 0; i < o.streams; ++i) s->workers.emplace_back([p = s.get()] { p->worker_loop(); });
-  *server = s.release();
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!server) return HSBA_ERR_INVALID;
+    *server = nullptr;
+    std::unique_ptr<hsba_server> s(new hsba_server());
+    std::memset(&s->opt, 0, sizeof(s->opt));
+    if (options) s->opt = *options;
+    hsba_server_options &o = s->opt;
+    if (o.precision != HSBA_PRECISION_FP64 && o.precision != HSBA_PRECISION_MIXED) return HSBA_ERR_INVALID;
+    if (o.loss != HSBA_LOSS_TRIVIAL && o.loss != HSBA_LOSS_HUBER) return HSBA_ERR_INVALID;
+    if (o.hbm_budget_bytes <= 0) o.hbm_budget_bytes = int64_t(16) << 30;
+    if (o.max_active_jobs <= 0) o.max_active_jobs = 4096;
+    if (o.streams <= 0) o.streams = 4;
+    if (o.max_batch_jobs <= 0) o.max_batch_jobs = 256;
+    o.streams = std::min(o.streams, 16);
+    int rc = s->arena.init(o.hbm_budget_bytes + o.hbm_budget_bytes / 4);
+    if (rc) return rc;
+    s->t0 = Clock::now();
+    for (int i = 0; i < o.streams; ++i) s->workers.emplace_back([p = s.get()] { p->worker_loop(); });
+    *server = s.release();
+    return HSBA_OK;
+  });
 }
 
 int hsba_server_destroy(hsba_server *s) {
-  if (!s) return HSBA_OK;
-  {
-    std::unique_lock<std::mutex> lk(s->mu);
-    s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
-    s->stop = true;
-  }
-  s->cv_work.notify_all();
-  for (auto &t : s->workers) t.join();
-  delete s;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!s) return HSBA_OK;
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+      s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (auto &t : s->workers) t.join();
+    delete s;
+    return HSBA_OK;
+  });
 }
 
 int hsba_server_submit(hsba_server *s, const hsba_problem_desc *desc, const double *x0, const double *y_obs,
                        const hsba_ycov_inverse *ycov, double *x_out, int64_t *job_id) {
-  if (!s || !desc || !x0 || !y_obs || !x_out) return HSBA_ERR_INVALID;
-  int64_t xn = 0, yn = 0;
-  int rc = hsba::problem_sizes(*desc, &xn, &yn);
-  if (rc) return rc;
-  if (!desc->feature_maps || (desc->num_images > 0 && !desc->image_camera_map)) return HSBA_ERR_INVALID;
-  std::unique_ptr<Job> j(new Job());
-  const hsba_problem_desc &d = *desc;
-  const uint32_t fix = d.fix_mask & 7u;
-  copy_in(j->fm, d.feature_maps, 2 * d.num_keys);
-  copy_in(j->icm, d.image_camera_map, d.num_images);
-  copy_in(j->intr, d.intrinsic_defaults, (int64_t)HSBA_NUM_INTRINSICS * d.num_cameras);
-  copy_in(j->fixp, (fix & HSBA_FIX_POINTS) ? d.fix_points : nullptr, 3 * d.num_points);
-  copy_in(j->fixi, (fix & HSBA_FIX_IMAGES) ? d.fix_images : nullptr, 6 * d.num_images);
-  {
-    int64_t xn_free = 0, dummy = 0;
-    hsba_problem_desc all_free = d;
-    all_free.fix_mask = 0;
-    (void)hsba::problem_sizes(all_free, &xn_free, &dummy);
-    const int64_t dcam = d.num_cameras > 0 ? (xn_free - 3 * d.num_points - 6 * d.num_images) / d.num_cameras : 0;
-    copy_in(j->fixc, (fix & HSBA_FIX_CAMERAS) ? d.fix_cameras : nullptr, dcam * d.num_cameras);
-  }
-  copy_in(j->pc, d.point_constraints, d.num_point_constraints);
-  copy_in(j->ic, d.image_constraints, d.num_image_constraints);
-  copy_in(j->cc, d.camera_constraints, d.num_camera_constraints);
-  copy_in(j->x0, x0, xn);
-  copy_in(j->y, y_obs, yn);
-  std::memset(&j->ycov, 0, sizeof(j->ycov));
-  if (ycov) {
-    copy_in(j->kb, ycov->key_blocks, 4 * d.num_keys);
-    copy_in(j->cw, ycov->constraint_weights, yn - 2 * d.num_keys);
-    j->ycov.key_blocks = j->kb.empty() ? nullptr : j->kb.data();
-    j->ycov.constraint_weights = j->cw.empty() ? nullptr : j->cw.data();
-    copy_in(j->pb, ycov->point_constraint_blocks, 9 * d.num_point_constraints);
-    j->ycov.point_constraint_blocks = j->pb.empty() ? nullptr : j->pb.data();
-  }
-  j->desc = d;
-  j->desc.feature_maps = j->fm.data();
-  j->desc.image_camera_map = j->icm.empty() ? nullptr : j->icm.data();
-  j->desc.intrinsic_defaults = j->intr.empty() ? nullptr : j->intr.data();
-  j->desc.fix_points = j->fixp.empty() ? nullptr : j->fixp.data();
-  j->desc.fix_images = j->fixi.empty() ? nullptr : j->fixi.data();
-  j->desc.fix_cameras = j->fixc.empty() ? nullptr : j->fixc.data();
-  j->desc.point_constraints = j->pc.empty() ? nullptr : j->pc.data();
-  j->desc.image_constraints = j->ic.empty() ? nullptr : j->ic.data();
-  j->desc.camera_constraints = j->cc.empty() ? nullptr : j->cc.data();
-  j->x_out = x_out;
-  const bool pcg = hsba::auto_pcg(d);
-  j->bytes = hsba::estimate_device_bytes(d, s->opt.precision, pcg, !j->kb.empty());
-  j->cls = std::to_string(fix) + "/" + std::to_string(d.intrinsic_computations_mask & 7u) + "/" +
-           std::to_string(d.intrinsic_free_mask) + "/" + (pcg ? "pcg" : "dense");
-  std::memset(&j->res, 0, sizeof(j->res));
-  if (j->bytes > s->opt.hbm_budget_bytes) return HSBA_ERR_ADMISSION;
-  {
-    std::lock_guard<std::mutex> g(s->mu);
-    j->id = s->next_id++;
-    j->res.job_id = j->id;
-    j->res.submit_time_s = s->now();
-    if (job_id) *job_id = j->id;
-    s->queue.push_back(std::move(j));
-    ++s->outstanding;
-  }
-  s->cv_work.notify_one();
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!s || !desc || !x0 || !y_obs || !x_out) return HSBA_ERR_INVALID;
+    int64_t xn = 0, yn = 0;
+    int rc = hsba::problem_sizes(*desc, &xn, &yn);
+    if (rc) return rc;
+    // a malformed job is refused here, so it never reaches (and fails) a batch
+    if ((rc = hsba::validate_desc(*desc, nullptr))) return rc;
+    std::unique_ptr<Job> j(new Job());
+    const hsba_problem_desc &d = *desc;
+    const uint32_t fix = d.fix_mask & 7u;
+    copy_in(j->fm, d.feature_maps, 2 * d.num_keys);
+    copy_in(j->icm, d.image_camera_map, d.num_images);
+    copy_in(j->intr, d.intrinsic_defaults, (int64_t)HSBA_NUM_INTRINSICS * d.num_cameras);
+    copy_in(j->fixp, (fix & HSBA_FIX_POINTS) ? d.fix_points : nullptr, 3 * d.num_points);
+    copy_in(j->fixi, (fix & HSBA_FIX_IMAGES) ? d.fix_images : nullptr, 6 * d.num_images);
+    {
+      int64_t xn_free = 0, dummy = 0;
+      hsba_problem_desc all_free = d;
+      all_free.fix_mask = 0;
+      (void)hsba::problem_sizes(all_free, &xn_free, &dummy);
+      const int64_t dcam = d.num_cameras > 0 ? (xn_free - 3 * d.num_points - 6 * d.num_images) / d.num_cameras : 0;
+      copy_in(j->fixc, (fix & HSBA_FIX_CAMERAS) ? d.fix_cameras : nullptr, dcam * d.num_cameras);
+    }
+    copy_in(j->pc, d.point_constraints, d.num_point_constraints);
+    copy_in(j->ic, d.image_constraints, d.num_image_constraints);
+    copy_in(j->cc, d.camera_constraints, d.num_camera_constraints);
+    copy_in(j->x0, x0, xn);
+    copy_in(j->y, y_obs, yn);
+    std::memset(&j->ycov, 0, sizeof(j->ycov));
+    if (ycov) {
+      copy_in(j->kb, ycov->key_blocks, 4 * d.num_keys);
+      copy_in(j->cw, ycov->constraint_weights, yn - 2 * d.num_keys);
+      j->ycov.key_blocks = j->kb.empty() ? nullptr : j->kb.data();
+      j->ycov.constraint_weights = j->cw.empty() ? nullptr : j->cw.data();
+      copy_in(j->pb, ycov->point_constraint_blocks, 9 * d.num_point_constraints);
+      j->ycov.point_constraint_blocks = j->pb.empty() ? nullptr : j->pb.data();
+    }
+    j->desc = d;
+    j->desc.feature_maps = j->fm.data();
+    j->desc.image_camera_map = j->icm.empty() ? nullptr : j->icm.data();
+    j->desc.intrinsic_defaults = j->intr.empty() ? nullptr : j->intr.data();
+    j->desc.fix_points = j->fixp.empty() ? nullptr : j->fixp.data();
+    j->desc.fix_images = j->fixi.empty() ? nullptr : j->fixi.data();
+    j->desc.fix_cameras = j->fixc.empty() ? nullptr : j->fixc.data();
+    j->desc.point_constraints = j->pc.empty() ? nullptr : j->pc.data();
+    j->desc.image_constraints = j->ic.empty() ? nullptr : j->ic.data();
+    j->desc.camera_constraints = j->cc.empty() ? nullptr : j->cc.data();
+    j->x_out = x_out;
+    const bool pcg = hsba::auto_pcg(d);
+    j->bytes = hsba::estimate_device_bytes(d, s->opt.precision, pcg, !j->kb.empty());
+    j->cls = std::to_string(fix) + "/" + std::to_string(d.intrinsic_computations_mask & 7u) + "/" +
+             std::to_string(d.intrinsic_free_mask) + "/" + (pcg ? "pcg" : "dense");
+    std::memset(&j->res, 0, sizeof(j->res));
+    if (j->bytes > s->opt.hbm_budget_bytes) return HSBA_ERR_ADMISSION;
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      j->id = s->next_id++;
+      j->res.job_id = j->id;
+      j->res.submit_time_s = s->now();
+      if (job_id) *job_id = j->id;
+      s->queue.push_back(std::move(j));
+      ++s->outstanding;
+    }
+    s->cv_work.notify_one();
+    return HSBA_OK;
+  });
 }
 
 int hsba_server_poll(hsba_server *s, hsba_job_result *results, int32_t max_results, int32_t *count) {
-  if (!s || !count || (max_results > 0 && !results)) return HSBA_ERR_INVALID;
-  std::lock_guard<std::mutex> g(s->mu);
-  int32_t n = 0;
-  while (n < max_results && !s->done.empty()) {
-    results[n++] = s->done.front()->res;
-    s->done.pop_front();
-  }
-  *count = n;
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!s || !count || (max_results > 0 && !results)) return HSBA_ERR_INVALID;
+    std::lock_guard<std::mutex> g(s->mu);
+    int32_t n = 0;
+    while (n < max_results && !s->done.empty()) {
+      results[n++] = s->done.front()->res;
+      s->done.pop_front();
+    }
+    *count = n;
+    return HSBA_OK;
+  });
 }
 
 int hsba_server_drain(hsba_server *s) {
-  if (!s) return HSBA_ERR_INVALID;
-  std::unique_lock<std::mutex> lk(s->mu);
-  s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
-  return HSBA_OK;
+  return hsba::guarded([&]() -> int {
+    if (!s) return HSBA_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+    return HSBA_OK;
+  });
 }
 
 double hsba_server_clock(const hsba_server *s) { return s ? s->now() : 0.0; }
