@@ -51,6 +51,12 @@ class hsba_problem_desc(C.Structure):
     ]
 
 
+class hsba_structure_sizes(C.Structure):
+    _fields_ = [("num_keys", C.c_int64), ("num_points", C.c_int64), ("num_point_image", C.c_int64),
+                ("num_point_camera", C.c_int64), ("num_image_camera", C.c_int64),
+                ("num_pairs", C.c_int64), ("num_incidences", C.c_int64)]
+
+
 class hsba_ycov_inverse(C.Structure):
     _fields_ = [("key_blocks", _dp), ("constraint_weights", _dp),
                 ("point_constraint_blocks", _dp)]
@@ -127,6 +133,13 @@ def dptr(a: Optional[np.ndarray]):
         return C.cast(None, _dp)
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "float64 C-contiguous required"
     return a.ctypes.data_as(_dp)
+
+
+def iptr(a: Optional[np.ndarray]):
+    if a is None:
+        return C.cast(None, _ip)
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], "int32 C-contiguous required"
+    return a.ctypes.data_as(_ip)
 
 
 class Desc:
@@ -247,6 +260,9 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.hsba_gcp_mahalanobis.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
     lib.hsba_gcp_lm_solve.argtypes = [P, _dp, _dp, _dp, _dp, C.POINTER(hsba_lm_options), _dp,
                                       C.POINTER(hsba_lm_stats)]
+    lib.hsba_problem_structure.argtypes = [P, C.POINTER(hsba_structure_sizes), _ip, _ip, _ip, _ip, _ip,
+                                           _ip, C.POINTER(C.c_int64), _ip]
+    lib.hsba_coupling_blocks.argtypes = [P, _dp, _dp, _dp]
     lib.hsba_covariance.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
     lib.hsba_scene_load_sfm.argtypes = [C.POINTER(hsba_sfm_files), C.c_uint32, C.c_uint32, C.POINTER(P)]
     lib.hsba_scene_load_bal.argtypes = [C.c_char_p, C.POINTER(P)]
@@ -261,7 +277,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 EXPORTED_SYMBOLS = [
     "hsba_version", "hsba_device_count", "hsba_problem_create", "hsba_problem_destroy",
     "hsba_problem_sizes", "hsba_evaluate", "hsba_jacobian", "hsba_build_normal",
-    "hsba_solve", "hsba_covariance", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
+    "hsba_solve", "hsba_covariance", "hsba_problem_structure", "hsba_coupling_blocks", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
     "hsba_gradient_dot", "hsba_mahalanobis", "hsba_lm_solve", "hsba_lm_begin",
     "hsba_lm_iterate", "hsba_lm_end", "hsba_batch_create",
     "hsba_batch_destroy", "hsba_batch_lm_solve", "hsba_server_create",

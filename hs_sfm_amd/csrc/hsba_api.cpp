@@ -225,6 +225,23 @@ int hsba_covariance(hsba_problem *p, const double *x, const double *y_obs, const
   });
 }
 
+int hsba_problem_structure(const hsba_problem *p, hsba_structure_sizes *sizes, int32_t *key_order,
+                           int32_t *point_ptr, int32_t *point_image, int32_t *point_camera,
+                           int32_t *image_camera, int32_t *pairs, int64_t *pair_ptr, int32_t *incidences) {
+  return hsba::guarded([&]() -> int {
+    if (!p || !sizes) return HSBA_ERR_INVALID;
+    return hsba::export_structure(p->eng->L, sizes, key_order, point_ptr, point_image, point_camera,
+                                  image_camera, pairs, pair_ptr, incidences);
+  });
+}
+
+int hsba_coupling_blocks(hsba_problem *p, double *point_image, double *point_camera, double *image_camera) {
+  return hsba::guarded([&]() -> int {
+    if (!p) return HSBA_ERR_INVALID;
+    return p->eng->coupling(point_image, point_camera, image_camera);
+  });
+}
+
 int hsba_solve(hsba_problem *p, double mu, double *delta_x) {
   return hsba::guarded([&]() -> int {
     if (!p || !delta_x) return HSBA_ERR_INVALID;
@@ -356,6 +373,33 @@ int hsba_batch_lm_solve(hsba_batch *b, const double *const *x0s, const double *c
     if (rc) return rc;
     if ((rc = b->eng->upload_obs(y_obs, ycovs))) return rc;
     rc = b->eng->lm(lm, stats);
+    const int rc2 = b->eng->download_x(x_outs);
+    return rc ? rc : rc2;
+  });
+}
+
+int hsba_batch_lm_begin(hsba_batch *b, const double *const *x0s, const double *const *y_obs,
+                        const hsba_ycov_inverse *ycovs, const hsba_lm_options *lm) {
+  return hsba::guarded([&]() -> int {
+    if (!b || !x0s || !y_obs || !lm) return HSBA_ERR_INVALID;
+    int rc = b->eng->upload_x(x0s);
+    if (rc) return rc;
+    if ((rc = b->eng->upload_obs(y_obs, ycovs))) return rc;
+    return b->eng->lm_begin(lm);
+  });
+}
+
+int hsba_batch_lm_iterate(hsba_batch *b, int32_t iterations, hsba_stage_times *times) {
+  return hsba::guarded([&]() -> int {
+    if (!b) return HSBA_ERR_INVALID;
+    return b->eng->lm_iterate(iterations, times);
+  });
+}
+
+int hsba_batch_lm_end(hsba_batch *b, double *const *x_outs, hsba_lm_stats *stats) {
+  return hsba::guarded([&]() -> int {
+    if (!b || !x_outs || !stats) return HSBA_ERR_INVALID;
+    const int rc = b->eng->lm_end(stats);
     const int rc2 = b->eng->download_x(x_outs);
     return rc ? rc : rc2;
   });

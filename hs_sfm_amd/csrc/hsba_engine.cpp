@@ -721,6 +721,57 @@ struct Engine final : EngineBase {
     return HSBA_OK;
   }
 
+  // Coupling blocks of the last build in the reference's block order:
+  // W_pi per key, W_pc per distinct (point, camera), W_ic per distinct
+  // (image, camera).  Device W_k = J_p^T Lambda [J_i | J_c] (3 x E per
+  // point-major key) and D_i (whose image-camera block is W_ic) are read back.
+  int coupling(double *pi, double *pc, double *ic) override {
+    if (!built) return HSBA_ERR_GENERIC;
+    const int E = L.E, IMG = L.IMG, D = L.D;
+    hsba_structure_sizes sz;
+    int rc = export_structure(L, &sz, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    std::vector<int32_t> pcs(2 * sz.num_point_camera), ics(2 * sz.num_image_camera);
+    if ((rc = export_structure(L, nullptr, nullptr, nullptr, nullptr, pcs.data(), ics.data(), nullptr, nullptr,
+                               nullptr)))
+      return rc;
+    std::vector<T> hW, hD;
+    if (L.pts_free && E > 0 && L.N > 0 && (pi || pc)) {
+      hW.resize((size_t)3 * E * L.N);
+      HCHK(hipMemcpyAsync(hW.data(), d.W, sizeof(T) * hW.size(), hipMemcpyDeviceToHost, st));
+    }
+    if (ic && sz.num_image_camera > 0) {
+      hD.resize((size_t)E * E * L.I);
+      HCHK(hipMemcpyAsync(hD.data(), d.Dimg, sizeof(T) * hD.size(), hipMemcpyDeviceToHost, st));
+    }
+    HCHK(hipStreamSynchronize(st));
+    if (pi && sz.num_point_image > 0)
+      for (int64_t s = 0; s < L.N; ++s) {
+        const int64_t k = L.key_orig[s];
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 6; ++b) pi[18 * k + 6 * a + b] = (double)hW[(size_t)(3 * s + a) * E + b];
+      }
+    if (pc && sz.num_point_camera > 0) {
+      for (int64_t q = 0; q < sz.num_point_camera; ++q) {
+        const int32_t p = pcs[2 * q], c = pcs[2 * q + 1];
+        double *o = pc + (size_t)q * 3 * D;
+        for (int j = 0; j < 3 * D; ++j) o[j] = 0.0;
+        for (int32_t s = L.pt_ptr[p]; s < L.pt_ptr[p + 1]; ++s) {
+          if (L.img_cam[L.key_img[s]] != c) continue;
+          for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < D; ++b) o[a * D + b] += (double)hW[(size_t)(3 * s + a) * E + IMG + b];
+        }
+      }
+    }
+    if (ic && sz.num_image_camera > 0)
+      for (int64_t q = 0; q < sz.num_image_camera; ++q) {
+        const int32_t im = ics[2 * q];
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < D; ++b) ic[(q * 6 + a) * D + b] = (double)hD[(size_t)im * E * E + a * E + IMG + b];
+      }
+    return HSBA_OK;
+  }
+
   // One damped Newton step on the resident normal equations (window 0):
   // CameraSharedNormalEquationSolver::operator().  S / rhs, when given,
   // receive the reduced system instead of (or besides) the step.

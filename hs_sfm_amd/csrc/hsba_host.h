@@ -62,6 +62,10 @@ int64_t layout_device_bytes(const Layout &L, int precision);
 int64_t estimate_device_bytes(const hsba_problem_desc &desc, int precision, bool pcg, bool has_cov);
 // x and y lengths of one problem in the reference layouts (before validation).
 int problem_sizes(const hsba_problem_desc &desc, int64_t *x_size, int64_t *y_size);
+// Index structure of window 0 (hsba_problem_structure).
+int export_structure(const Layout &L, hsba_structure_sizes *sizes, int32_t *key_order, int32_t *point_ptr,
+                     int32_t *point_image, int32_t *point_camera, int32_t *image_camera, int32_t *pairs,
+                     int64_t *pair_ptr, int32_t *incidences);
 // The checks build_layout makes on one problem alone (sizes, id ranges,
 // duplicate keys, constraint records), without building anything: lets the
 // serving loop refuse a malformed job at submit instead of failing its batch.
@@ -128,6 +132,8 @@ struct EngineBase {
   virtual int download_normal(double *V, double *Ui, double *Uc, double *g) = 0;
   virtual int solve(double mu, double *delta, double *S, double *rhs) = 0;
   virtual int covariance(double *Vp, double *Ui, double *Uc, double *reduced) = 0;
+  // coupling blocks W_pi / W_pc / W_ic of the last build (window 0)
+  virtual int coupling(double *pi, double *pc, double *ic) = 0;
   virtual int evaluate(double *y) = 0;
   virtual int jacobian(double *jp, double *ji, double *jc) = 0;
   virtual int scalars(double *maxdiag, double *ginf) = 0;

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <unordered_map>
 
 #include "hsba_host.h"
 
@@ -471,6 +472,79 @@ int64_t layout_device_bytes(const Layout &L, int precision) {
   b += (int64_t)L.chunk_kind.size() * (16 + 32) + (int64_t)L.nwin * (sizeof(WinInfo) + sizeof(WinState));
   b += (int64_t)L.ich_img.size() * (12 + t * (E * (E + 1) / 2 + E)) + (L.I + 1) * 4;  // image-key chunks
   return b;
+}
+
+int export_structure(const Layout &L, hsba_structure_sizes *sz, int32_t *key_order, int32_t *point_ptr,
+                     int32_t *point_image, int32_t *point_camera, int32_t *image_camera, int32_t *pairs,
+                     int64_t *pair_ptr, int32_t *incidences) {
+  // window 0 of the layout (a single problem: every offset is zero)
+  const HostWindow &h = L.hw[0];
+  const int64_t N = h.N, P = h.P;
+  std::vector<int32_t> slot_of(N);
+  for (int64_t s = 0; s < N; ++s) slot_of[L.key_orig[s]] = (int32_t)s;
+  const bool has_pi = L.pts_free && L.imgs_free;
+  const bool has_pc = L.pts_free && L.cams_free && L.D > 0;
+  const bool has_ic = L.imgs_free && L.cams_free && L.D > 0;
+  // first appearance in key order of each distinct (point, camera) and
+  // (image, camera): the reference's std::set insertion numbering
+  std::vector<int32_t> pc, ic;
+  if (has_pc) {
+    std::unordered_map<uint64_t, int64_t> seen;
+    seen.reserve((size_t)N * 2);
+    for (int64_t k = 0; k < N; ++k) {
+      const int32_t s = slot_of[k];
+      const int32_t p = L.key_pt[s], c = L.img_cam[L.key_img[s]];
+      if (seen.emplace(((uint64_t)(uint32_t)p << 32) | (uint32_t)c, (int64_t)pc.size() / 2).second) {
+        pc.push_back(p);
+        pc.push_back(c);
+      }
+    }
+  }
+  if (has_ic) {
+    std::vector<char> seen(h.I, 0);
+    for (int64_t k = 0; k < N; ++k) {
+      const int32_t im = L.key_img[slot_of[k]];
+      if (!seen[im]) {
+        seen[im] = 1;
+        ic.push_back(im);
+        ic.push_back(L.img_cam[im]);
+      }
+    }
+  }
+  if (sz) {
+    sz->num_keys = N;
+    sz->num_points = P;
+    sz->num_point_image = has_pi ? N : 0;
+    sz->num_point_camera = (int64_t)pc.size() / 2;
+    sz->num_image_camera = (int64_t)ic.size() / 2;
+    sz->num_pairs = L.npairs;
+    sz->num_incidences = L.incidences;
+  }
+  if (key_order) std::copy(L.key_orig.begin(), L.key_orig.begin() + N, key_order);
+  if (point_ptr) std::copy(L.pt_ptr.begin(), L.pt_ptr.begin() + P + 1, point_ptr);
+  if (point_image && has_pi)
+    for (int64_t k = 0; k < N; ++k) {
+      point_image[2 * k] = L.key_pt[slot_of[k]];
+      point_image[2 * k + 1] = L.key_img[slot_of[k]];
+    }
+  if (point_camera) std::copy(pc.begin(), pc.end(), point_camera);
+  if (image_camera) std::copy(ic.begin(), ic.end(), image_camera);
+  for (int64_t q = 0; q < L.npairs; ++q) {
+    const int64_t b = L.pair_ptr[q], e = L.pair_ptr[q + 1];
+    if (pairs) {
+      // both images of a pair from its first incidence (every pair has one)
+      pairs[2 * q] = L.key_img[inc_a(L.inc[b])];
+      pairs[2 * q + 1] = L.key_img[inc_b(L.inc[b])];
+    }
+    if (pair_ptr) pair_ptr[q] = b;
+    if (incidences)
+      for (int64_t j = b; j < e; ++j) {
+        incidences[2 * j] = L.key_orig[inc_a(L.inc[j])];
+        incidences[2 * j + 1] = L.key_orig[inc_b(L.inc[j])];
+      }
+  }
+  if (pair_ptr) pair_ptr[L.npairs] = L.incidences;
+  return HSBA_OK;
 }
 
 int validate_desc(const hsba_problem_desc &ds, std::string *why) {

@@ -119,6 +119,36 @@ class Problem:
         rc = self.lib.hsba_reduced_system(self.h, mu, abi.dptr(S), abi.dptr(rhs))
         return rc, S, rhs
 
+    def structure(self):
+        """hsba_problem_structure: the index structure as numpy arrays."""
+        sz = abi.hsba_structure_sizes()
+        abi.check(self.lib.hsba_problem_structure(self.h, C.byref(sz), *([abi.iptr(None)] * 6),
+                                                  C.cast(None, C.POINTER(C.c_int64)), abi.iptr(None)),
+                  "hsba_problem_structure")
+        out = dict(key_order=np.zeros(sz.num_keys, np.int32), point_ptr=np.zeros(sz.num_points + 1, np.int32),
+                   point_image=np.zeros((sz.num_point_image, 2), np.int32),
+                   point_camera=np.zeros((sz.num_point_camera, 2), np.int32),
+                   image_camera=np.zeros((sz.num_image_camera, 2), np.int32),
+                   pairs=np.zeros((sz.num_pairs, 2), np.int32), pair_ptr=np.zeros(sz.num_pairs + 1, np.int64),
+                   incidences=np.zeros((sz.num_incidences, 2), np.int32))
+        abi.check(self.lib.hsba_problem_structure(
+            self.h, C.byref(sz), abi.iptr(out["key_order"]), abi.iptr(out["point_ptr"]),
+            abi.iptr(out["point_image"]), abi.iptr(out["point_camera"]), abi.iptr(out["image_camera"]),
+            abi.iptr(out["pairs"]), out["pair_ptr"].ctypes.data_as(C.POINTER(C.c_int64)),
+            abi.iptr(out["incidences"])), "hsba_problem_structure")
+        return out
+
+    def coupling_blocks(self):
+        """hsba_coupling_blocks of the last build: W_pi, W_pc, W_ic."""
+        st = self.structure()
+        d = self.cam_d
+        Wpi = np.zeros((len(st["point_image"]), 3, 6))
+        Wpc = np.zeros((len(st["point_camera"]), 3, max(d, 1)))
+        Wic = np.zeros((len(st["image_camera"]), 6, max(d, 1)))
+        abi.check(self.lib.hsba_coupling_blocks(self.h, abi.dptr(Wpi), abi.dptr(Wpc), abi.dptr(Wic)),
+                  "hsba_coupling_blocks")
+        return Wpi, Wpc[:, :, :d], Wic[:, :, :d]
+
     def max_diagonal(self):
         v = C.c_double()
         abi.check(self.lib.hsba_max_diagonal(self.h, C.byref(v)), "hsba_max_diagonal")

@@ -221,6 +221,58 @@ int hsba_build_normal(hsba_problem *problem, const double *x,
                       double *point_blocks, double *image_blocks,
                       double *camera_blocks, double *gradient);
 
+/* Sparsity of the normal equations and of the reduced camera system, as built
+ * at hsba_problem_create (host memory only; every index is exact).
+ *
+ * Coupling sets of N in the reference's block order
+ * (CameraSharedNormalEquationBuilder::ComputeNormalMatrix,
+ * camera_shared_normal_equation_builder.hpp:137-205; block storage
+ * camera_shared_normal_matrix.hpp:315-443):
+ *   point_image  [num_point_image][2]  (point, image) of every key, in key
+ *                order (block id = key + 1, :141-153); present when points
+ *                and images are free
+ *   point_camera [num_point_camera][2] distinct (point, camera), in the order
+ *                in which they first appear in key order (the std::set
+ *                insertion with value size()+1, :157-179)
+ *   image_camera [num_image_camera][2] distinct (image, camera), likewise
+ *                (:180-204)
+ * Device layout of the keys and of the Schur complement (no reference
+ * counterpart; the reference loops over all image pairs and points):
+ *   key_order    [num_keys]      caller key index of each point-major slot
+ *                                (a point's keys contiguous, images ascending)
+ *   point_ptr    [num_points+1]  slots of point p: point_ptr[p] .. point_ptr[p+1]
+ *   pairs        [num_pairs][2]  image pairs (a <= b) sharing a point, in
+ *                                (a, b) order: the nonzero image blocks S_ab
+ *                                of camera_shared_normal_equation_solver.hpp:198-268
+ *   pair_ptr     [num_pairs+1]   incidences of pair q: pair_ptr[q] .. pair_ptr[q+1]
+ *   incidences   [num_incidences][2]  (caller key on image a, caller key on
+ *                                image b) of every point seeing both, in point order
+ * Any array may be NULL; sizes is always filled.  Under the PCG solver only
+ * the diagonal pairs (a, a) exist. */
+typedef struct {
+  int64_t num_keys;
+  int64_t num_points;
+  int64_t num_point_image;
+  int64_t num_point_camera;
+  int64_t num_image_camera;
+  int64_t num_pairs;
+  int64_t num_incidences;
+} hsba_structure_sizes;
+int hsba_problem_structure(const hsba_problem *problem, hsba_structure_sizes *sizes,
+                           int32_t *key_order, int32_t *point_ptr, int32_t *point_image,
+                           int32_t *point_camera, int32_t *image_camera, int32_t *pairs,
+                           int64_t *pair_ptr, int32_t *incidences);
+
+/* Coupling blocks of N at the last hsba_build_normal, in the order of the
+ * sets of hsba_problem_structure (camera_shared_normal_equation_builder.hpp:249-307):
+ *   point_image  [num_point_image][3][6]   W_pi = J_p^T Sigma^-1 J_i of each key
+ *   point_camera [num_point_camera][3][d]  W_pc summed over the point's keys on
+ *                                          the camera's images
+ *   image_camera [num_image_camera][6][d]  W_ic summed over the image's keys
+ * Any pointer may be NULL. */
+int hsba_coupling_blocks(hsba_problem *problem, double *point_image, double *point_camera,
+                         double *image_camera);
+
 /* CameraSharedNormalEquationSolver::operator() on the normal equations of the
  * last hsba_build_normal, damped by mu (camera_shared_normal_equation_solver.hpp:75-114). */
 int hsba_solve(hsba_problem *problem, double mu, double *delta_x);
@@ -455,6 +507,14 @@ int hsba_batch_lm_solve(hsba_batch *batch, const double *const *x0s,
                         const hsba_ycov_inverse *ycovs, /* [count] or NULL */
                         const hsba_lm_options *lm,
                         double *const *x_outs, hsba_lm_stats *stats /* [count] */);
+
+/* Stepwise form of hsba_batch_lm_solve (as hsba_lm_begin / _iterate / _end
+ * for one problem): per-stage device times of the whole batch, for timing.
+ * No reference counterpart. */
+int hsba_batch_lm_begin(hsba_batch *batch, const double *const *x0s, const double *const *y_obs,
+                        const hsba_ycov_inverse *ycovs, const hsba_lm_options *lm);
+int hsba_batch_lm_iterate(hsba_batch *batch, int32_t iterations, hsba_stage_times *times);
+int hsba_batch_lm_end(hsba_batch *batch, double *const *x_outs, hsba_lm_stats *stats);
 
 /* ---- serving loop (continuous batching) -------------------------------- */
 typedef struct hsba_server hsba_server;

@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <set>
 #include <vector>
 
 #include "../include/hsba.h"
@@ -935,6 +936,63 @@ double inf_norm(const OProblem &o) {
   return m;
 }
 
+// ---- sparsity of N and of the reduced system --------------------------------
+// Coupling sets in the reference's block order
+// (BA/camera_shared_normal_equation_builder.hpp:137-205): every key adds a
+// (point, image) triplet whose block id is the triplet count + 1 (:141-153);
+// (point, camera) and (image, camera) triplets go through a std::set ordered
+// by (row, col) (comparator :66-69), inserted with id size() + 1, so a pair
+// seen again keeps the id of its first insertion (:157-204).  Blocks are
+// stored at id - 1 (BA/camera_shared_normal_matrix.hpp:315-362).
+struct OTriplet {
+  int64_t row, col, value;
+  bool operator<(const OTriplet &b) const { return row < b.row || (!(b.row < row) && col < b.col); }
+};
+
+// (row, col) of every block, indexed by block id - 1.
+void coupling_set(const OProblem &o, int which, std::vector<int64_t> &rc) {
+  rc.clear();
+  if (which == 0) {
+    if (!o.pts_free || !o.imgs_free) return;
+    for (int64_t k = 0; k < o.N; ++k) { rc.push_back(o.key_pt[k]); rc.push_back(o.key_img[k]); }
+    return;
+  }
+  if (!o.cams_free || o.d == 0 || (which == 1 ? !o.pts_free : !o.imgs_free)) return;
+  std::set<OTriplet> set;
+  for (int64_t k = 0; k < o.N; ++k) {
+    const int64_t row = which == 1 ? o.key_pt[k] : o.key_img[k];
+    set.insert(OTriplet{row, (int64_t)o.icm[o.key_img[k]], (int64_t)set.size() + 1});
+  }
+  rc.assign(2 * set.size(), 0);
+  for (const OTriplet &t : set) { rc[2 * (t.value - 1)] = t.row; rc[2 * (t.value - 1) + 1] = t.col; }
+}
+
+// Image pairs (a <= b) that share a point and, per pair, the (key on a, key
+// on b) of every point seeing both, points in id order.  These are the image
+// blocks S_ab that the reference's Schur loops emit (the `exist` rule of
+// BA/camera_shared_normal_equation_solver.hpp:200-268: a block exists when
+// some point couples both images).
+void pair_structure(const OProblem &o, std::vector<int64_t> &pairs, std::vector<int64_t> &ptr,
+                    std::vector<int64_t> &inc) {
+  std::map<std::pair<int64_t, int64_t>, std::vector<int64_t>> m;
+  for (int64_t p = 0; p < o.P; ++p)
+    for (int64_t qa = o.pt_begin[p]; qa < o.pt_begin[p + 1]; ++qa)
+      for (int64_t qb = o.pt_begin[p]; qb < o.pt_begin[p + 1]; ++qb) {
+        const int64_t ka = o.pt_keys[qa], kb = o.pt_keys[qb];
+        if (o.key_img[ka] > o.key_img[kb]) continue;
+        std::vector<int64_t> &v = m[{o.key_img[ka], o.key_img[kb]}];
+        v.push_back(ka);
+        v.push_back(kb);
+      }
+  pairs.clear(); ptr.assign(1, 0); inc.clear();
+  for (const auto &e : m) {
+    pairs.push_back(e.first.first);
+    pairs.push_back(e.first.second);
+    inc.insert(inc.end(), e.second.begin(), e.second.end());
+    ptr.push_back((int64_t)inc.size() / 2);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1014,6 +1072,58 @@ int oracle_build_normal(void *h, const double *x, const double *y_obs, const dou
   if (Wpc) std::copy(o.Wpc.begin(), o.Wpc.end(), Wpc);
   if (Wic) std::copy(o.Wic.begin(), o.Wic.end(), Wic);
   if (g) std::copy(o.g.begin(), o.g.end(), g);
+  return 0;
+}
+
+// Coupling sets and blocks of the last build (block-id order); counts first
+// (any array pointer may be null).  which: 0 point-image, 1 point-camera,
+// 2 image-camera.  Blocks: [n][3][6], [n][3][d], [n][6][d].
+int oracle_coupling(void *h, int which, int64_t *count, int32_t *rc_out, double *blocks) {
+  OProblem &o = *static_cast<OProblem *>(h);
+  std::vector<int64_t> rc;
+  coupling_set(o, which, rc);
+  const int64_t n = (int64_t)rc.size() / 2;
+  *count = n;
+  if (rc_out)
+    for (int64_t j = 0; j < 2 * n; ++j) rc_out[j] = (int32_t)rc[j];
+  if (!blocks || n == 0) return 0;
+  if (!o.built) return HSBA_ERR_GENERIC;
+  const int d = o.d;
+  if (which == 0) {
+    std::copy(o.Wpi.begin(), o.Wpi.end(), blocks);
+    return 0;
+  }
+  // AddPointCameraBlock / AddImageCameraBlock (BA/camera_shared_normal_matrix.hpp:335-366):
+  // every key's block is added at its pair's id
+  std::map<std::pair<int64_t, int64_t>, int64_t> id;
+  for (int64_t q = 0; q < n; ++q) id[{rc[2 * q], rc[2 * q + 1]}] = q;
+  const int rows = which == 1 ? 3 : 6;
+  std::fill(blocks, blocks + n * rows * d, 0.0);
+  if (which == 1) {
+    for (int64_t k = 0; k < o.N; ++k) {
+      double *b = blocks + id[{o.key_pt[k], (int64_t)o.icm[o.key_img[k]]}] * 3 * d;
+      for (int j = 0; j < 3 * d; ++j) b[j] += o.Wpc[3 * d * k + j];
+    }
+  } else {
+    // the builder's per-image sums (one camera per image)
+    for (int64_t q = 0; q < n; ++q)
+      std::copy(o.Wic.begin() + 6 * d * rc[2 * q], o.Wic.begin() + 6 * d * (rc[2 * q] + 1), blocks + q * 6 * d);
+  }
+  return 0;
+}
+
+// Pair structure of the reduced system: counts first, then the arrays
+// (pairs [n][2], ptr [n+1], incidences [m][2] as caller key indices).
+int oracle_pairs(void *h, int64_t *num_pairs, int64_t *num_inc, int32_t *pairs, int64_t *ptr, int32_t *inc) {
+  OProblem &o = *static_cast<OProblem *>(h);
+  std::vector<int64_t> pr, pp, ic;
+  if (o.pts_free && (o.imgs_free || (o.cams_free && o.d > 0))) pair_structure(o, pr, pp, ic);
+  else pp.assign(1, 0);
+  *num_pairs = (int64_t)pr.size() / 2;
+  *num_inc = (int64_t)ic.size() / 2;
+  if (pairs) for (size_t j = 0; j < pr.size(); ++j) pairs[j] = (int32_t)pr[j];
+  if (ptr) std::copy(pp.begin(), pp.end(), ptr);
+  if (inc) for (size_t j = 0; j < ic.size(); ++j) inc[j] = (int32_t)ic[j];
   return 0;
 }
 

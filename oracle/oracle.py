@@ -56,6 +56,9 @@ def lib():
         L.oracle_mahalanobis.restype = C.c_double
         L.oracle_lm.argtypes = [P, _dp, _dp, _dp, _dp, C.POINTER(hsba_lm_options), C.c_int, _dp,
                                 C.POINTER(hsba_lm_stats), C.POINTER(LMTrace)]
+        L.oracle_coupling.argtypes = [P, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int32), _dp]
+        L.oracle_pairs.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
         L.oracle_dense_normal.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         L.oracle_set_point_blocks.argtypes = [P, _dp]
         L.oracle_set_loss.argtypes = [P, C.c_int, C.c_double]
@@ -118,6 +121,28 @@ class Oracle:
             *(dptr(out[k]) for k in ("V", "Ui", "Uc", "Wpi", "Wpc", "Wic", "g")))
         assert rc == 0
         return out
+
+    def coupling(self, which, blocks=True):
+        """Coupling set `which` (0 point-image, 1 point-camera, 2 image-camera)
+        in the reference's block-id order, and its blocks at the last build."""
+        n = C.c_int64()
+        lib().oracle_coupling(self.h, which, C.byref(n), None, None)
+        rc = np.zeros((n.value, 2), np.int32)
+        rows, cols = [(3, 6), (3, self.cam_d), (6, self.cam_d)][which]
+        W = np.zeros((n.value, rows, max(cols, 1))) if blocks else None
+        r = lib().oracle_coupling(self.h, which, C.byref(n), rc.ctypes.data_as(C.POINTER(C.c_int32)),
+                                  dptr(W) if blocks else None)
+        assert r == 0
+        return rc, (W[:, :, :cols] if blocks else None)
+
+    def pairs(self):
+        """Image pairs sharing a point, their incidence offsets and incidences."""
+        n, m = C.c_int64(), C.c_int64()
+        lib().oracle_pairs(self.h, C.byref(n), C.byref(m), None, None, None)
+        pr, ptr, inc = np.zeros((n.value, 2), np.int32), np.zeros(n.value + 1, np.int64), np.zeros((m.value, 2), np.int32)
+        lib().oracle_pairs(self.h, C.byref(n), C.byref(m), pr.ctypes.data_as(C.POINTER(C.c_int32)),
+                           ptr.ctypes.data_as(C.POINTER(C.c_int64)), inc.ctypes.data_as(C.POINTER(C.c_int32)))
+        return pr, ptr, inc
 
     def reduced_system(self, mu, literal=False):
         n = self.n_reduced

@@ -2,14 +2,19 @@
 
 Every reduction, factorization and LM decision of the library is per window,
 so a window's arithmetic must not depend on the other windows of its batch or
-on when the serving loop admitted it.  Also checks the serving loop's
-admission control and bookkeeping.
+on when the serving loop admitted it.  Every served job's final RMSE is also
+checked against the FP64 oracle's (1e-3 px).  Also checks the serving loop's
+admission control and bookkeeping, that one failing job leaves the jobs
+batched with it as if alone, and that a malformed job is refused at submit.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
 from hs_sfm_amd import abi, synthetic as syn
 from hs_sfm_amd.hsba import Batch, Problem, Server
+from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -61,7 +66,55 @@ def test_server_results_match_alone(gpu, streams, batch):
         xa, sta = alone(s, iters)
         assert np.array_equal(srv.x_out(jid), xa), s.name
         assert r.stats.final_cost == sta.final_cost
+        _, xo, _, _ = Oracle(s).lm(max_iterations=iters)
+        assert abs(syn.reprojection_rmse(s, srv.x_out(jid)) - syn.reprojection_rmse(s, xo)) < 1e-3, s.name
     assert srv.poll(8) == []
+    srv.close()
+
+
+def with_unobserved_image(s):
+    """The scene plus one image that no key observes (its reduced-system
+    block is missing, so the reference's SimplicialLDLT fails: -1)."""
+    P, I = s.num_points, s.num_images
+    x0 = np.concatenate([s.x0[:3 * P + 6 * I], s.x0[3 * P:3 * P + 6], s.x0[3 * P + 6 * I:]])
+    return dataclasses.replace(s, num_images=I + 1, image_camera_map=np.append(s.image_camera_map, 0),
+                               x0=x0, x_true=None, name=s.name + "+unobserved")
+
+
+def test_failing_job_leaves_its_batch_alone(gpu):
+    jobs = mixed_jobs()[:6]
+    bad = with_unobserved_image(jobs[2])
+    iters = 4
+    srv = Server(streams=1, max_batch_jobs=8, max_iterations=iters)
+    ids = [srv.submit(srv.prepare(s)) for s in jobs[:2] + [bad] + jobs[2:]]
+    srv.drain()
+    res = {r.job_id: r for r in srv.poll(64)}
+    for jid, s in zip(ids, jobs[:2] + [bad] + jobs[2:]):
+        r = res[jid]
+        if s is bad:
+            assert r.status == -1  # HSBA_ERR_GENERIC, the reference's -1
+            continue
+        assert r.status == 0, (s.name, r.status)
+        xa, _ = alone(s, iters)
+        assert np.array_equal(srv.x_out(jid), xa), s.name
+    srv.close()
+
+
+def test_malformed_job_refused_at_submit(gpu):
+    s = mixed_jobs()[0]
+    fm = s.feature_maps.copy()
+    fm[1] = fm[0]  # a duplicate (image, point) key
+    dup = dataclasses.replace(s, feature_maps=fm)
+    srv = Server(streams=1, max_iterations=2)
+    with pytest.raises(RuntimeError, match="HSBA_ERR_INVALID"):
+        srv.submit(srv.prepare(dup))
+    out = fm.copy()
+    out[0, 1] = s.num_points  # point id out of range
+    with pytest.raises(RuntimeError, match="HSBA_ERR_INVALID"):
+        srv.submit(srv.prepare(dataclasses.replace(s, feature_maps=out)))
+    jid = srv.submit(srv.prepare(s))  # the server still serves good jobs
+    srv.drain()
+    assert [r.status for r in srv.poll(4) if r.job_id == jid] == [0]
     srv.close()
 
 
