@@ -22,6 +22,7 @@ ERRORS = {
     -1: "HSBA_ERR_GENERIC", -2: "HSBA_ERR_INVALID", -3: "HSBA_ERR_NOT_SPD",
     -4: "HSBA_ERR_NAN", -5: "HSBA_ERR_OOM", -6: "HSBA_ERR_DEVICE",
     -7: "HSBA_ERR_ADMISSION", -8: "HSBA_ERR_UNSUPPORTED", -9: "HSBA_ERR_NOT_READY",
+    -10: "HSBA_ERR_TOO_LARGE",
 }
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
@@ -264,6 +265,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
                                            _ip, C.POINTER(C.c_int64), _ip]
     lib.hsba_coupling_blocks.argtypes = [P, _dp, _dp, _dp]
     lib.hsba_covariance.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
+    lib.hsba_covariance_static.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
     lib.hsba_scene_load_sfm.argtypes = [C.POINTER(hsba_sfm_files), C.c_uint32, C.c_uint32, C.POINTER(P)]
     lib.hsba_scene_load_bal.argtypes = [C.c_char_p, C.POINTER(P)]
     lib.hsba_scene_get.argtypes = [P, C.POINTER(hsba_scene_view)]
@@ -277,7 +279,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 EXPORTED_SYMBOLS = [
     "hsba_version", "hsba_device_count", "hsba_problem_create", "hsba_problem_destroy",
     "hsba_problem_sizes", "hsba_evaluate", "hsba_jacobian", "hsba_build_normal",
-    "hsba_solve", "hsba_covariance", "hsba_problem_structure", "hsba_coupling_blocks", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
+    "hsba_solve", "hsba_covariance", "hsba_covariance_static", "hsba_problem_structure", "hsba_coupling_blocks", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
     "hsba_gradient_dot", "hsba_mahalanobis", "hsba_lm_solve", "hsba_lm_begin",
     "hsba_lm_iterate", "hsba_lm_end", "hsba_batch_create",
     "hsba_batch_destroy", "hsba_batch_lm_solve", "hsba_server_create",

@@ -20,6 +20,7 @@ LIB = os.path.join(OUT_DIR, "libhsba.so")
 SOURCES = [
     "hsba_kernels.hip",
     "hsba_cholesky.hip",
+    "hsba_scratch.hip",
     "hsba_layout.cpp",
     "hsba_engine.cpp",
     "hsba_server.cpp",

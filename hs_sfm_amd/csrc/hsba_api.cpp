@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -206,22 +207,52 @@ int hsba_build_normal(hsba_problem *p, const double *x, const double *y_obs, con
   });
 }
 
+namespace {
+
+// hsba_covariance and hsba_covariance_static: scratch == nullptr allocates the
+// device buffers of the inverse for the call, else uses the fixed region.
+int covariance_impl(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
+                    double *point_blocks, double *image_blocks, double *camera_blocks, double *reduced,
+                    double *scratch) {
+  int rc = set_x(p, x);
+  if (rc) return rc;
+  // N does not depend on y except through robust-loss weights
+  std::vector<double> zeros;
+  if (!y_obs) {
+    zeros.assign(p->eng->L.hw[0].y_size, 0.0);
+    y_obs = zeros.data();
+  }
+  const double *ys[1] = {y_obs};
+  if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
+  if ((rc = p->eng->build_normal(nullptr, nullptr, nullptr, nullptr))) return rc;
+  return p->eng->covariance(point_blocks, image_blocks, camera_blocks, reduced, scratch);
+}
+
+}  // namespace
+
 int hsba_covariance(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
                     double *point_blocks, double *image_blocks, double *camera_blocks, double *reduced) {
   return hsba::guarded([&]() -> int {
     if (!p || !x) return HSBA_ERR_INVALID;
-    int rc = set_x(p, x);
-    if (rc) return rc;
-    // N does not depend on y except through robust-loss weights
-    std::vector<double> zeros;
-    if (!y_obs) {
-      zeros.assign(p->eng->L.hw[0].y_size, 0.0);
-      y_obs = zeros.data();
+    return covariance_impl(p, x, y_obs, ycov, point_blocks, image_blocks, camera_blocks, reduced, nullptr);
+  });
+}
+
+int hsba_covariance_static(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
+                           double *point_blocks, double *image_blocks, double *camera_blocks, double *reduced) {
+  return hsba::guarded([&]() -> int {
+    if (!p || !x) return HSBA_ERR_INVALID;
+    const Layout &L = p->eng->L;
+    if (L.nwin != 1 || L.wi[0].ld > hsba::kScratchMaxLd || L.P > hsba::kScratchMaxPoints) {
+      if (std::getenv("HSBA_VERBOSE"))
+        std::fprintf(stderr, "hsba_covariance_static: %lld reduced rows (max %lld) / %lld points (max %lld)\n",
+                     (long long)L.wi[0].n, (long long)hsba::kScratchMaxLd, (long long)L.P,
+                     (long long)hsba::kScratchMaxPoints);
+      return HSBA_ERR_TOO_LARGE;
     }
-    const double *ys[1] = {y_obs};
-    if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
-    if ((rc = p->eng->build_normal(nullptr, nullptr, nullptr, nullptr))) return rc;
-    return p->eng->covariance(point_blocks, image_blocks, camera_blocks, reduced);
+    double *scratch = nullptr;
+    if (hsba::scratch_region(&scratch)) return HSBA_ERR_DEVICE;
+    return covariance_impl(p, x, y_obs, ycov, point_blocks, image_blocks, camera_blocks, reduced, scratch);
   });
 }
 

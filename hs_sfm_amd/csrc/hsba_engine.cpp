@@ -148,7 +148,10 @@ struct Engine final : EngineBase {
     A(ck, d.nchunks); A(cbg, d.nchunks); A(cen, d.nchunks); A(cw, d.nchunks);
     A(wi, L.nwin);
     A(d.ws, L.nwin);
+    // observations and the whitening factors of their inverse covariances
+    // (allocated up front, so that no later upload allocates)
     double *key_uv; A(key_uv, 2 * L.N);
+    double *key_cov; A(key_cov, 3 * L.N);
     A(d.pts, 3 * L.P); A(d.pts_new, 3 * L.P); A(d.dpt, 3 * L.P);
     A(d.V, 6 * L.P); A(d.gp, 3 * L.P); A(d.Linv, 6 * L.P); A(d.q, 3 * L.P); A(d.tpt, 3 * L.P);
     A(d.imgs, 6 * L.I); A(d.imgs_new, 6 * L.I);
@@ -199,7 +202,7 @@ struct Engine final : EngineBase {
         (rc = upload(cw, L.chunk_win, st)) || (rc = upload(wi, L.wi, st)) ||
         (rc = upload(small_list_dev, L.small_list, st)))
       return rc;
-    d.key_img = key_img; d.key_pt = key_pt; d.key_orig = key_orig; d.key_uv = key_uv; d.key_cov = nullptr;
+    d.key_img = key_img; d.key_pt = key_pt; d.key_orig = key_orig; d.key_uv = key_uv; d.key_cov = key_cov;
     d.pt_ptr = pt_ptr; d.pt_win = pt_win;
     d.img_ptr = img_ptr; d.img_keys = img_keys; d.img_win = img_win; d.img_cam = img_cam; d.img_row = img_row;
     d.cam_ptr = cam_ptr; d.cam_imgs = cam_imgs; d.cam_win = cam_win; d.cam_row = cam_row;
@@ -344,15 +347,7 @@ struct Engine final : EngineBase {
     }
     int rc = upload((double *)d.key_uv, uv, st);
     if (rc) return rc;
-    if (any_cov) {
-      if (!d.key_cov) {
-        double *p;
-        if ((rc = dalloc(p, 3 * L.N, owned, arena))) return rc;
-        d.key_cov = p;
-        drop_graph();
-      }
-      if ((rc = upload((double *)d.key_cov, cov, st))) return rc;
-    }
+    if (any_cov && (rc = upload((double *)d.key_cov, cov, st))) return rc;
     if (d.has_cov != (any_cov ? 1 : 0) && gexec) {
       drop_graph();
     }
@@ -779,13 +774,14 @@ struct Engine final : EngineBase {
   // inverse of the reduced system: the Schur complement S = U - W^T V^-1 W is
   // factored, S^-1 formed from the factor on the device, and each point's
   // block from S^-1 and its Z rows.
-  int covariance(double *Vp, double *Ui, double *Uc, double *red) override {
+  int covariance(double *Vp, double *Ui, double *Uc, double *red, double *scratch) override {
     if (!built) return HSBA_ERR_GENERIC;
     if (L.pcg || L.nwin != 1) return HSBA_ERR_UNSUPPORTED;
     if (L.hw[0].unobserved) return HSBA_ERR_NOT_SPD;
+    const WinInfo &wi = L.wi[0];
+    if (scratch && (wi.ld > kScratchMaxLd || L.P > kScratchMaxPoints)) return HSBA_ERR_TOO_LARGE;
     int rc = set_state(nullptr, 0.0, 1, 0, 1);
     if (rc) return rc;
-    const WinInfo &wi = L.wi[0];
     ks->point_factor(d, st);
     ks->key_z(d, st);
     ks->pairs(d, st);
@@ -796,24 +792,38 @@ struct Engine final : EngineBase {
     if ((rc = get_state())) return rc;
     if (ws_host[0].chol_fail) return HSBA_ERR_NOT_SPD;
     const int64_t ld = wi.ld, n = wi.n;
+    // L^-T (Y), S^-1 and the point blocks: from the fixed scratch region, or
+    // allocated for this call.  Every element that is read is written first
+    // (Y's upper block triangle column by column, both triangles of S^-1,
+    // every point block), so the scratch needs no clearing between calls.
     double *Y = nullptr, *Sinv = nullptr, *pc = nullptr;
+    const bool own = scratch == nullptr;
     auto release = [&]() {
+      if (!own) return;
       if (Y) (void)hipFree(Y);
       if (Sinv) (void)hipFree(Sinv);
       if (pc) (void)hipFree(pc);
     };
     std::vector<double> hS;
     if (n > 0) {
-      const size_t bytes = sizeof(double) * (size_t)(wi.s_off + ld * ld);
-      if (hipMalloc((void **)&Y, bytes) != hipSuccess || hipMalloc((void **)&Sinv, bytes) != hipSuccess) {
-        release();
-        return HSBA_ERR_OOM;
+      if (own) {
+        const size_t bytes = sizeof(double) * (size_t)(wi.s_off + ld * ld);
+        if (hipMalloc((void **)&Y, bytes) != hipSuccess || hipMalloc((void **)&Sinv, bytes) != hipSuccess) {
+          release();
+          return HSBA_ERR_OOM;
+        }
+      } else {
+        // one window: wi.s_off == 0
+        Y = scratch;
+        Sinv = scratch + kScratchMaxLd * kScratchMaxLd;
       }
       const bool large = !L.large_list.empty();
       chol_inverse_impl(plan, 0, large, Y, Sinv, st);
     }
     if (L.pts_free && L.P && Vp) {
-      if (hipMalloc((void **)&pc, sizeof(double) * 9 * L.P) != hipSuccess) {
+      if (!own) {
+        pc = scratch + 2 * kScratchMaxLd * kScratchMaxLd;
+      } else if (hipMalloc((void **)&pc, sizeof(double) * 9 * L.P) != hipSuccess) {
         release();
         return HSBA_ERR_OOM;
       }

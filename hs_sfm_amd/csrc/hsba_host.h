@@ -131,7 +131,10 @@ struct EngineBase {
   virtual int build_normal(double *V, double *Ui, double *Uc, double *g) = 0;
   virtual int download_normal(double *V, double *Ui, double *Uc, double *g) = 0;
   virtual int solve(double mu, double *delta, double *S, double *rhs) = 0;
-  virtual int covariance(double *Vp, double *Ui, double *Uc, double *reduced) = 0;
+  // scratch: null = device buffers are allocated for the call; else the
+  // fixed scratch region (no allocation; sizes bounded by kScratchMaxLd /
+  // kScratchMaxPoints, HSBA_ERR_TOO_LARGE beyond them)
+  virtual int covariance(double *Vp, double *Ui, double *Uc, double *reduced, double *scratch) = 0;
   // coupling blocks W_pi / W_pc / W_ic of the last build (window 0)
   virtual int coupling(double *pi, double *pc, double *ic) = 0;
   virtual int evaluate(double *y) = 0;

@@ -56,4 +56,15 @@ void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int
 // scratch at the window's offset) receives L^-T, Sinv the full symmetric inverse.
 void chol_inverse_impl(const CholPlan &plan, int w, bool large, double *Y, double *Sinv, hipStream_t st);
 
+// Fixed device scratch of the allocation-free covariance entry point
+// (hsba_covariance_static): L^-T and S^-1 of one reduced system of at most
+// kScratchMaxLd padded rows, then the point blocks of at most
+// kScratchMaxPoints points.  8.5 MiB per GPU, held until the process ends.
+constexpr int64_t kScratchMaxLd = 512;
+constexpr int64_t kScratchMaxPoints = 65536;
+constexpr int64_t kScratchDoubles = 2 * kScratchMaxLd * kScratchMaxLd + 9 * kScratchMaxPoints;
+// Device address of the current GPU's scratch (looked up once per device and
+// cached); 0 on success.
+int scratch_region(double **ptr);
+
 }  // namespace hsba

@@ -460,7 +460,7 @@ int64_t layout_device_bytes(const Layout &L, int precision) {
   const int64_t t = precision == HSBA_PRECISION_FP64 ? 8 : 4;
   const int64_t E = L.E, D = L.D;
   int64_t b = 0;
-  b += L.N * (4 * 3 + 16) + (L.has_cov ? L.N * 24 : 0);           // key arrays
+  b += L.N * (4 * 3 + 16 + 24);                                   // key arrays, whitening factors
   b += L.N * (3 * E * t + z_stride((int)E, (int)t) * t);           // W, Z
   if (L.pcg) b += L.N * (z_stride((int)E, (int)t) * t + 8);        // image-major Z copy
   b += L.P * (8 + 3 * 8 * 3 + t * 18);                              // points
@@ -659,7 +659,7 @@ int64_t estimate_device_bytes(const hsba_problem_desc &ds, int precision, bool p
   const int64_t chunks = (N + kChunk - 1) / kChunk + (3 * P + kChunk - 1) / kChunk + (6 * I + kChunk - 1) / kChunk +
                          (D * C + kChunk - 1) / kChunk + 4;
   int64_t b = 0;
-  b += N * (4 * 3 + 16) + (has_cov ? N * 24 : 0);
+  b += N * (4 * 3 + 16 + 24);
   b += N * (3 * E * t + z_stride((int)E, (int)t) * t) + N * 4;     // W, Z, image key lists
   if (pcg) b += N * (z_stride((int)E, (int)t) * t + 8);             // image-major Z copy
   b += P * (8 + 3 * 8 * 3 + t * 18 + 24);

@@ -97,9 +97,10 @@ class Problem:
         rc = self.lib.hsba_solve(self.h, mu, abi.dptr(delta))
         return rc, delta
 
-    def covariance(self, x=None, y_obs=None):
+    def covariance(self, x=None, y_obs=None, static=False):
         """Diagonal blocks of N^-1 at x and the dense inverse of the reduced
-        system (hsba_covariance); returns (rc, dict)."""
+        system (hsba_covariance, or hsba_covariance_static with static=True:
+        no device allocation); returns (rc, dict)."""
         s = self.scene
         x = s.x0 if x is None else x
         y_obs = s.y_obs if y_obs is None else y_obs
@@ -107,9 +108,9 @@ class Problem:
         out = dict(V=np.zeros((s.num_points, 3, 3)), Ui=np.zeros((s.num_images, 6, 6)),
                    Uc=np.zeros((s.num_cameras, max(d, 1), max(d, 1))),
                    reduced=np.zeros((self.n_reduced, self.n_reduced)))
-        rc = self.lib.hsba_covariance(self.h, self._d(x), self._d(y_obs), C.byref(self.desc.ycov),
-                                      abi.dptr(out["V"]), abi.dptr(out["Ui"]), abi.dptr(out["Uc"]),
-                                      abi.dptr(out["reduced"]))
+        fn = self.lib.hsba_covariance_static if static else self.lib.hsba_covariance
+        rc = fn(self.h, self._d(x), self._d(y_obs), C.byref(self.desc.ycov), abi.dptr(out["V"]),
+                abi.dptr(out["Ui"]), abi.dptr(out["Uc"]), abi.dptr(out["reduced"]))
         out["Uc"] = out["Uc"][:, :d, :d]
         return rc, out
 

@@ -45,6 +45,7 @@ extern "C" {
 #define HSBA_ERR_ADMISSION     -7  /* job larger than the server's HBM budget */
 #define HSBA_ERR_UNSUPPORTED   -8  /* option combination not implemented   */
 #define HSBA_ERR_NOT_READY     -9  /* server job still running             */
+#define HSBA_ERR_TOO_LARGE    -10  /* beyond the fixed scratch of an allocation-free entry point */
 
 /* ---- masks (bit i == reference enum value i) ---------------------------- */
 /* IntrinsicComputationsMask, camera_shared_common_types.hpp:14-22 */
@@ -298,6 +299,21 @@ int hsba_reduced_system(hsba_problem *problem, double mu,
 int hsba_covariance(hsba_problem *problem, const double *x, const double *y_obs,
                     const hsba_ycov_inverse *ycov, double *point_blocks, double *image_blocks,
                     double *camera_blocks, double *reduced);
+
+/* hsba_covariance without any device allocation, for callers that cannot
+ * call hipMalloc (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid it).  No counterpart in the reference.  The same
+ * kernels and results as hsba_covariance; the device buffers of the inverse
+ * (L^-T and S^-1 of the reduced system, the point blocks) come from one fixed
+ * scratch region per GPU (8.5 MiB, held until the process ends).  Limits: a
+ * reduced system of at most 512 rows after padding to whole 64-row tiles and
+ * at most 65536 points; beyond them it returns HSBA_ERR_TOO_LARGE and does
+ * nothing.  Every call on one GPU shares that GPU's region: calls that overlap
+ * on one GPU (several host threads, or problems on different streams) race and
+ * must be serialised by the caller. */
+int hsba_covariance_static(hsba_problem *problem, const double *x, const double *y_obs,
+                           const hsba_ycov_inverse *ycov, double *point_blocks, double *image_blocks,
+                           double *camera_blocks, double *reduced);
 
 /* LM scalar helpers on the last build:
  *   max diag(N)            camera_shared_normal_matrix_max_diagonal_value_calculator.hpp:26-38

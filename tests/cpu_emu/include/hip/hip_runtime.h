@@ -29,6 +29,8 @@
 #define __forceinline__ inline
 #define __launch_bounds__(...)
 #define __shared__ static
+#define __align__(n) alignas(n)
+#define HIP_SYMBOL(x) (&(x))
 
 struct dim3 {
   unsigned x, y, z;
@@ -120,6 +122,8 @@ enum hipStreamCaptureMode { hipStreamCaptureModeGlobal, hipStreamCaptureModeThre
 inline const char *hipGetErrorString(hipError_t e) { return e == hipSuccess ? "success" : "emulated error"; }
 inline hipError_t hipGetLastError() { return hipSuccess; }
 inline hipError_t hipGetDeviceCount(int *n) { *n = 1; return hipSuccess; }
+inline hipError_t hipGetDevice(int *d) { *d = 0; return hipSuccess; }
+inline hipError_t hipGetSymbolAddress(void **p, const void *sym) { *p = const_cast<void *>(sym); return hipSuccess; }
 inline hipError_t hipMalloc(void **p, size_t bytes) {
   *p = std::malloc(bytes ? bytes : 1);
   if (!*p) return hipErrorOutOfMemory;
