@@ -266,6 +266,10 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.hsba_coupling_blocks.argtypes = [P, _dp, _dp, _dp]
     lib.hsba_covariance.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
     lib.hsba_covariance_static.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
+    lib.hsba_batch_lm_begin.argtypes = [P, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(hsba_ycov_inverse),
+                                        C.POINTER(hsba_lm_options)]
+    lib.hsba_batch_lm_iterate.argtypes = [P, C.c_int32, C.POINTER(hsba_stage_times)]
+    lib.hsba_batch_lm_end.argtypes = [P, C.POINTER(_dp), C.POINTER(hsba_lm_stats)]
     lib.hsba_scene_load_sfm.argtypes = [C.POINTER(hsba_sfm_files), C.c_uint32, C.c_uint32, C.POINTER(P)]
     lib.hsba_scene_load_bal.argtypes = [C.c_char_p, C.POINTER(P)]
     lib.hsba_scene_get.argtypes = [P, C.POINTER(hsba_scene_view)]
@@ -282,7 +286,8 @@ EXPORTED_SYMBOLS = [
     "hsba_solve", "hsba_covariance", "hsba_covariance_static", "hsba_problem_structure", "hsba_coupling_blocks", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
     "hsba_gradient_dot", "hsba_mahalanobis", "hsba_lm_solve", "hsba_lm_begin",
     "hsba_lm_iterate", "hsba_lm_end", "hsba_batch_create",
-    "hsba_batch_destroy", "hsba_batch_lm_solve", "hsba_server_create",
+    "hsba_batch_destroy", "hsba_batch_lm_solve", "hsba_batch_lm_begin", "hsba_batch_lm_iterate",
+    "hsba_batch_lm_end", "hsba_server_create",
     "hsba_server_destroy", "hsba_server_submit", "hsba_server_poll", "hsba_server_drain",
     "hsba_server_clock", "hsba_naive_problem_create", "hsba_naive_problem_destroy",
     "hsba_naive_evaluate", "hsba_naive_jacobian", "hsba_naive_build_normal", "hsba_naive_solve",
