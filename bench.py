@@ -1,27 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X LM bundle adjuster.
 
-Default workload (one GPU): the "mid" scene — 200 cameras with 9 parameters
-(angle-axis, translation, f, k1, k2), 50k points, ~1.0M keys, each point seen
-by 3..40 cameras (truncated geometric, mean 20) — solved in mixed precision
-(FP32 Jacobian / Hessian / Schur, FP64 Cholesky, cost and gain ratio).
+Default run (one GPU, no flags): one JSON line whose headline is the "mid"
+scene — 200 cameras with 9 parameters (angle-axis, translation, f, k1, k2),
+50k points, ~1.0M keys, each point seen by 3..40 cameras (truncated geometric,
+mean 20) — solved in mixed precision (FP32 Jacobian / Hessian / Schur, FP64
+Cholesky, cost and gain ratio).  A "step" is one LM iteration (J + N + g
+build, Schur, FP64 Cholesky, point back-substitution, candidate cost, accept /
+reject); the timed region runs exactly --steps iterations from x0 with every
+input already resident in HBM, bracketed by device synchronisation.
 
-A "step" is one LM iteration (J + N + g build, Schur, FP64 Cholesky, point
-back-substitution, candidate cost, accept / reject).  The timed region runs
-exactly --steps iterations from x0 with every input already resident in HBM,
-bracketed by device synchronisation.  Prints one JSON line.
-
-Other workloads: --workload ring | windows | large | serving.
+The same line carries the other workloads as sub-objects under "workloads":
+  ring     10 cameras, 500 points, ~4.5k keys, FP64, 20 iterations
+  windows  512 independent windows (5..40 cameras, 200..5000 points) batched
+           in one set of launches per LM iteration vs solved one at a time
+  large    2000 cameras x 9 along a trajectory, 600k points, ~5.1M keys,
+           block-Jacobi PCG (rtol 1e-6), 30 iterations
+  serving  Poisson arrivals, 95 % windows / 5 % 1M-key jobs, p50 / p99
+Each is also available alone with --workload.
 """
 from __future__ import annotations
 
 import argparse
+import concurrent.futures as cf
 import json
 import os
 import platform
-import subprocess
 import sys
 import time
+import traceback
 
 import numpy as np
 
@@ -35,6 +42,7 @@ METRIC = "BA LM iters/s (1M obs) + batched solves/s, p99 ms, at FP64-matched RMS
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP64_TFLOPS = 78.6
+CHECK_THREADS = 16  # host threads for the oracle checks (the box's CPU share)
 
 
 def cpu_model() -> str:
@@ -64,6 +72,11 @@ def linearize_bytes(scene: syn.Scene, E: int, tbytes: int) -> float:
     return float(N * (16 + 4 + 4 + 3 * E * tbytes) + P * (24 + 4 + 9 * tbytes))
 
 
+def chol_flops(n: int) -> float:
+    """Dense Cholesky n^3 / 3 plus the two triangular solves 2 n^2."""
+    return n ** 3 / 3 + 2 * n ** 2
+
+
 def load_traffic(name: str):
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(path):
@@ -80,23 +93,50 @@ def oracle_reference(name: str):
     return None
 
 
-def cpu_baseline_mid(scene: syn.Scene, iters: int = 1):
+def roofline_entry(kind, achieved, peak, unit, algorithmic, traffic=None, kernel=None):
+    out = {"bound": kind, "achieved": achieved, "peak": peak, "unit": unit, "frac": achieved / peak,
+           "algorithmic": algorithmic, "traffic": traffic}
+    if kernel:
+        out["kernel"] = kernel
+    return out
+
+
+def oracle_rmse(scene, iters):
+    """FP64 oracle's RMSE after `iters` LM iterations (checker only)."""
+    from oracle.oracle import Oracle
+    _, xo, _, _ = Oracle(scene).lm(max_iterations=iters)
+    return syn.reprojection_rmse(scene, xo)
+
+
+def oracle_rmse_all(scenes, iters):
+    """Oracle RMSE of every scene, on host threads (the oracle is a ctypes
+    call, which releases the GIL)."""
+    with cf.ThreadPoolExecutor(max_workers=CHECK_THREADS) as ex:
+        return list(ex.map(lambda s: oracle_rmse(s, iters), scenes))
+
+
+# ------------------------------------------------------------------ configs[1]
+def cpu_baseline_mid(scene: syn.Scene, samples: int = 3):
     """The FP64 oracle (single-threaded C++ restatement of the reference path)
-    on the same scene: time of `iters` LM iterations, from two runs (0 and
-    `iters` iterations) so the initial linearization is not counted."""
-    sys.path.insert(0, os.path.join(ROOT))
+    on the same scene: one LM iteration per sample, from two runs (0 and 1
+    iterations) so the initial linearization is not counted; median of
+    `samples` samples."""
     from oracle.oracle import Oracle  # CPU baseline leg only
     o = Oracle(scene)
-    t0 = time.perf_counter()
-    o.lm(max_iterations=0)
-    t1 = time.perf_counter()
-    o.lm(max_iterations=iters)
-    t2 = time.perf_counter()
-    per_iter = max(1e-9, (t2 - t1) - (t1 - t0)) / iters
-    return {"value": 1.0 / per_iter, "unit": "iters/s", "cores": 1, "kind": "port",
-            "sample": f"{iters} LM iteration(s) of the same {scene.num_keys}-key scene "
-                      f"(FP64, point-wise Schur + dense LDL^T, single thread; "
-                      f"initial linearization excluded), cpu={cpu_model()}, nproc={os.cpu_count()}"}
+    per = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        o.lm(max_iterations=0)
+        t1 = time.perf_counter()
+        o.lm(max_iterations=1)
+        t2 = time.perf_counter()
+        per.append(max(1e-9, (t2 - t1) - (t1 - t0)))
+    med = float(np.median(per))
+    return {"value": 1.0 / med, "unit": "iters/s", "cores": 1, "kind": "port",
+            "sample": f"median of {samples} samples of one LM iteration of the same {scene.num_keys}-key scene "
+                      f"(FP64, point-wise Schur + dense LDL^T, single thread; initial linearization excluded; "
+                      f"per-sample s: {', '.join(f'{t:.2f}' for t in per)}), cpu={cpu_model()}, "
+                      f"nproc={os.cpu_count()}"}
 
 
 def run_mid(args):
@@ -119,41 +159,34 @@ def run_mid(args):
     rc, x, st = p.lm_end()
     dt = t1 - t0
     rmse = syn.reprojection_rmse(scene, x)
-    # stage breakdown: the same K iterations replayed with HIP event nodes
-    # captured around each stage of the graph (library stream)
+    # stage breakdown: the same K iterations replayed with HIP events recorded
+    # between the stage graphs on the library's stream
     abi.check(p.lm_begin(max_iterations=K), "lm_begin")
     rc, stages = p.lm_iterate(K, timed=True)
     abi.check(rc, "lm_iterate(timed)")
     p.lm_end()
     flops = pair_flops(scene, E)
     lin_bytes = linearize_bytes(scene, E, tbytes)
+    n = 9 * scene.num_images
+    cf_ = chol_flops(n)
+    kernels = {
+        "k_pairs": roofline_entry("mfma", flops / (stages["pairs_ms"] * 1e-3) / 1e12,
+                                  PEAK_FP32_TFLOPS if tbytes == 4 else PEAK_FP64_TFLOPS, "TFLOP/s", flops,
+                                  (load_traffic(f"k_pairs_mid_{args.precision}") or {}).get("bytes_per_launch")),
+        "cholesky": roofline_entry("mfma", cf_ / (stages["cholesky_ms"] * 1e-3) / 1e12, PEAK_FP64_TFLOPS,
+                                   "TFLOP/s", cf_,
+                                   (load_traffic(f"cholesky_mid_{args.precision}") or {}).get("bytes_per_launch")),
+        "k_linearize": roofline_entry("hbm", lin_bytes / (stages["linearize_ms"] * 1e-3) / 1e9, PEAK_HBM_GBS,
+                                      "GB/s", lin_bytes),
+    }
     dominant = max(("pairs_ms", "cholesky_ms", "linearize_ms", "image_reduce_ms", "point_factor_ms",
                     "assemble_ms", "backsub_ms", "step_cost_ms"), key=lambda k: stages[k])
-    n = 9 * scene.num_images
-    chol_flops = n ** 3 / 3 + 2 * n ** 2
-    kernels = {
-        "k_pairs": {"bound": "mfma", "unit": "TFLOP/s", "algorithmic": flops,
-                    "achieved": flops / (stages["pairs_ms"] * 1e-3) / 1e12,
-                    "peak": PEAK_FP32_TFLOPS if tbytes == 4 else PEAK_FP64_TFLOPS,
-                    "traffic": (load_traffic(f"k_pairs_mid_{args.precision}") or {}).get("bytes_per_launch")},
-        "cholesky": {"bound": "mfma", "unit": "TFLOP/s", "algorithmic": chol_flops,
-                     "achieved": chol_flops / (stages["cholesky_ms"] * 1e-3) / 1e12, "peak": PEAK_FP64_TFLOPS,
-                     "traffic": (load_traffic(f"cholesky_mid_{args.precision}") or {}).get("bytes_per_launch")},
-        "k_linearize": {"bound": "hbm", "unit": "GB/s", "algorithmic": lin_bytes,
-                        "achieved": lin_bytes / (stages["linearize_ms"] * 1e-3) / 1e9, "peak": PEAK_HBM_GBS,
-                        "traffic": None},
-    }
-    for k in kernels.values():
-        k["frac"] = k["achieved"] / k["peak"]
     dom_kernel = {"pairs_ms": "k_pairs", "cholesky_ms": "cholesky", "linearize_ms": "k_linearize"}.get(dominant,
                                                                                                        "k_pairs")
-    kd = kernels[dom_kernel]
-    roof = {"kernel": {"k_pairs": "k_pairs (camera-pair Schur products, FP32 FMA)",
-                       "cholesky": "dense FP64 Cholesky of the 1800 x 1800 reduced system "
-                                   "(k_chol_first + k_chol_step + k_chol_back)",
-                       "k_linearize": "k_linearize (residual + Jacobian + point blocks)"}[dom_kernel],
-            "bound": kd["bound"], "achieved": kd["achieved"], "peak": kd["peak"], "unit": kd["unit"],
-            "frac": kd["frac"], "traffic": kd["traffic"]}
+    names = {"k_pairs": "k_pairs_mma (camera-pair Schur products on the FP32 matrix cores)",
+             "cholesky": f"dense FP64 Cholesky + triangular solves of the {n} x {n} reduced system",
+             "k_linearize": "k_linearize (residual + Jacobian + point blocks)"}
+    roof = dict(kernels[dom_kernel], kernel=names[dom_kernel])
     ref = oracle_reference("mid")
     out = {
         "metric": METRIC, "value": K / dt, "unit": "iters/s", "n_gpus": 1, "steps": K, "warmup": W,
@@ -162,28 +195,154 @@ def run_mid(args):
         "config": {"workload": "mid: 200 cameras x 9 params (r, t, f, k1, k2), 50k points, "
                                f"{scene.num_keys} keys, LM from x0",
                    "cameras": scene.num_images, "points": scene.num_points, "keys": scene.num_keys,
-                   "precision": args.precision, "reduced_size": 9 * scene.num_images},
+                   "precision": args.precision, "reduced_size": n},
         "roofline": roof, "kernels": kernels,
         "stage_ms": stages, "dominant_stage": dominant,
         "lm": {"status": st.status, "iterations": st.iterations, "accepted": st.accepted,
                "initial_cost": st.initial_cost, "final_cost": st.final_cost, "rmse_px": rmse,
+               "mean_error_px": syn.mean_reprojection_error(scene, x),
                "rmse_x0_px": syn.reprojection_rmse(scene, scene.x0)},
+        "fp64_gpu_plain": {"status": "switched off (HSBA_ERR_UNSUPPORTED)",
+                           "why": "an FP64 LM on this scene faulted the GPU once and no cause in the code has "
+                                  "been shown; the FP64 path with camera blocks >= 7 stays off"},
     }
     if ref and ref.get("iterations") == K:
         out["lm"]["fp64_oracle_rmse_px"] = ref["rmse_px"]
         out["lm"]["rmse_delta_px"] = abs(rmse - ref["rmse_px"])
         out["lm"]["final_cost_rel_err"] = abs(st.final_cost - ref["final_cost"]) / ref["final_cost"]
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_mid(scene, iters=args.cpu_iters)
+        out["cpu_baseline"] = cpu_baseline_mid(scene, samples=args.cpu_samples)
     p.close()
     return out
 
 
-def run_large(args):
-    """Large banded strip: 2000 cameras x 9, 600k points, ~5.1M keys, banded; block-Jacobi
+# ------------------------------------------------------------------ configs[0]
+def run_ring(args, K=20):
+    """10 cameras on a 10 m ring, 500 points, ~4.5k keys, FP64 (camera block
+    6: fixed shared intrinsics), 20 iterations; CPU baseline = the oracle with
+    the reference's literal O(I^2 P) Schur loops."""
+    scene = syn.ring_scene()
+    p = Problem(scene, precision=abi.PRECISION_FP64)
+    p.lm(max_iterations=K)  # warm-up: captures the iteration graph
+    abi.check(p.lm_begin(max_iterations=K), "lm_begin")
+    t0 = time.perf_counter()
+    abi.check(p.lm_iterate(K)[0], "lm_iterate")
+    t1 = time.perf_counter()
+    rc, x, st = p.lm_end()
+    p.close()
+    from oracle.oracle import Oracle  # checker / CPU baseline
+    o = Oracle(scene)
+    c0 = time.perf_counter()
+    _, xo, sto, _ = o.lm(max_iterations=K, literal=True)
+    c1 = time.perf_counter()
+    rm, rmo = syn.reprojection_rmse(scene, x), syn.reprojection_rmse(scene, xo)
+    return {"metric": "BA LM iters/s (ring, 10 cameras, FP64)", "value": K / (t1 - t0), "unit": "iters/s",
+            "steps": K, "ms_per_step": 1e3 * (t1 - t0) / K, "dtype": "fp64",
+            "config": {"workload": f"ring: 10 cameras, 500 points, {scene.num_keys} keys, FP64"},
+            "rmse_px": rm, "fp64_oracle_rmse_px": rmo, "rmse_delta_px": abs(rm - rmo),
+            "final_cost_rel_err": abs(st.final_cost - sto.final_cost) / sto.final_cost,
+            "cpu_baseline": {"value": K / (c1 - c0), "unit": "iters/s", "cores": 1, "kind": "port",
+                             "sample": f"{K} LM iterations of the same scene with the reference's literal "
+                                       f"O(I^2 P) Schur loops, cpu={cpu_model()}"}}
+
+
+# ------------------------------------------------------------------ configs[3]
+def window_flops(scenes, E=6):
+    pf = sum(pair_flops(s, E) for s in scenes)
+    cf_ = sum(chol_flops(6 * s.num_images) for s in scenes)
+    return pf, cf_
+
+
+def run_windows(args):
+    """512 independent windows (5..40 cameras, 200..5000 points, 6-parameter
+    cameras with fixed shared intrinsics): one batch (one set of launches per
+    LM iteration for every window) vs the same windows solved one at a time
+    through the single-problem path.  Both legs include the host-to-device
+    upload of x0 and the observations and the download of x (PCIe-inclusive);
+    the batch's graph capture and each single problem's are done in an
+    untimed warm-up solve."""
+    scenes = syn.windows_batch(count=args.windows)
+    K = args.steps
+    n = len(scenes)
+    b = Batch(scenes, precision=abi.PRECISION_MIXED)
+    b.lm(max_iterations=K)  # warm-up
+    t0 = time.perf_counter()
+    rc, xs, sts = b.lm(max_iterations=K)
+    t1 = time.perf_counter()
+    abi.check(rc, "batch lm")
+    batched = n / (t1 - t0)
+    # per-stage device time of the batch (HIP events between the stage graphs)
+    abi.check(b.lm_begin(max_iterations=K), "batch lm_begin")
+    rc, stages = b.lm_iterate(K, timed=True)
+    abi.check(rc, "batch lm_iterate(timed)")
+    b.lm_end()
+    b.close()
+    # one at a time: one live problem (its own stream and graph), created,
+    # warmed up, solved (timed) and destroyed in turn
+    single_s = 0.0
+    for s in scenes:
+        p = Problem(s, precision=abi.PRECISION_MIXED)
+        p.lm(max_iterations=K)
+        t = time.perf_counter()
+        rc1, _, _ = p.lm(max_iterations=K)
+        single_s += time.perf_counter() - t
+        abi.check(rc1, "single lm")
+        p.close()
+    one_at_a_time = n / single_s
+    # accuracy: every window against the FP64 oracle
+    ro = oracle_rmse_all(scenes, K)
+    deltas = [abs(syn.reprojection_rmse(s, x) - r) for s, x, r in zip(scenes, xs, ro)]
+    # CPU baseline: the oracle solving a sample of windows on one core
+    from oracle.oracle import Oracle  # CPU baseline leg only
+    sample = scenes[:args.cpu_windows]
+    c0 = time.perf_counter()
+    for s in sample:
+        Oracle(s).lm(max_iterations=K)
+    c1 = time.perf_counter()
+    keys = sum(s.num_keys for s in scenes)
+    pf, cfl = window_flops(scenes)
+    lin = sum(linearize_bytes(s, 6, 4) for s in scenes)
+    kernels = {
+        "k_pairs": roofline_entry("mfma", pf / (stages["pairs_ms"] * 1e-3) / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s", pf),
+        "cholesky": roofline_entry("mfma", cfl / (stages["cholesky_ms"] * 1e-3) / 1e12, PEAK_FP64_TFLOPS,
+                                   "TFLOP/s", cfl),
+        "k_linearize": roofline_entry("hbm", lin / (stages["linearize_ms"] * 1e-3) / 1e9, PEAK_HBM_GBS, "GB/s", lin),
+    }
+    dominant = max(("pairs_ms", "cholesky_ms", "linearize_ms", "image_reduce_ms", "point_factor_ms",
+                    "assemble_ms", "backsub_ms", "step_cost_ms"), key=lambda k: stages[k])
+    dom_kernel = {"pairs_ms": "k_pairs", "cholesky_ms": "cholesky", "linearize_ms": "k_linearize"}.get(dominant,
+                                                                                                       "k_pairs")
+    names = {"k_pairs": "k_pairs (E = 6 camera-pair Schur products, one wavefront per pair, FP32)",
+             "cholesky": "k_chol_small (one workgroup per window: FP64 factor + both solves)",
+             "k_linearize": "k_linearize (residual + Jacobian + point blocks)"}
+    return {"metric": "batched BA solves/s (512 windows)", "value": batched, "unit": "solves/s", "steps": K,
+            "ms_per_batch_solve": 1e3 * (t1 - t0), "dtype": "fp32+fp64",
+            "config": {"workload": f"windows: {n} windows, 5..40 cameras x 6, 200..5000 points, "
+                                   f"{keys} keys, {K} LM iterations each"},
+            "one_at_a_time_solves_per_s": one_at_a_time, "speedup_vs_one_at_a_time": batched / one_at_a_time,
+            "rmse_worst_delta_vs_fp64_oracle_px": max(deltas), "oracle_checked_windows": len(deltas),
+            "statuses": sorted(set(int(s.status) for s in sts)),
+            "stage_ms": stages, "dominant_stage": dominant,
+            "roofline": dict(kernels[dom_kernel], kernel=names[dom_kernel]), "kernels": kernels,
+            "cpu_baseline": {"value": len(sample) / (c1 - c0), "unit": "solves/s", "cores": 1, "kind": "port",
+                             "sample": f"the FP64 oracle solving the first {len(sample)} windows ({K} LM "
+                                       f"iterations each, point-wise Schur + dense LDL^T), one thread, "
+                                       f"cpu={cpu_model()}"}}
+
+
+# ------------------------------------------------------------------ configs[2]
+def pcg_bytes(scene: syn.Scene, E: int = 9) -> float:
+    """Algorithmic HBM bytes of one PCG iteration of the implicit Schur
+    complement: both matvec passes read every key's Z block (3 E FP32 values,
+    point-major and image-major copies) and a 4-byte index, and the point-space
+    intermediate (3 doubles per point) is written once and read once."""
+    return float(scene.num_keys * (2 * 3 * E * 4 + 8) + scene.num_points * 48)
+
+
+def run_large(args, K=30):
+    """Large banded strip: 2000 cameras x 9, 600k points, ~5.1M keys; block-Jacobi
     PCG on the implicit Schur complement (FP32 matvec, FP64 vectors), rtol 1e-6."""
     scene = syn.large_scene()
-    K = args.steps
     p = Problem(scene, precision=abi.PRECISION_MIXED, solver=abi.SOLVER_PCG, pcg_rtol=1e-6)
     abi.check(p.lm_begin(max_iterations=K), "lm_begin")
     t0 = time.perf_counter()
@@ -193,68 +352,40 @@ def run_large(args):
     rc, x, st = p.lm_end()
     abi.check(p.lm_begin(max_iterations=K), "lm_begin")
     rc, stages = p.lm_iterate(K, timed=True)
-    p.lm_end()
-    return {"metric": "BA LM iters/s (5.1M keys, PCG)", "value": K / (t1 - t0), "unit": "iters/s",
-            "n_gpus": 1, "steps": K, "warmup": 0, "ms_per_step": 1e3 * (t1 - t0) / K,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
-            "data": "synthetic",
+    rc2, _, st2 = p.lm_end()
+    p.close()
+    # PCG loop time per PCG iteration (the host polls the convergence flags
+    # every 8 iterations, so this includes those round trips)
+    pcg_it = max(1, st2.pcg_iterations)
+    per_it_s = stages["cholesky_ms"] * 1e-3 * K / pcg_it
+    b = pcg_bytes(scene)
+    return {"metric": "BA LM iters/s (5.1M keys, PCG)", "value": K / (t1 - t0), "unit": "iters/s", "steps": K,
+            "ms_per_step": 1e3 * (t1 - t0) / K, "dtype": "fp32+fp64",
             "config": {"workload": "large: 2000 cameras x 9, 600k points, banded, PCG rtol 1e-6",
                        "keys": scene.num_keys, "reduced_size": 9 * scene.num_images},
             "stage_ms": stages,
+            "roofline": roofline_entry("hbm", b / per_it_s / 1e9, PEAK_HBM_GBS, "GB/s", b,
+                                       kernel="one PCG iteration (k_pcg_points + k_pcg_images + dot / vector "
+                                              "kernels), loop time per iteration incl. host polls"),
             "lm": {"status": st.status, "iterations": st.iterations, "accepted": st.accepted,
                    "pcg_iterations": st.pcg_iterations, "initial_cost": st.initial_cost,
                    "final_cost": st.final_cost, "rmse_px": syn.reprojection_rmse(scene, x),
                    "rmse_x0_px": syn.reprojection_rmse(scene, scene.x0),
-                   "rmse_true_px": syn.reprojection_rmse(scene, scene.x_true)}}
+                   "rmse_true_px": syn.reprojection_rmse(scene, scene.x_true)},
+            "cpu_baseline": None,
+            "cpu_baseline_note": "not timed: the oracle's dense LDL^T of the 18000 x 18000 reduced system "
+                                 "(~2e12 flop per iteration) exceeds the bounded CPU sample"}
 
 
-def run_windows(args):
-    """Batch of 512 independent windows (5..40 cameras, 200..5000 points) in
-    one batch (one set of launches per LM iteration) vs the same windows solved
-    one at a time through the single-problem path."""
-    scenes = syn.windows_batch(count=args.windows)
-    K = args.steps
-    b = Batch(scenes, precision=abi.PRECISION_MIXED)
-    b.lm(max_iterations=K)  # warm-up
-    t0 = time.perf_counter()
-    rc, xs, sts = b.lm(max_iterations=K)
-    t1 = time.perf_counter()
-    abi.check(rc, "batch lm")
-    batched = len(scenes) / (t1 - t0)
-    # one at a time (problem creation excluded, like the batch's)
-    probs = [Problem(s, precision=abi.PRECISION_MIXED) for s in scenes]
-    probs[0].lm(max_iterations=K)
-    t2 = time.perf_counter()
-    singles = [p.lm(max_iterations=K) for p in probs]
-    t3 = time.perf_counter()
-    one_at_a_time = len(scenes) / (t3 - t2)
-    for p in probs:
-        p.close()
-    # accuracy on a sample against the FP64 oracle (CPU)
-    from oracle.oracle import Oracle  # checker only
-    worst = 0.0
-    sample = list(range(0, len(scenes), max(1, len(scenes) // args.check_windows)))
-    for w in sample:
-        _, xo, _, _ = Oracle(scenes[w]).lm(max_iterations=K)
-        worst = max(worst, abs(syn.reprojection_rmse(scenes[w], xs[w]) - syn.reprojection_rmse(scenes[w], xo)))
-    keys = sum(s.num_keys for s in scenes)
-    return {"metric": "batched BA solves/s (512 windows)", "value": batched, "unit": "solves/s",
-            "n_gpus": 1, "steps": K, "warmup": 1, "ms_per_step": 1e3 * (t1 - t0) / K,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
-            "data": "synthetic",
-            "config": {"workload": f"windows: {len(scenes)} windows, 5..40 cameras x 6, 200..5000 points, "
-                                   f"{keys} keys, {K} LM iterations each"},
-            "one_at_a_time_solves_per_s": one_at_a_time, "speedup_vs_one_at_a_time": batched / one_at_a_time,
-            "rmse_worst_delta_vs_fp64_oracle_px": worst, "oracle_checked_windows": len(sample),
-            "statuses": sorted(set(int(s.status) for s in sts))}
-
-
+# ------------------------------------------------------------------ configs[4]
 def run_serving(args):
     """Serving loop: Poisson arrivals at each rate in --rates; 95% of jobs from
     the windows distribution, 5% of the 200-camera x 9, ~1M-key class (the
     mid scene itself).  Jobs are drawn with replacement from a pool of
-    pre-built problems; every result is compared bit for bit with the same
-    problem solved alone, and a sample with the FP64 oracle."""
+    pre-built problems; every served result is compared bit for bit with the
+    same problem solved alone, and every pool problem's alone result with the
+    FP64 oracle (the 1M-key one with its committed 20-iteration oracle run),
+    so every served job's RMSE is checked."""
     from hs_sfm_amd.hsba import Server
     rng = np.random.default_rng(syn.SEED_SERVING)
     cams, pts = syn.window_sizes(rng, args.pool)
@@ -263,7 +394,6 @@ def run_serving(args):
     K = args.steps
     srv = Server(streams=args.streams, max_batch_jobs=args.max_batch, max_iterations=K,
                  hbm_budget_bytes=args.hbm_budget_gib << 30)
-    # the same problems alone (reference results for the bit-for-bit check)
     alone = []
     for s in pool + [mid]:
         pr = Problem(s, precision=abi.PRECISION_MIXED)
@@ -279,8 +409,7 @@ def run_serving(args):
         gaps = rng.exponential(1.0 / lam, n_jobs)
         kinds = np.where(rng.random(n_jobs) < 0.05, len(pool), rng.integers(0, len(pool), n_jobs))
         ids, which = [], []
-        t_start = srv.clock()
-        due = t_start
+        due = srv.clock()
         for g, k in zip(gaps, kinds):
             due += g
             while srv.clock() < due:
@@ -312,48 +441,26 @@ def run_serving(args):
                      "failed": bad})
         print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
     srv.close()
-    # accuracy of a sample against the FP64 oracle; the mid class against its
-    # committed 20-iteration oracle reference
-    from oracle.oracle import Oracle  # checker only
-    worst = 0.0
-    for w in range(0, len(pool), max(1, len(pool) // args.check_windows)):
-        _, xo, _, _ = Oracle(pool[w]).lm(max_iterations=K)
-        worst = max(worst, abs(syn.reprojection_rmse(pool[w], alone[w]) - syn.reprojection_rmse(pool[w], xo)))
+    ro = oracle_rmse_all(pool, K)
+    worst = max(abs(syn.reprojection_rmse(s, a) - r) for s, a, r in zip(pool, alone, ro))
     ref = oracle_reference("mid")
     mid_delta = abs(syn.reprojection_rmse(mid, alone[-1]) - ref["rmse_px"]) if ref and ref["iterations"] == K else None
     best = max(rows, key=lambda r: r["solves_per_s"])
     return {"metric": "serving solves/s at Poisson arrivals (95% windows, 5% 1M-key), p50/p99 ms",
-            "value": best["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": K, "warmup": 0,
-            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "fp32+fp64", "data": "synthetic",
+            "value": best["solves_per_s"], "unit": "solves/s", "steps": K, "dtype": "fp32+fp64",
             "config": {"workload": f"serving: pool of {len(pool)} windows + the 1M-key scene, {K} LM iterations "
                                    f"per job, {args.streams} streams, batches of <= {args.max_batch}"},
             "rates": rows, "results_identical_to_alone": mismatches == 0, "mismatches": mismatches,
-            "rmse_worst_delta_vs_fp64_oracle_px": worst, "mid_rmse_delta_vs_fp64_oracle_px": mid_delta}
+            "rmse_worst_delta_vs_fp64_oracle_px": worst, "oracle_checked_pool_problems": len(pool),
+            "mid_rmse_delta_vs_fp64_oracle_px": mid_delta}
 
 
-def run_ring(args):
-    scene = syn.ring_scene()
-    p = Problem(scene, precision=abi.PRECISION_FP64)
-    K = args.steps
-    p.lm(max_iterations=K)
-    abi.check(p.lm_begin(max_iterations=K), "lm_begin")
-    t0 = time.perf_counter()
-    abi.check(p.lm_iterate(K)[0], "lm_iterate")
-    t1 = time.perf_counter()
-    rc, x, st = p.lm_end()
-    from oracle.oracle import Oracle  # checker / CPU baseline
-    o = Oracle(scene)
-    c0 = time.perf_counter()
-    _, xo, sto, _ = o.lm(max_iterations=K, literal=True)
-    c1 = time.perf_counter()
-    return {"metric": "BA LM iters/s (ring, 10 cameras, FP64)", "value": K / (t1 - t0), "unit": "iters/s",
-            "n_gpus": 1, "steps": K, "warmup": 1, "ms_per_step": 1e3 * (t1 - t0) / K,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
-            "data": "synthetic", "config": {"workload": f"ring: 10 cameras, 500 points, {scene.num_keys} keys"},
-            "rmse_px": syn.reprojection_rmse(scene, x), "fp64_oracle_rmse_px": syn.reprojection_rmse(scene, xo),
-            "cpu_baseline": {"value": K / (c1 - c0), "unit": "iters/s", "cores": 1, "kind": "port",
-                             "sample": f"{K} LM iterations with the reference's literal O(I^2 P) Schur loops"}}
+def standalone(out, args):
+    """A sub-workload result printed as its own line (--workload X)."""
+    base = {"n_gpus": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "data": "synthetic"}
+    base.update(out)
+    return base
 
 
 def main():
@@ -361,8 +468,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="mid", choices=["mid", "large", "windows", "ring", "serving"])
-    ap.add_argument("--rates", type=float, nargs="+", default=[100, 200, 500, 1000, 2000, 5000])
+    ap.add_argument("--workload", default="default",
+                    choices=["default", "mid", "large", "windows", "ring", "serving"])
+    ap.add_argument("--extras", default="ring,windows,large,serving",
+                    help="workloads carried in the default line (comma list, or 'none')")
+    ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 2000, 5000])
     ap.add_argument("--seconds", type=float, default=1.0, help="arrival window per rate (serving)")
     ap.add_argument("--max-jobs", type=int, default=1500, help="job cap per rate (serving)")
     ap.add_argument("--pool", type=int, default=128, help="distinct window problems (serving)")
@@ -370,13 +480,17 @@ def main():
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--hbm-budget-gib", type=int, default=64)
     ap.add_argument("--windows", type=int, default=512)
-    ap.add_argument("--check-windows", type=int, default=16)
+    ap.add_argument("--cpu-windows", type=int, default=16)
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=1)
+    ap.add_argument("--cpu-samples", type=int, default=3)
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="MiB copied device-to-device before the run (reference kernel for HBM counters)")
     args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build measures one GPU (every GPU box here has one); multi-GPU "
+                                   "sharding is design-only"}))
+        sys.exit(2)
     if args.copy_calibration > 0:
         import torch  # device buffers for the calibration copy only
         src = torch.ones(args.copy_calibration << 18, dtype=torch.float32, device="cuda")
@@ -385,14 +499,35 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU sharding is design-only"}))
-        sys.exit(2)
-    run = {"mid": run_mid, "large": run_large, "windows": run_windows, "ring": run_ring,
-           "serving": run_serving}[args.workload]
-    if args.workload == "large" and args.steps == 20:
-        args.steps = 30  # the configuration's LM iteration count
-    out = run(args)
+    if args.workload != "default":
+        fn = {"mid": run_mid, "large": run_large, "windows": run_windows, "ring": run_ring,
+              "serving": run_serving}[args.workload]
+        out = fn(args)
+        print(json.dumps(out if args.workload == "mid" else standalone(out, args)))
+        return
+    out = run_mid(args)
+    extras = [] if args.extras == "none" else [e for e in args.extras.split(",") if e]
+    out["workloads"] = {}
+    for name in extras:
+        fn = {"ring": run_ring, "windows": run_windows, "large": run_large, "serving": run_serving}[name]
+        t = time.perf_counter()
+        print(f"bench: {name} ...", file=sys.stderr, flush=True)
+        try:
+            r = fn(args)
+        except Exception as e:  # a failing extra leaves the headline line intact
+            traceback.print_exc()
+            r = {"error": f"{type(e).__name__}: {e}"}
+        r["wall_s"] = time.perf_counter() - t
+        out["workloads"][name] = r
+        print(f"bench: {name} done in {r['wall_s']:.1f} s", file=sys.stderr, flush=True)
+    w = out["workloads"].get("windows", {})
+    if "value" in w:
+        out["batched_solves_per_s"] = w["value"]
+        out["one_at_a_time_solves_per_s"] = w["one_at_a_time_solves_per_s"]
+        out["batched_speedup"] = w["speedup_vs_one_at_a_time"]
+    s = out["workloads"].get("serving", {})
+    if "rates" in s:
+        out["serving_p99_ms"] = {str(r["rate_per_s"]): r["p99_ms"] for r in s["rates"]}
     print(json.dumps(out))
 
 

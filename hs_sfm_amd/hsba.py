@@ -244,6 +244,30 @@ class Batch:
         rc = self.lib.hsba_batch_lm_solve(self.h, X0, Y, self.ycov_arr, C.byref(opt), XO, st)
         return rc, xs, list(st)
 
+    def lm_begin(self, x0s=None, max_iterations=20, tau=1e-3, eps1=0.0, eps2=0.0):
+        """Stepwise batch solve (hsba_batch_lm_begin / _iterate / _end)."""
+        n = len(self.scenes)
+        x0s = [s.x0 for s in self.scenes] if x0s is None else x0s
+        self._x0 = [np.ascontiguousarray(x, np.float64) for x in x0s]
+        self._y = [np.ascontiguousarray(s.y_obs, np.float64) for s in self.scenes]
+        X0 = (_dp * n)(*[abi.dptr(a) for a in self._x0])
+        Y = (_dp * n)(*[abi.dptr(a) for a in self._y])
+        opt = abi.hsba_lm_options(max_iterations=max_iterations, tau=tau, eps1=eps1, eps2=eps2)
+        return self.lib.hsba_batch_lm_begin(self.h, X0, Y, self.ycov_arr, C.byref(opt))
+
+    def lm_iterate(self, iterations, timed=False):
+        t = abi.hsba_stage_times() if timed else None
+        rc = self.lib.hsba_batch_lm_iterate(self.h, iterations, C.byref(t) if timed else None)
+        return rc, (t.as_dict() if timed else None)
+
+    def lm_end(self):
+        n = len(self.scenes)
+        xs = [np.zeros(s.x_size) for s in self.scenes]
+        XO = (_dp * n)(*[abi.dptr(a) for a in xs])
+        st = (abi.hsba_lm_stats * n)()
+        rc = self.lib.hsba_batch_lm_end(self.h, XO, st)
+        return rc, xs, list(st)
+
 
 class NaiveProblem:
     """ba_naive problem over the C ABI: x = [cameras 6C | points 3P], features
