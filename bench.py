@@ -277,6 +277,10 @@ def run_windows(args):
     abi.check(rc, "batch lm_iterate(timed)")
     b.lm_end()
     b.close()
+    keys = sum(s.num_keys for s in scenes)
+    if args.batch_only:  # kernel profiles of the batched launch alone
+        return {"metric": "batched BA solves/s (512 windows)", "value": batched, "unit": "solves/s", "steps": K,
+                "config": {"workload": f"windows: {n} windows, {keys} keys, batch only"}, "stage_ms": stages}
     # one at a time: one live problem (its own stream and graph), created,
     # warmed up, solved (timed) and destroyed in turn
     single_s = 0.0
@@ -299,7 +303,6 @@ def run_windows(args):
     for s in sample:
         Oracle(s).lm(max_iterations=K)
     c1 = time.perf_counter()
-    keys = sum(s.num_keys for s in scenes)
     pf, cfl = window_flops(scenes)
     lin = sum(linearize_bytes(s, 6, 4) for s in scenes)
     kernels = {
@@ -481,6 +484,8 @@ def main():
     ap.add_argument("--hbm-budget-gib", type=int, default=64)
     ap.add_argument("--windows", type=int, default=512)
     ap.add_argument("--cpu-windows", type=int, default=16)
+    ap.add_argument("--batch-only", action="store_true",
+                    help="windows: only the batched solve (no one-at-a-time leg, no oracle checks)")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=3)
