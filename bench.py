@@ -197,7 +197,7 @@ def run_mid(args):
                    "cameras": scene.num_images, "points": scene.num_points, "keys": scene.num_keys,
                    "precision": args.precision, "reduced_size": n},
         "roofline": roof, "kernels": kernels,
-        "stage_ms": stages, "dominant_stage": dominant,
+        "stage_ms": stages, "dominant_stage": dominant, "autotuner": abi.tuning_table(),
         "lm": {"status": st.status, "iterations": st.iterations, "accepted": st.accepted,
                "initial_cost": st.initial_cost, "final_cost": st.final_cost, "rmse_px": rmse,
                "mean_error_px": syn.mean_reprojection_error(scene, x),
@@ -315,7 +315,8 @@ def run_windows(args):
                     "assemble_ms", "backsub_ms", "step_cost_ms"), key=lambda k: stages[k])
     dom_kernel = {"pairs_ms": "k_pairs", "cholesky_ms": "cholesky", "linearize_ms": "k_linearize"}.get(dominant,
                                                                                                        "k_pairs")
-    names = {"k_pairs": "k_pairs (E = 6 camera-pair Schur products, one wavefront per pair, FP32)",
+    names = {"k_pairs": "camera-pair Schur products, E = 6, FP32 (k_pairs_mma or k_pairs per window class, "
+                        "as the autotuner chose)",
              "cholesky": "k_chol_small (one workgroup per window: FP64 factor + both solves)",
              "k_linearize": "k_linearize (residual + Jacobian + point blocks)"}
     return {"metric": "batched BA solves/s (512 windows)", "value": batched, "unit": "solves/s", "steps": K,
@@ -325,7 +326,7 @@ def run_windows(args):
             "one_at_a_time_solves_per_s": one_at_a_time, "speedup_vs_one_at_a_time": batched / one_at_a_time,
             "rmse_worst_delta_vs_fp64_oracle_px": max(deltas), "oracle_checked_windows": len(deltas),
             "statuses": sorted(set(int(s.status) for s in sts)),
-            "stage_ms": stages, "dominant_stage": dominant,
+            "stage_ms": stages, "dominant_stage": dominant, "autotuner": abi.tuning_table(),
             "roofline": dict(kernels[dom_kernel], kernel=names[dom_kernel]), "kernels": kernels,
             "cpu_baseline": {"value": len(sample) / (c1 - c0), "unit": "solves/s", "cores": 1, "kind": "port",
                              "sample": f"the FP64 oracle solving the first {len(sample)} windows ({K} LM "

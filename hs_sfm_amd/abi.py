@@ -28,6 +28,7 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 SOLVER_AUTO, SOLVER_DENSE, SOLVER_PCG = 0, 1, 2
 LOSS_TRIVIAL, LOSS_HUBER = 0, 1
+PAIRS_TUNED, PAIRS_VALU, PAIRS_MFMA = 0, 1, 2
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -66,7 +67,12 @@ class hsba_ycov_inverse(C.Structure):
 class hsba_options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("solver", C.c_int32), ("pcg_rtol", C.c_double),
                 ("pcg_max_iterations", C.c_int32), ("point_group_lanes", C.c_int32),
-                ("loss", C.c_int32), ("reserved_", C.c_int32), ("loss_scale", C.c_double)]
+                ("loss", C.c_int32), ("pair_kernel", C.c_int32), ("loss_scale", C.c_double)]
+
+
+class hsba_tune_entry(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("camera_block", C.c_int32), ("bucket", C.c_int32),
+                ("variant", C.c_int32), ("ms", C.c_double * 2), ("pairs", C.c_int64)]
 
 
 class hsba_lm_options(C.Structure):
@@ -261,6 +267,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.hsba_gcp_mahalanobis.argtypes = [P, _dp, _dp, _dp, _dp, _dp]
     lib.hsba_gcp_lm_solve.argtypes = [P, _dp, _dp, _dp, _dp, C.POINTER(hsba_lm_options), _dp,
                                       C.POINTER(hsba_lm_stats)]
+    lib.hsba_tuning_table.argtypes = [C.POINTER(hsba_tune_entry), C.c_int32, C.POINTER(C.c_int32)]
+    lib.hsba_tuning_reset.argtypes = []
     lib.hsba_problem_structure.argtypes = [P, C.POINTER(hsba_structure_sizes), _ip, _ip, _ip, _ip, _ip,
                                            _ip, C.POINTER(C.c_int64), _ip]
     lib.hsba_coupling_blocks.argtypes = [P, _dp, _dp, _dp]
@@ -283,7 +291,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 EXPORTED_SYMBOLS = [
     "hsba_version", "hsba_device_count", "hsba_problem_create", "hsba_problem_destroy",
     "hsba_problem_sizes", "hsba_evaluate", "hsba_jacobian", "hsba_build_normal",
-    "hsba_solve", "hsba_covariance", "hsba_covariance_static", "hsba_problem_structure", "hsba_coupling_blocks", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
+    "hsba_solve", "hsba_covariance", "hsba_covariance_static", "hsba_problem_structure", "hsba_coupling_blocks",
+    "hsba_tuning_table", "hsba_tuning_reset", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
     "hsba_gradient_dot", "hsba_mahalanobis", "hsba_lm_solve", "hsba_lm_begin",
     "hsba_lm_iterate", "hsba_lm_end", "hsba_batch_create",
     "hsba_batch_destroy", "hsba_batch_lm_solve", "hsba_batch_lm_begin", "hsba_batch_lm_iterate",
@@ -301,3 +310,18 @@ EXPORTED_SYMBOLS = [
 def check(rc: int, what: str = ""):
     if rc != HSBA_OK:
         raise RuntimeError(f"{what} failed: {rc} ({ERRORS.get(rc, '?')})")
+
+
+def tuning_table():
+    """The autotuner's decisions: list of dicts (precision, camera block E,
+    incidence bucket, chosen pair kernel, measured ms of each candidate)."""
+    lib = load_library()
+    n = C.c_int32()
+    check(lib.hsba_tuning_table(None, 0, C.byref(n)), "hsba_tuning_table")
+    arr = (hsba_tune_entry * max(1, n.value))()
+    check(lib.hsba_tuning_table(arr, n.value, C.byref(n)), "hsba_tuning_table")
+    names = ["valu_wave_per_pair", "mfma_workgroup_per_pair"]
+    return [{"precision": "fp64" if e.precision == PRECISION_FP64 else "mixed", "camera_block": e.camera_block,
+             "incidences_per_pair_bucket": f"[{2 ** e.bucket}, {2 ** (e.bucket + 1)})",
+             "chosen": names[e.variant], "ms": {names[0]: e.ms[0], names[1]: e.ms[1]}, "pairs_timed": e.pairs}
+            for e in arr[:n.value]]

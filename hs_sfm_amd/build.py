@@ -27,6 +27,7 @@ SOURCES = [
     "hsba_naive.cpp",
     "hsba_io.cpp",
     "hsba_api.cpp",
+    "hsba_tune.cpp",
 ]
 ARCH = os.environ.get("HSBA_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

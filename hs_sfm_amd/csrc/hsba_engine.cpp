@@ -58,8 +58,10 @@ struct Engine final : EngineBase {
   std::vector<void *> owned;
   hipStream_t st = nullptr;
   bool own_stream = false;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;  // `graph_iters` LM iterations in one graph
   hipGraph_t graph = nullptr;
+  int graph_iters = 0;
+  static constexpr int kMaxGraphIterations = 64;
   int *small_list_dev = nullptr;
   CholPlan plan;
   std::vector<WinState> ws_host;
@@ -237,6 +239,106 @@ struct Engine final : EngineBase {
     HCHK(hipGetLastError());
     ws_host.assign(L.nwin, WinState());
     HCHK(hipStreamSynchronize(st));
+    return setup_pairs();
+  }
+
+  // ---- K4 kernel choice (autotuner) -------------------------------------
+  // Pairs run by each pair kernel (index 0 VALU, 1 matrix cores); a list that
+  // covers every pair is passed as null (pairs 0 .. npairs).
+  int32_t *plist_dev[2] = {nullptr, nullptr};
+  int plist_n[2] = {0, 0};
+  bool plist_all[2] = {false, false};
+
+  void enqueue_pairs() {
+    for (int v = 0; v < 2; ++v)
+      if (plist_n[v]) ks->pairs(d, st, v, plist_all[v] ? nullptr : plist_dev[v], plist_n[v]);
+  }
+
+  // Average device time (ms) of `reps` launches of pair kernel `variant` over
+  // the device list `dl` of `n` pairs, after one untimed launch.
+  int time_pairs(int variant, const int32_t *dl, int n, int reps, double *ms) {
+    hipEvent_t e0, e1;
+    HCHK(hipEventCreate(&e0));
+    HCHK(hipEventCreate(&e1));
+    ks->pairs(d, st, variant, dl, n);
+    HCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) ks->pairs(d, st, variant, dl, n);
+    HCHK(hipEventRecord(e1, st));
+    HCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+    return HSBA_OK;
+  }
+
+  // Chooses the pair kernel of every window.  A window's class is (precision,
+  // E, floor(log2(mean incidences per pair))); the first engine that meets a
+  // class with two candidate kernels times both on that class's pairs (zeroed
+  // Z rows, every window in a trial) and the process-wide table keeps the
+  // faster one.  The matrix-core kernel is a candidate for E <= 15 in mixed
+  // precision (the FP64 path keeps the register-tile kernel for E <= 6).
+  int setup_pairs() {
+    if (!L.pts_free || L.E == 0 || L.npairs == 0) return HSBA_OK;
+    std::vector<int64_t> inc_w(L.nwin, 0), np_w(L.nwin, 0);
+    for (int64_t q = 0; q < L.npairs; ++q) {
+      const int w = L.pair_win[q];
+      np_w[w] += 1;
+      inc_w[w] += L.pair_ptr[q + 1] - L.pair_ptr[q];
+    }
+    std::map<int, std::vector<int>> classes;  // bucket -> windows
+    for (int w = 0; w < L.nwin; ++w) {
+      if (!np_w[w]) continue;
+      const double mean = (double)inc_w[w] / (double)np_w[w];
+      int b = 0;
+      while (b < 40 && std::ldexp(1.0, b + 1) <= mean) ++b;
+      classes[b].push_back(w);
+    }
+    const bool mfma = ks->has_pair_variant(1) && precision == HSBA_PRECISION_MIXED;
+    std::vector<int> wvar(L.nwin, 0);
+    {
+      std::lock_guard<std::mutex> g(tune_mutex());
+      for (const auto &kv : classes) {
+        int var = 0;
+        if (pair_kernel == HSBA_PAIRS_VALU || !mfma) {
+          var = 0;
+        } else if (pair_kernel == HSBA_PAIRS_MFMA) {
+          var = 1;
+        } else if (!tune_lookup_locked(precision, L.E, kv.first, &var)) {
+          std::vector<char> in(L.nwin, 0);
+          for (int w : kv.second) in[w] = 1;
+          std::vector<int32_t> cl;
+          for (int64_t q = 0; q < L.npairs; ++q)
+            if (in[L.pair_win[q]]) cl.push_back((int32_t)q);
+          int32_t *dl = nullptr;
+          int rc = dalloc(dl, (int64_t)cl.size(), owned, arena);
+          if (rc) return rc;
+          if ((rc = upload(dl, cl, st))) return rc;
+          if ((rc = set_state(nullptr, 0.0, 1, 0, 1))) return rc;
+          HCHK(hipMemsetAsync(d.Z, 0, sizeof(T) * (size_t)z_stride(L.E, sizeof(T)) * (size_t)std::max<int64_t>(1, L.N), st));
+          HCHK(hipMemsetAsync(d.q, 0, sizeof(T) * 3 * (size_t)std::max<int64_t>(1, L.P), st));
+          double ms[2] = {0.0, 0.0};
+          for (int v = 0; v < 2; ++v)
+            if ((rc = time_pairs(v, dl, (int)cl.size(), 3, &ms[v]))) return rc;
+          var = ms[1] < ms[0] ? 1 : 0;
+          tune_store_locked(precision, L.E, kv.first, var, ms, (int64_t)cl.size());
+        }
+        for (int w : kv.second) wvar[w] = var;
+      }
+    }
+    std::vector<int32_t> lists[2];
+    for (int64_t q = 0; q < L.npairs; ++q) lists[wvar[L.pair_win[q]]].push_back((int32_t)q);
+    for (int v = 0; v < 2; ++v) {
+      plist_n[v] = (int)lists[v].size();
+      plist_all[v] = plist_n[v] == L.npairs;
+      if (plist_n[v] && !plist_all[v]) {
+        int rc = dalloc(plist_dev[v], plist_n[v], owned, arena);
+        if (rc) return rc;
+        if ((rc = upload(plist_dev[v], lists[v], st))) return rc;
+      }
+    }
+    HCHK(hipStreamSynchronize(st));
     return HSBA_OK;
   }
 
@@ -385,7 +487,7 @@ struct Engine final : EngineBase {
   void enqueue_trial() {
     ks->point_factor(d, st);
     ks->key_z(d, st);
-    ks->pairs(d, st);
+    enqueue_pairs();
     ks->assemble(d, st, L.lower_total, (int)L.rows_total);
     chol_factor_solve_impl(plan, small_list_dev, (int)L.small_list.size(), L.large_list.data(),
                            (int)L.large_list.size(), st);
@@ -404,7 +506,7 @@ struct Engine final : EngineBase {
       ks->lm_control(d, st, 1);
       ks->point_factor(d, st);
       ks->key_z(d, st);
-      ks->pairs(d, st);
+      enqueue_pairs();
       ks->assemble(d, st, 0, (int)L.rows_total);
       ks->pcg_setup(d, st, glog2);
     } else if (part == 1) {
@@ -556,7 +658,7 @@ struct Engine final : EngineBase {
         ks->point_factor(d, st);
         ks->key_z(d, st);
         break;
-      case 1: ks->pairs(d, st); break;  // K4
+      case 1: enqueue_pairs(); break;  // K4
       case 2: ks->assemble(d, st, L.lower_total, (int)L.rows_total); break;
       case 3:  // K5
         chol_factor_solve_impl(plan, small_list_dev, (int)L.small_list.size(), L.large_list.data(),
@@ -585,17 +687,36 @@ struct Engine final : EngineBase {
     if (iterations <= 0) return HSBA_OK;
     if (L.pcg) return pcg_iterate(iterations, times);
     if (!times) {
-      if (!gexec) {
+      // The whole run of `iterations` LM iterations is one captured graph and
+      // one launch (up to kMaxGraphIterations; longer runs replay a graph of
+      // that many iterations).  The graph is kept for the next call with the
+      // same count.
+      const int n = std::min(iterations, kMaxGraphIterations);
+      if (!gexec || graph_iters != n) {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
         HCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        enqueue_iteration();
+        for (int it = 0; it < n; ++it) enqueue_iteration();
         hipError_t e1 = hipStreamEndCapture(st, &graph);
         if (e1 != hipSuccess) {
           std::fprintf(stderr, "hsba: capture failed: %s\n", hipGetErrorString(e1));
           return HSBA_ERR_DEVICE;
         }
         HCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+        graph_iters = n;
       }
-      for (int it = 0; it < iterations; ++it) HCHK(hipGraphLaunch(gexec, st));
+      int left = iterations;
+      while (left >= n) {
+        HCHK(hipGraphLaunch(gexec, st));
+        left -= n;
+      }
+      if (left > 0) {
+        // remainder: eager launches (same kernels, same order)
+        for (int it = 0; it < left; ++it) enqueue_iteration();
+        HCHK(hipGetLastError());
+      }
       HCHK(hipStreamSynchronize(st));
       return HSBA_OK;
     }
@@ -784,7 +905,7 @@ struct Engine final : EngineBase {
     if (rc) return rc;
     ks->point_factor(d, st);
     ks->key_z(d, st);
-    ks->pairs(d, st);
+    enqueue_pairs();
     ks->assemble(d, st, L.lower_total, (int)L.rows_total);
     chol_factor_solve_impl(plan, small_list_dev, (int)L.small_list.size(), L.large_list.data(),
                            (int)L.large_list.size(), st);
@@ -872,7 +993,7 @@ struct Engine final : EngineBase {
       if ((rc = capture_pcg())) return rc;
       ks->point_factor(d, st);
       ks->key_z(d, st);
-      ks->pairs(d, st);
+      enqueue_pairs();
       ks->assemble(d, st, 0, (int)L.rows_total);
       ks->pcg_setup(d, st, glog2);
       if ((rc = pcg_solve_loop())) return rc;
@@ -891,7 +1012,7 @@ struct Engine final : EngineBase {
     }
     ks->point_factor(d, st);
     ks->key_z(d, st);
-    ks->pairs(d, st);
+    enqueue_pairs();
     ks->assemble(d, st, L.lower_total, (int)L.rows_total);
     if (S || rhs) {
       HCHK(hipStreamSynchronize(st));

@@ -84,6 +84,13 @@ int guarded(F &&f) noexcept {
   }
 }
 
+// Autotuner table (hsba_tune.cpp).  Callers hold tune_mutex() around a
+// lookup, the timing of the candidates and the store, so one class is
+// measured once even when several engines are created concurrently.
+std::mutex &tune_mutex();
+bool tune_lookup_locked(int precision, int E, int bucket, int *variant);
+void tune_store_locked(int precision, int E, int bucket, int variant, const double ms[2], int64_t pairs);
+
 // First-fit sub-allocator over one device allocation, with coalescing of
 // freed neighbours (256-byte granules).  Thread-safe.  The serving loop
 // allocates its HBM budget once and every job's buffers come from it, so no
@@ -116,6 +123,7 @@ struct EngineBase {
   int loss = HSBA_LOSS_TRIVIAL;
   double loss_scale = 1.0;
   int glog2 = 3;
+  int pair_kernel = 0;  // 0 tuned, 1 wave-per-pair VALU, 2 matrix cores
   virtual int init(hipStream_t stream) = 0;
   virtual int upload_x(const double *const *xs) = 0;
   virtual int upload_obs(const double *const *ys, const hsba_ycov_inverse *ycovs) = 0;

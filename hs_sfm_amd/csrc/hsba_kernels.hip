@@ -378,13 +378,14 @@ __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
 // fixed butterfly.  Replaces the O(I^2 P) loops of
 // camera_shared_normal_equation_solver.hpp:198-443.
 template <typename T, int E>
-__global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
+__global__ void __launch_bounds__(256) k_pairs(Dev<T> d, const int32_t *plist, int count) {
   constexpr int ZS = z_stride(E, sizeof(T));
   constexpr int VW = 16 / sizeof(T), NV = (3 * E + VW - 1) / VW;
   typedef T vec __attribute__((ext_vector_type(VW)));
-  const int pair = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (pair >= d.npairs) return;
+  if (slot >= count) return;
+  const int pair = plist ? plist[slot] : slot;
   if (!win_trial(d.ws[d.pair_win[pair]])) return;
   const bool diag = d.pair_img[pair] >= 0;
   T acc[E * E];
@@ -457,7 +458,7 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d) {
   }
 }
 
-// K4 on the matrix cores for wide camera blocks (7 <= E <= 15).  For a pair
+// K4 on the matrix cores (camera blocks up to E = 15).  For a pair
 // (a, b), P_ab = A^T B where A and B stack the Z rows of the shared points (3
 // per point); for a diagonal pair column E of B carries q_p, so column E of
 // the product is rp_i.  One workgroup per pair walks its incidences 64 at a
@@ -490,8 +491,8 @@ struct Mma<double> {
 };
 
 template <typename T, int E>
-__global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d) {
-  static_assert(E >= 7 && E <= 15, "wide camera blocks, rhs column");
+__global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d, const int32_t *plist) {
+  static_assert(E >= 1 && E <= 15, "one 16-wide tile holds the E columns and the rhs column");
   constexpr int ZS = z_stride(E, sizeof(T));
   constexpr int VW = 16 / sizeof(T), NV = ZS / VW;
   constexpr int SI = 64;  // incidences per stage
@@ -500,7 +501,7 @@ __global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d) {
   __shared__ vec sz[2][SI][NV];
   __shared__ T sq[SI][4];
   __shared__ T part[3][64][4];
-  const int pair = blockIdx.x;
+  const int pair = plist ? plist[blockIdx.x] : blockIdx.x;
   if (!win_trial(d.ws[d.pair_win[pair]])) return;
   const bool diag = d.pair_img[pair] >= 0;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
@@ -1459,12 +1460,16 @@ struct KernelSetImpl final : KernelSet<T> {
     if (!d.pts_free || E == 0 || d.N == 0) return;
     hipLaunchKernelGGL((k_key_z<T, (E > 0 ? E : 1)>), dim3((d.N + 63) / 64), dim3(256), 0, st, d);
   }
-  void pairs(const Dev<T> &d, hipStream_t st) override {
-    if (!d.pts_free || E == 0 || d.npairs == 0) return;
-    if constexpr (E >= 7 && E <= 15)
-      hipLaunchKernelGGL((k_pairs_mma<T, E>), dim3(d.npairs), dim3(256), 0, st, d);
-    else
-      hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3((d.npairs + 3) / 4), dim3(256), 0, st, d);
+  bool has_pair_variant(int variant) const override { return variant == 0 ? E > 0 : (E >= 1 && E <= 15); }
+  void pairs(const Dev<T> &d, hipStream_t st, int variant, const int32_t *plist, int count) override {
+    if (!d.pts_free || E == 0 || count <= 0) return;
+    if constexpr (E >= 1 && E <= 15) {
+      if (variant == 1) {
+        hipLaunchKernelGGL((k_pairs_mma<T, E>), dim3(count), dim3(256), 0, st, d, plist);
+        return;
+      }
+    }
+    hipLaunchKernelGGL((k_pairs<T, (E > 0 ? E : 1)>), dim3((count + 3) / 4), dim3(256), 0, st, d, plist, count);
   }
   void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) override {
     if (E == 0 || rows == 0) return;

@@ -28,23 +28,26 @@ _dp = C.POINTER(C.c_double)
 
 
 def _opts(precision=abi.PRECISION_MIXED, solver=abi.SOLVER_AUTO, pcg_rtol=1e-6,
-          pcg_max_iterations=500, point_group_lanes=0, loss=abi.LOSS_TRIVIAL, loss_scale=1.0):
+          pcg_max_iterations=500, point_group_lanes=0, loss=abi.LOSS_TRIVIAL, loss_scale=1.0,
+          pair_kernel=abi.PAIRS_TUNED):
     return abi.hsba_options(precision=precision, solver=solver, pcg_rtol=pcg_rtol,
                             pcg_max_iterations=pcg_max_iterations,
-                            point_group_lanes=point_group_lanes, loss=loss, loss_scale=loss_scale)
+                            point_group_lanes=point_group_lanes, loss=loss, loss_scale=loss_scale,
+                            pair_kernel=pair_kernel)
 
 
 class Problem:
     """Device-resident problem (hsba_problem) built from a Scene."""
 
     def __init__(self, scene: Scene, precision=abi.PRECISION_MIXED, solver=abi.SOLVER_AUTO,
-                 pcg_rtol=1e-6, point_group_lanes=0, loss=abi.LOSS_TRIVIAL, loss_scale=1.0):
+                 pcg_rtol=1e-6, point_group_lanes=0, loss=abi.LOSS_TRIVIAL, loss_scale=1.0,
+                 pair_kernel=abi.PAIRS_TUNED):
         self.lib = abi.load_library()
         self.scene = scene
         self.desc = abi.Desc(scene)
         self.h = C.c_void_p()
         o = _opts(precision, solver, pcg_rtol, point_group_lanes=point_group_lanes, loss=loss,
-                  loss_scale=loss_scale)
+                  loss_scale=loss_scale, pair_kernel=pair_kernel)
         rc = self.lib.hsba_problem_create(C.byref(self.desc.desc), C.byref(o), C.byref(self.h))
         abi.check(rc, "hsba_problem_create")
         xs, ys, dd, nr = (C.c_int64() for _ in range(4))
@@ -207,7 +210,8 @@ class Problem:
 class Batch:
     """Many independent windows solved by one set of launches per LM iteration."""
 
-    def __init__(self, scenes: Sequence[Scene], precision=abi.PRECISION_MIXED, point_group_lanes=0):
+    def __init__(self, scenes: Sequence[Scene], precision=abi.PRECISION_MIXED, point_group_lanes=0,
+                 pair_kernel=abi.PAIRS_TUNED):
         self.lib = abi.load_library()
         self.scenes = list(scenes)
         self.descs = [abi.Desc(s) for s in self.scenes]
@@ -215,7 +219,7 @@ class Batch:
         self.desc_arr = (abi.hsba_problem_desc * n)(*[d.desc for d in self.descs])
         self.ycov_arr = (abi.hsba_ycov_inverse * n)(*[d.ycov for d in self.descs])
         self.h = C.c_void_p()
-        o = _opts(precision, point_group_lanes=point_group_lanes)
+        o = _opts(precision, point_group_lanes=point_group_lanes, pair_kernel=pair_kernel)
         rc = self.lib.hsba_batch_create(self.desc_arr, n, C.byref(o), C.byref(self.h))
         abi.check(rc, "hsba_batch_create")
 

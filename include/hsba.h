@@ -150,9 +150,17 @@ typedef struct {
   int32_t pcg_max_iterations;       /* default 500 */
   int32_t point_group_lanes;        /* lanes per point in point kernels: 0 = tuned */
   int32_t loss;                     /* HSBA_LOSS_* (0 = trivial)             */
-  int32_t reserved_;
+  int32_t pair_kernel;              /* HSBA_PAIRS_* (0 = chosen by the autotuner) */
   double  loss_scale;               /* a of the Huber loss (0 = 1.0)         */
 } hsba_options;
+
+/* Camera-pair Schur kernel (no reference counterpart).  TUNED: the library's
+ * autotuner times both kernels on the first problem of each window class
+ * (precision, camera block width, incidences per pair) and keeps the faster
+ * one for every later window of that class (hsba_tuning_table). */
+#define HSBA_PAIRS_TUNED  0
+#define HSBA_PAIRS_VALU   1         /* one wavefront per pair, register tiles */
+#define HSBA_PAIRS_MFMA   2         /* one workgroup per pair, matrix cores (camera block <= 15) */
 
 /* Optimizer settings: CameraSharedLevenbergMarquardtOptimizor ctor
  * (camera_shared_levenberg_marquardt_optimizor.hpp:36-41). */
@@ -358,6 +366,20 @@ typedef struct {
 } hsba_stage_times;
 int hsba_lm_iterate(hsba_problem *problem, int32_t iterations, hsba_stage_times *times);
 int hsba_lm_end(hsba_problem *problem, double *x_out, hsba_lm_stats *stats);
+
+/* The autotuner's decisions so far (process-wide).  No reference
+ * counterpart. */
+typedef struct {
+  int32_t precision;                /* HSBA_PRECISION_* */
+  int32_t camera_block;             /* E: image + camera parameters of a reduced block row */
+  int32_t bucket;                   /* floor(log2(mean incidences per camera pair)) of the windows */
+  int32_t variant;                  /* chosen pair kernel: 0 VALU, 1 matrix cores */
+  double  ms[2];                    /* measured time of each candidate (0 = not a candidate) */
+  int64_t pairs;                    /* camera pairs the candidates were timed on */
+} hsba_tune_entry;
+int hsba_tuning_table(hsba_tune_entry *entries, int32_t max_entries, int32_t *count);
+/* Forgets every decision (the next problem of each class is timed again). */
+int hsba_tuning_reset(void);
 
 /* ---- ba_naive family ---------------------------------------------------- */
 /* 6-DoF cameras in normalised image coordinates, x = [cameras 6C | points 3P]

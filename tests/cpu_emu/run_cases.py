@@ -123,6 +123,18 @@ def case_batch():
     return f"batch of {len(scenes)} windows bit-identical to the windows alone"
 
 
+def case_batch_xcd():
+    """Sixteen windows in one batch, bit-identical to each window alone."""
+    rng = np.random.default_rng(4)
+    scenes = [syn.window_scene(rng, 5 + (j % 3), 12 + j, name=f"x{j}") for j in range(16)]
+    rc, xs, _ = Batch(scenes).lm(max_iterations=1)
+    assert rc == 0
+    for s, x in zip(scenes, xs):
+        rc1, xa, _ = Problem(s).lm(max_iterations=1)
+        assert rc1 == 0 and np.array_equal(x, xa)
+    return f"batch of {len(scenes)} windows bit-identical to the windows alone"
+
+
 def case_covariance_static():
     s = sparse_ring(12, 30, seed=7)
     ids = [0, 1, 2]
@@ -138,7 +150,7 @@ def case_covariance_static():
 
 
 CASES = {"tiled_cholesky": case_tiled_cholesky, "small_mma": case_small_mma, "lm_tiled": case_lm_tiled,
-         "batch": case_batch, "covariance_static": case_covariance_static}
+         "batch": case_batch, "batch_xcd": case_batch_xcd, "covariance_static": case_covariance_static}
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)
