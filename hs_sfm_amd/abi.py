@@ -29,6 +29,8 @@ PRECISION_MIXED = 1
 SOLVER_AUTO, SOLVER_DENSE, SOLVER_PCG = 0, 1, 2
 LOSS_TRIVIAL, LOSS_HUBER = 0, 1
 PAIRS_TUNED, PAIRS_VALU, PAIRS_MFMA = 0, 1, 2
+COV_AUTO, COV_SCHUR, COV_DENSE = 0, 1, 2
+COV_DENSE_MAX = 4096
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -73,6 +75,11 @@ class hsba_options(C.Structure):
 class hsba_tune_entry(C.Structure):
     _fields_ = [("precision", C.c_int32), ("camera_block", C.c_int32), ("bucket", C.c_int32),
                 ("variant", C.c_int32), ("ms", C.c_double * 2), ("pairs", C.c_int64)]
+
+
+class hsba_covariance_info(C.Structure):
+    _fields_ = [("method", C.c_int32), ("rank", C.c_int32), ("sweeps", C.c_int32), ("reserved_", C.c_int32),
+                ("smallest_kept", C.c_double)]
 
 
 class hsba_lm_options(C.Structure):
@@ -274,6 +281,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
     lib.hsba_coupling_blocks.argtypes = [P, _dp, _dp, _dp]
     lib.hsba_covariance.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
     lib.hsba_covariance_static.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), _dp, _dp, _dp, _dp]
+    lib.hsba_x_covariance.argtypes = [P, _dp, _dp, C.POINTER(hsba_ycov_inverse), C.c_int32, _dp,
+                                      C.POINTER(hsba_covariance_info)]
     lib.hsba_batch_lm_begin.argtypes = [P, C.POINTER(_dp), C.POINTER(_dp), C.POINTER(hsba_ycov_inverse),
                                         C.POINTER(hsba_lm_options)]
     lib.hsba_batch_lm_iterate.argtypes = [P, C.c_int32, C.POINTER(hsba_stage_times)]
@@ -291,7 +300,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
 EXPORTED_SYMBOLS = [
     "hsba_version", "hsba_device_count", "hsba_problem_create", "hsba_problem_destroy",
     "hsba_problem_sizes", "hsba_evaluate", "hsba_jacobian", "hsba_build_normal",
-    "hsba_solve", "hsba_covariance", "hsba_covariance_static", "hsba_problem_structure", "hsba_coupling_blocks",
+    "hsba_solve", "hsba_covariance", "hsba_covariance_static", "hsba_x_covariance", "hsba_problem_structure", "hsba_coupling_blocks",
     "hsba_tuning_table", "hsba_tuning_reset", "hsba_reduced_system", "hsba_max_diagonal", "hsba_gradient_inf_norm",
     "hsba_gradient_dot", "hsba_mahalanobis", "hsba_lm_solve", "hsba_lm_begin",
     "hsba_lm_iterate", "hsba_lm_end", "hsba_batch_create",

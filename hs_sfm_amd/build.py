@@ -21,6 +21,7 @@ SOURCES = [
     "hsba_kernels.hip",
     "hsba_cholesky.hip",
     "hsba_scratch.hip",
+    "hsba_dense.hip",
     "hsba_layout.cpp",
     "hsba_engine.cpp",
     "hsba_server.cpp",

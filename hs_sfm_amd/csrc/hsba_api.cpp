@@ -213,9 +213,8 @@ namespace {
 
 // hsba_covariance and hsba_covariance_static: scratch == nullptr allocates the
 // device buffers of the inverse for the call, else uses the fixed region.
-int covariance_impl(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
-                    double *point_blocks, double *image_blocks, double *camera_blocks, double *reduced,
-                    double *scratch) {
+// Normal equations at x (window 0) for the covariance entry points.
+int build_at(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov) {
   int rc = set_x(p, x);
   if (rc) return rc;
   // N does not depend on y except through robust-loss weights
@@ -226,7 +225,14 @@ int covariance_impl(hsba_problem *p, const double *x, const double *y_obs, const
   }
   const double *ys[1] = {y_obs};
   if ((rc = p->eng->upload_obs(ys, ycov))) return rc;
-  if ((rc = p->eng->build_normal(nullptr, nullptr, nullptr, nullptr))) return rc;
+  return p->eng->build_normal(nullptr, nullptr, nullptr, nullptr);
+}
+
+int covariance_impl(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
+                    double *point_blocks, double *image_blocks, double *camera_blocks, double *reduced,
+                    double *scratch) {
+  const int rc = build_at(p, x, y_obs, ycov);
+  if (rc) return rc;
   return p->eng->covariance(point_blocks, image_blocks, camera_blocks, reduced, scratch);
 }
 
@@ -255,6 +261,16 @@ int hsba_covariance_static(hsba_problem *p, const double *x, const double *y_obs
     double *scratch = nullptr;
     if (hsba::scratch_region(&scratch)) return HSBA_ERR_DEVICE;
     return covariance_impl(p, x, y_obs, ycov, point_blocks, image_blocks, camera_blocks, reduced, scratch);
+  });
+}
+
+int hsba_x_covariance(hsba_problem *p, const double *x, const double *y_obs, const hsba_ycov_inverse *ycov,
+                      int method, double *x_covariance, hsba_covariance_info *info) {
+  return hsba::guarded([&]() -> int {
+    if (!p || !x || !x_covariance) return HSBA_ERR_INVALID;
+    int rc = build_at(p, x, y_obs, ycov);
+    if (rc) return rc;
+    return p->eng->x_covariance(x_covariance, method, info);
   });
 }
 

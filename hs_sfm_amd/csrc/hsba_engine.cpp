@@ -42,6 +42,26 @@ int dalloc(X *&p, int64_t n, std::vector<void *> &owned, DeviceArena *arena) {
   return HSBA_OK;
 }
 
+// Device buffers of one call (hipMalloc), freed when the call returns.
+struct CallBuffers {
+  std::vector<void *> ptrs;
+  ~CallBuffers() {
+    for (void *p : ptrs) (void)hipFree(p);
+  }
+  template <typename X>
+  int get(X *&p, int64_t n) {
+    p = nullptr;
+    if (hipMalloc((void **)&p, sizeof(X) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return HSBA_ERR_OOM;
+    ptrs.push_back((void *)p);
+    return HSBA_OK;
+  }
+};
+
+// Singular values at or below this are dropped from the pseudo-inverse
+// (camera_shared_analytical_x_covariance_calculator.hpp:78).
+constexpr double kPinvThreshold = 1e-6;
+constexpr int kJacobiMaxSweeps = 64;
+
 template <typename X>
 int upload(X *dst, const std::vector<X> &src, hipStream_t st) {
   if (src.empty()) return HSBA_OK;
@@ -979,6 +999,119 @@ struct Engine final : EngineBase {
     if (red)
       for (int64_t i = 0; i < n; ++i)
         for (int64_t j = 0; j < n; ++j) red[i * n + j] = hS[(size_t)i * ld + j];
+    return HSBA_OK;
+  }
+
+  // Point factors, Z, pairs, the reduced system and its factorization of
+  // window 0 at damping mu (no LM state); the state is read back, so
+  // ws_host[0].chol_fail tells whether N + mu I was positive definite.
+  int factor_at(double mu) {
+    int rc = set_state(nullptr, mu, 1, 0, 1);
+    if (rc) return rc;
+    ks->point_factor(d, st);
+    ks->key_z(d, st);
+    enqueue_pairs();
+    ks->assemble(d, st, L.lower_total, (int)L.rows_total);
+    chol_factor_solve_impl(plan, small_list_dev, (int)L.small_list.size(), L.large_list.data(),
+                           (int)L.large_list.size(), st);
+    HCHK(hipGetLastError());
+    return get_state();
+  }
+
+  // Full dense covariance (window 0) in the x layout: the pseudo-inverse of N
+  // with singular values at or below kPinvThreshold dropped.  Schur route:
+  // C_rr = S^-1, C_pr = -G S^-1 and C_pp = V^-1 + G S^-1 G^T with the dense
+  // point rows G = V^-1 W = L^-T Z, as matrix-core GEMMs.  Dense route: N
+  // assembled densely (V blocks, W rows and their transpose, U = the reduced
+  // matrix without pair terms or damping) and pseudo-inverted through its
+  // eigen-decomposition by the cyclic Jacobi method.
+  int x_covariance(double *out, int method, hsba_covariance_info *info) override {
+    if (!built) return HSBA_ERR_GENERIC;
+    if (L.pcg || L.nwin != 1) return HSBA_ERR_UNSUPPORTED;
+    if (L.hw[0].unobserved) return HSBA_ERR_NOT_SPD;
+    if (method < HSBA_COV_AUTO || method > HSBA_COV_DENSE) return HSBA_ERR_INVALID;
+    const WinInfo &wi = L.wi[0];
+    const int64_t n = wi.n, ld = wi.ld;
+    const int64_t P3 = L.pts_free ? 3 * L.P : 0, xs = P3 + n;
+    hsba_covariance_info inf;
+    std::memset(&inf, 0, sizeof(inf));
+    if (xs == 0) {
+      if (info) *info = inf;
+      return HSBA_OK;
+    }
+    int rc;
+    bool schur = method == HSBA_COV_SCHUR;
+    if (method == HSBA_COV_AUTO) {
+      // every singular value of N exceeds the cut-off exactly when
+      // N - cut-off I is positive definite
+      if ((rc = factor_at(-kPinvThreshold))) return rc;
+      schur = !ws_host[0].chol_fail;
+    }
+    if (!schur && xs > HSBA_COV_DENSE_MAX) return HSBA_ERR_TOO_LARGE;
+    CallBuffers buf;
+    double *dout = nullptr;
+    if ((rc = buf.get(dout, xs * xs))) return rc;
+    if (schur) {
+      if ((rc = factor_at(0.0))) return rc;
+      if (ws_host[0].chol_fail) return HSBA_ERR_NOT_SPD;
+      double *Y = nullptr, *Sinv = nullptr;
+      if (n > 0) {
+        if ((rc = buf.get(Y, ld * ld)) || (rc = buf.get(Sinv, ld * ld))) return rc;
+        chol_inverse_impl(plan, 0, !L.large_list.empty(), Y, Sinv, st);
+      }
+      if (P3 > 0 && n > 0) {
+        const int64_t P3p = (P3 + 63) / 64 * 64;
+        double *G = nullptr, *Tn = nullptr;
+        if ((rc = buf.get(G, P3p * ld)) || (rc = buf.get(Tn, P3p * ld))) return rc;
+        HCHK(hipMemsetAsync(G, 0, sizeof(double) * (size_t)(P3p * ld), st));
+        ks->point_rows(d, st, 1, G, ld, wi.r_off);
+        dense_gemm(false, false, P3p, ld, ld, -1.0, G, ld, Sinv, ld, Tn, ld, false, st);  // Tn = -G S^-1 = C_pr
+        dense_gemm(false, true, P3, P3, ld, -1.0, Tn, ld, G, ld, dout, xs, false, st);    // G S^-1 G^T
+        dense_copy(Tn, ld, dout + P3, xs, P3, n, 0, st);
+        dense_copy(Tn, ld, dout + P3 * xs, xs, n, P3, 1, st);
+      } else if (P3 > 0) {
+        HCHK(hipMemsetAsync(dout, 0, sizeof(double) * (size_t)(xs * xs), st));
+      }
+      if (P3 > 0) ks->point_diag(d, st, 1, dout, xs);
+      if (n > 0) dense_copy(Sinv, ld, dout + P3 * xs + P3, xs, n, n, 0, st);
+      inf.method = HSBA_COV_SCHUR;
+      inf.rank = (int32_t)xs;
+    } else {
+      const int64_t xp = (xs + 63) / 64 * 64;
+      double *A = nullptr, *Vm = nullptr, *work = nullptr;
+      int *rot = nullptr;
+      if ((rc = buf.get(A, xp * xp)) || (rc = buf.get(Vm, xp * xp)) || (rc = buf.get(work, 2 * xs + 2)) ||
+          (rc = buf.get(rot, kJacobiMaxSweeps)))
+        return rc;
+      HCHK(hipMemsetAsync(A, 0, sizeof(double) * (size_t)(xp * xp), st));
+      if (P3 > 0) ks->point_diag(d, st, 0, A, xp);
+      if (P3 > 0 && n > 0) {
+        ks->point_rows(d, st, 0, A + P3, xp, wi.r_off);
+        dense_copy(A + P3, xp, A + P3 * xp, xp, n, P3, 1, st);
+      }
+      if (n > 0) {
+        if (L.npairs > 0) HCHK(hipMemsetAsync(d.pair_out, 0, sizeof(T) * (size_t)(L.E * L.E * L.npairs), st));
+        if ((rc = set_state(nullptr, 0.0, 1, 0, 1))) return rc;
+        ks->assemble(d, st, L.lower_total, (int)L.rows_total);
+        dense_copy(d.S + wi.s_off, ld, A + P3 * xp + P3, xp, n, n, 2, st);
+      }
+      HCHK(hipGetLastError());
+      int sweeps = 0;
+      std::vector<double> lam;
+      if ((rc = dense_pinv(A, Vm, xs, xp, kPinvThreshold, dout, xs, kJacobiMaxSweeps, rot, work, lam, &sweeps, st)))
+        return rc;
+      inf.method = HSBA_COV_DENSE;
+      inf.sweeps = sweeps;
+      for (double v : lam)
+        if (std::fabs(v) > kPinvThreshold) {
+          ++inf.rank;
+          inf.smallest_kept = inf.rank == 1 ? std::fabs(v) : std::min(inf.smallest_kept, std::fabs(v));
+        }
+    }
+    HCHK(hipGetLastError());
+    HCHK(hipMemcpyAsync(out, dout, sizeof(double) * (size_t)(xs * xs), hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    if (info) *info = inf;
     return HSBA_OK;
   }
 

@@ -143,6 +143,9 @@ struct EngineBase {
   // fixed scratch region (no allocation; sizes bounded by kScratchMaxLd /
   // kScratchMaxPoints, HSBA_ERR_TOO_LARGE beyond them)
   virtual int covariance(double *Vp, double *Ui, double *Uc, double *reduced, double *scratch) = 0;
+  // dense pseudo-inverse of N at the last build (window 0), [x_size]^2 in the
+  // x layout (hsba_x_covariance)
+  virtual int x_covariance(double *out, int method, hsba_covariance_info *info) = 0;
   // coupling blocks W_pi / W_pc / W_ic of the last build (window 0)
   virtual int coupling(double *pi, double *pc, double *ic) = 0;
   virtual int evaluate(double *y) = 0;

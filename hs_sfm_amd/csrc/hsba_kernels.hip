@@ -1082,6 +1082,73 @@ __global__ void __launch_bounds__(256) k_point_cov(Dev<T> d, const double *Sinv,
       out[9 * (int64_t)p + 3 * r + c] = Li[0][r] * t[0][c] + Li[1][r] * t[1][c] + Li[2][r] * t[2][c];
 }
 
+// Dense point rows of the coupling block (full posterior covariance): one
+// thread per (point p, row a) writes out[(3 p + a) * ld + j] over the reduced
+// columns j of each of p's keys — which = 0: W_k row a; which = 1: row a of
+// L_p^-T Z_k = sum_{c >= a} (L_p^-1)[c][a] Z_k[c].  Image columns are written
+// (a point has one key per image), camera columns accumulated over the keys
+// whose images share the camera, in key order.
+template <typename T, int IMG, int D>
+__global__ void __launch_bounds__(256) k_point_rows(Dev<T> d, int which, double *out, int64_t ld, int r_off) {
+  constexpr int E = IMG + D;
+  constexpr int ZS = z_stride(E > 0 ? E : 1, sizeof(T));
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * (int64_t)d.P) return;
+  const int p = (int)(t / 3), a = (int)(t - 3 * (int64_t)p);
+  const T *Lp = d.Linv + 6 * p;
+  // column a of L_p^-1 (rows c >= a): lower-triangular storage l00 l10 l11 l20 l21 l22
+  const double lc[3] = {a == 0 ? (double)Lp[0] : 0.0, a == 0 ? (double)Lp[1] : (a == 1 ? (double)Lp[2] : 0.0),
+                        a == 0 ? (double)Lp[3] : (a == 1 ? (double)Lp[4] : (double)Lp[5])};
+  double *row = out + t * ld;
+  for (int s = d.pt_ptr[p]; s < d.pt_ptr[p + 1]; ++s) {
+    const int im = d.key_img[s];
+    double v[E > 0 ? E : 1];
+    if (which == 0) {
+#pragma unroll
+      for (int j = 0; j < E; ++j) v[j] = (double)d.W[((int64_t)3 * s + a) * E + j];
+    } else {
+      const T *z = d.Z + (int64_t)s * ZS;
+#pragma unroll
+      for (int j = 0; j < E; ++j)
+        v[j] = lc[0] * (double)z[j] + lc[1] * (double)z[E + j] + lc[2] * (double)z[2 * E + j];
+    }
+    if (IMG) {
+      const int r = d.img_row[im] - r_off;
+#pragma unroll
+      for (int j = 0; j < IMG; ++j) row[r + j] = v[j];
+    }
+    if (D) {
+      const int r = d.cam_row[d.img_cam[im]] - r_off;
+#pragma unroll
+      for (int j = 0; j < D; ++j) row[r + j] += v[IMG + j];
+    }
+  }
+}
+
+// Point diagonal blocks of the dense matrices: which = 0 stores V_p, which = 1
+// adds V_p^-1 = L_p^-T L_p^-1, at out[(3 p + a) * ld + 3 p + b].
+template <typename T>
+__global__ void __launch_bounds__(256) k_point_diag(Dev<T> d, int which, double *out, int64_t ld) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= d.P) return;
+  double m[3][3];
+  if (which == 0) {
+    const T *v = d.V + 6 * p;
+    const int sym[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) m[a][b] = (double)v[sym[a][b]];
+  } else {
+    const T *Lp = d.Linv + 6 * p;
+    const double Li[3][3] = {{(double)Lp[0], 0.0, 0.0}, {(double)Lp[1], (double)Lp[2], 0.0},
+                             {(double)Lp[3], (double)Lp[4], (double)Lp[5]}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) m[a][b] = Li[0][a] * Li[0][b] + Li[1][a] * Li[1][b] + Li[2][a] * Li[2][b];
+  }
+  double *o = out + (int64_t)3 * p * ld + 3 * p;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) o[a * ld + b] = which == 0 ? m[a][b] : o[a * ld + b] + m[a][b];
+}
+
 // ------------------------------------------------------------ export kernels
 // Per-key residuals and Jacobian blocks in the caller's key order (FP64),
 // used by hsba_evaluate / hsba_jacobian.
@@ -1537,6 +1604,15 @@ struct KernelSetImpl final : KernelSet<T> {
   void point_cov(const Dev<T> &d, hipStream_t st, const double *Sinv, int64_t ld, int r_off, double *out) override {
     if (d.P == 0) return;
     hipLaunchKernelGGL((k_point_cov<T, IMG, D>), dim3((d.P + 3) / 4), dim3(256), 0, st, d, Sinv, ld, r_off, out);
+  }
+  void point_rows(const Dev<T> &d, hipStream_t st, int which, double *out, int64_t ld, int r_off) override {
+    if (d.P == 0 || IMG + D == 0) return;
+    hipLaunchKernelGGL((k_point_rows<T, IMG, D>), dim3((3 * d.P + 255) / 256), dim3(256), 0, st, d, which, out, ld,
+                       r_off);
+  }
+  void point_diag(const Dev<T> &d, hipStream_t st, int which, double *out, int64_t ld) override {
+    if (d.P == 0) return;
+    hipLaunchKernelGGL((k_point_diag<T>), dim3((d.P + 255) / 256), dim3(256), 0, st, d, which, out, ld);
   }
   void export_keys(const Dev<T> &d, hipStream_t st, double *y, double *jp, double *ji, double *jc,
                    int want_j) override {

@@ -117,6 +117,20 @@ class Problem:
         out["Uc"] = out["Uc"][:, :d, :d]
         return rc, out
 
+    def x_covariance(self, x=None, y_obs=None, method=abi.COV_AUTO):
+        """hsba_x_covariance: the full dense covariance [x_size][x_size] at x
+        (the pseudo-inverse of N with singular values <= 1e-6 dropped);
+        returns (rc, C, info dict)."""
+        s = self.scene
+        x = s.x0 if x is None else x
+        y_obs = s.y_obs if y_obs is None else y_obs
+        out = np.zeros((self.x_size, self.x_size))
+        info = abi.hsba_covariance_info()
+        rc = self.lib.hsba_x_covariance(self.h, self._d(x), self._d(y_obs), C.byref(self.desc.ycov), method,
+                                        abi.dptr(out), C.byref(info))
+        return rc, out, dict(method=info.method, rank=info.rank, sweeps=info.sweeps,
+                             smallest_kept=info.smallest_kept)
+
     def reduced_system(self, mu):
         n = self.n_reduced
         S, rhs = np.zeros((n, n)), np.zeros(n)

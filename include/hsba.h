@@ -45,7 +45,8 @@ extern "C" {
 #define HSBA_ERR_ADMISSION     -7  /* job larger than the server's HBM budget */
 #define HSBA_ERR_UNSUPPORTED   -8  /* option combination not implemented   */
 #define HSBA_ERR_NOT_READY     -9  /* server job still running             */
-#define HSBA_ERR_TOO_LARGE    -10  /* beyond the fixed scratch of an allocation-free entry point */
+#define HSBA_ERR_TOO_LARGE    -10  /* beyond the fixed scratch of an allocation-free entry point, or
+                                       * beyond the size limit of the dense covariance route */
 
 /* ---- masks (bit i == reference enum value i) ---------------------------- */
 /* IntrinsicComputationsMask, camera_shared_common_types.hpp:14-22 */
@@ -322,6 +323,46 @@ int hsba_covariance(hsba_problem *problem, const double *x, const double *y_obs,
 int hsba_covariance_static(hsba_problem *problem, const double *x, const double *y_obs,
                            const hsba_ycov_inverse *ycov, double *point_blocks, double *image_blocks,
                            double *camera_blocks, double *reduced);
+
+/* Full dense posterior covariance at x: the output of
+ * CameraSharedAnalyticalXCovarianceCalculator::operator()
+ * (camera_shared_analytical_x_covariance_calculator.hpp:45-90), the
+ * pseudo-inverse of the dense normal matrix N = J^T Sigma^-1 J in which
+ * singular values at or below 1e-6 are dropped (:73-87), written row-major
+ * into x_covariance[x_size][x_size] in the x layout (points | images |
+ * cameras; every off-diagonal block included).  Routes:
+ *   HSBA_COV_SCHUR  N^-1 through the Schur complement on the device:
+ *                   S^-1 from the factored reduced system,
+ *                   C_pr = -V^-1 W S^-1 and C_pp = V^-1 + V^-1 W S^-1 W^T V^-1
+ *                   as matrix-core GEMMs.  Equal to the pseudo-inverse when
+ *                   every singular value of N exceeds 1e-6.
+ *   HSBA_COV_DENSE  the dense N eigen-decomposed on the device by the cyclic
+ *                   Jacobi method (for symmetric N the SVD, as the reference
+ *                   takes it), any N, e.g. a free gauge; O(x_size^3) per sweep,
+ *                   x_size at most HSBA_COV_DENSE_MAX (HSBA_ERR_TOO_LARGE
+ *                   beyond).
+ *   HSBA_COV_AUTO   factors N - 1e-6 I (point blocks and reduced system); when
+ *                   it is positive definite every singular value exceeds 1e-6
+ *                   and the Schur route gives the pseudo-inverse, else the
+ *                   dense route runs.
+ * info (may be NULL) reports the route taken, the number of singular values
+ * kept, the Jacobi sweeps and the smallest kept singular value (dense route).
+ * y_obs may be NULL.  Dense-solver problems only (HSBA_ERR_UNSUPPORTED under
+ * PCG). */
+#define HSBA_COV_AUTO      0
+#define HSBA_COV_SCHUR     1
+#define HSBA_COV_DENSE     2
+#define HSBA_COV_DENSE_MAX 4096
+typedef struct hsba_covariance_info {
+  int32_t method;        /* HSBA_COV_SCHUR or HSBA_COV_DENSE */
+  int32_t rank;          /* singular values kept (x_size on the Schur route) */
+  int32_t sweeps;        /* Jacobi sweeps of the dense route */
+  int32_t reserved_;
+  double smallest_kept;  /* smallest kept singular value (dense route), else 0 */
+} hsba_covariance_info;
+int hsba_x_covariance(hsba_problem *problem, const double *x, const double *y_obs,
+                      const hsba_ycov_inverse *ycov, int method, double *x_covariance,
+                      hsba_covariance_info *info);
 
 /* LM scalar helpers on the last build:
  *   max diag(N)            camera_shared_normal_matrix_max_diagonal_value_calculator.hpp:26-38

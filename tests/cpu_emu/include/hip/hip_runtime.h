@@ -72,6 +72,7 @@ using std::fabs;
 using std::fmax;
 using std::sqrt;
 inline int min(int a, int b) { return a < b ? a : b; }
+inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int max(int a, int b) { return a > b ? a : b; }
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }

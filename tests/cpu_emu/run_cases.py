@@ -149,8 +149,38 @@ def case_covariance_static():
     return "covariance_static bit-identical to covariance"
 
 
+def case_x_covariance():
+    """Full dense covariance vs the oracle's SVD pseudo-inverse on a ring
+    with a free gauge (dense Jacobi route; the singular values the device
+    drops are those numpy's SVD finds at or below 1e-6), then with the gauge
+    fixed by three constrained points (Schur route)."""
+    from oracle.covariance import dense_normal, x_covariance
+    s = sparse_ring(6, 16, seed=3, kmax=4)
+    o = Oracle(s)
+    sv = np.linalg.svd(dense_normal(o, s.x0), compute_uv=False)
+    free = Problem(s, precision=abi.PRECISION_FP64)
+    rc, Gf, info_f = free.x_covariance()
+    assert rc == 0 and info_f["method"] == abi.COV_DENSE, (rc, info_f)
+    assert info_f["rank"] == int((sv > 1e-6).sum()) < free.x_size, (info_f, sv.min())
+    e_free = rel(Gf, x_covariance(o, s.x0))
+    assert e_free < 1e-8, e_free
+    ids = [0, 1, 2]
+    s.point_constraints = np.array([[i, 7] for i in ids], np.int64)
+    s.y_obs = np.concatenate([s.y_obs[:2 * s.num_keys], s.x_true[:9]])
+    s.constraint_weights = np.full(9, 100.0)
+    p = Problem(s, precision=abi.PRECISION_FP64)
+    C = x_covariance(Oracle(s), s.x0)
+    rc, Ga, info_a = p.x_covariance()
+    assert rc == 0 and info_a["method"] == abi.COV_SCHUR, (rc, info_a)
+    e_s = rel(Ga, C)
+    assert e_s < 1e-8, e_s
+    return (f"x_size {p.x_size}: free gauge (dense, rank {info_f['rank']}, {info_f['sweeps']} sweeps) "
+            f"{e_free:.1e}; fixed gauge (Schur) {e_s:.1e}")
+
+
 CASES = {"tiled_cholesky": case_tiled_cholesky, "small_mma": case_small_mma, "lm_tiled": case_lm_tiled,
-         "batch": case_batch, "batch_xcd": case_batch_xcd, "covariance_static": case_covariance_static}
+         "batch": case_batch, "batch_xcd": case_batch_xcd, "covariance_static": case_covariance_static,
+         "x_covariance": case_x_covariance}
 
 if __name__ == "__main__":
     names = sys.argv[1:] or list(CASES)

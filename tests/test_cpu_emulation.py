@@ -17,8 +17,9 @@ EMU = os.path.join(HERE, "cpu_emu")
 CLANG = "/opt/rocm/llvm/bin/clang++"
 
 # the tiled factorization and both solves, the matrix-core pair kernel and
-# the one-workgroup factorization, batched windows, the allocation-free covariance
-CASES = ["tiled_cholesky", "small_mma", "batch", "covariance_static"]
+# the one-workgroup factorization, batched windows, the allocation-free
+# covariance, the full dense covariance (Jacobi and Schur routes, GEMM)
+CASES = ["tiled_cholesky", "small_mma", "batch", "covariance_static", "x_covariance"]
 
 
 @pytest.mark.skipif(not os.path.exists(CLANG), reason="ROCm clang++ not installed")
