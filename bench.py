@@ -17,6 +17,9 @@ The same line carries the other workloads as sub-objects under "workloads":
   large    2000 cameras x 9 along a trajectory, 600k points, ~5.1M keys,
            block-Jacobi PCG (rtol 1e-6), 30 iterations
   serving  Poisson arrivals, 95 % windows / 5 % 1M-key jobs, p50 / p99
+  covariance  the full dense x covariance (hsba_x_covariance, FP64) of a
+           30-camera ring with 500 points, gauge fixed (Schur route) and
+           free (dense Jacobi route), against the oracle's SVD
 Each is also available alone with --workload.
 """
 from __future__ import annotations
@@ -459,6 +462,68 @@ def run_serving(args):
             "mid_rmse_delta_vs_fp64_oracle_px": mid_delta}
 
 
+# ------------------------------------------------------------------ covariance
+def run_covariance(args, reps=3):
+    """hsba_x_covariance: the reference's whole dense x covariance (pseudo-
+    inverse of N with singular values <= 1e-6 dropped), FP64, on a ring of 30
+    cameras (6 parameters) and 500 points.  fixed: three points constrained in
+    all coordinates (gauge fixed, Schur route); free: no constraints (seven
+    null directions, dense Jacobi route).  The timed call includes the normal
+    equations at x, the device work and the copy of x_size^2 doubles to the
+    host.  Each result is checked against the oracle's SVD pseudo-inverse; the
+    CPU baseline is that oracle (dense N + LAPACK SVD, one BLAS thread)."""
+    from oracle.covariance import dense_normal, pseudo_inverse  # checker / CPU baseline
+    from oracle.oracle import Oracle
+
+    def scene(fixed):
+        s = syn.ring_scene(num_cameras=30, num_points=500)
+        if fixed:
+            ids = [0, 1, 2]
+            s.point_constraints = np.array([[i, 7] for i in ids], np.int64)
+            s.y_obs = np.concatenate([s.y_obs[:2 * s.num_keys], s.x_true[:9]])
+            s.constraint_weights = np.full(9, 100.0)
+        return s
+
+    out = {}
+    for name, fixed in (("fixed", True), ("free", False)):
+        s = scene(fixed)
+        p = Problem(s, precision=abi.PRECISION_FP64)
+        rc, G, info = p.x_covariance()  # warm-up (and the checked result)
+        abi.check(rc, "hsba_x_covariance")
+        ts = []
+        for _ in range(reps if fixed else 1):
+            t0 = time.perf_counter()
+            abi.check(p.x_covariance()[0], "hsba_x_covariance")
+            ts.append(time.perf_counter() - t0)
+        p.close()
+        o = Oracle(s)
+        N = dense_normal(o, s.x0)
+        try:
+            from threadpoolctl import threadpool_limits
+            limit = threadpool_limits(1)
+        except ImportError:
+            limit = None
+        c0 = time.perf_counter()
+        C = pseudo_inverse(N)
+        c1 = time.perf_counter()
+        if limit is not None:
+            limit.restore_original_limits()
+        err = float(np.abs(G - C).max() / np.abs(C).max())
+        t = float(np.median(ts))
+        out[name] = {"x_size": int(G.shape[0]), "route": {1: "schur", 2: "dense"}[info["method"]],
+                     "kept": info["rank"], "jacobi_sweeps": info["sweeps"], "ms": 1e3 * t,
+                     "rel_err_vs_fp64_oracle": err,
+                     "cpu_baseline": {"value": 1.0 / (c1 - c0), "unit": "covariances/s", "cores": 1,
+                                      "kind": "port",
+                                      "sample": f"the oracle's LAPACK SVD pseudo-inverse of the same dense N "
+                                                f"({G.shape[0]}^2), one BLAS thread, cpu={cpu_model()}"}}
+    return {"metric": "full dense x covariances/s (FP64)", "value": 1e3 / out["fixed"]["ms"],
+            "unit": "covariances/s", "dtype": "fp64",
+            "config": {"workload": "covariance: ring of 30 cameras x 6, 500 points, x_size "
+                                   f"{out['fixed']['x_size']}; gauge fixed (Schur) and free (dense Jacobi)"},
+            "fixed": out["fixed"], "free": out["free"]}
+
+
 def standalone(out, args):
     """A sub-workload result printed as its own line (--workload X)."""
     base = {"n_gpus": 1, "warmup": 0, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -473,8 +538,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="default",
-                    choices=["default", "mid", "large", "windows", "ring", "serving"])
-    ap.add_argument("--extras", default="ring,windows,large,serving",
+                    choices=["default", "mid", "large", "windows", "ring", "serving", "covariance"])
+    ap.add_argument("--extras", default="ring,windows,large,serving,covariance",
                     help="workloads carried in the default line (comma list, or 'none')")
     ap.add_argument("--rates", type=float, nargs="+", default=[100, 500, 2000, 5000])
     ap.add_argument("--seconds", type=float, default=1.0, help="arrival window per rate (serving)")
@@ -507,7 +572,7 @@ def main():
         del src, dst
     if args.workload != "default":
         fn = {"mid": run_mid, "large": run_large, "windows": run_windows, "ring": run_ring,
-              "serving": run_serving}[args.workload]
+              "serving": run_serving, "covariance": run_covariance}[args.workload]
         out = fn(args)
         print(json.dumps(out if args.workload == "mid" else standalone(out, args)))
         return
@@ -515,7 +580,8 @@ def main():
     extras = [] if args.extras == "none" else [e for e in args.extras.split(",") if e]
     out["workloads"] = {}
     for name in extras:
-        fn = {"ring": run_ring, "windows": run_windows, "large": run_large, "serving": run_serving}[name]
+        fn = {"ring": run_ring, "windows": run_windows, "large": run_large, "serving": run_serving,
+              "covariance": run_covariance}[name]
         t = time.perf_counter()
         print(f"bench: {name} ...", file=sys.stderr, flush=True)
         try:

@@ -330,44 +330,47 @@ __global__ void k_point_factor(Dev<T> d) {
   d.q[3 * p + 2] = T(i20 * g0 + i21 * g1 + i22 * g2);
 }
 
-// Z_k = L_p^-1 W_k.  A workgroup owns a chunk of 64 consecutive keys: their
-// point factors are staged in LDS and every thread produces contiguous Z
-// elements, so both W reads and Z writes are coalesced.
+// Z_k = L_p^-1 W_k, one thread per key.  The thread's W loads go out before
+// the dependent point / window / factor lookups, so they overlap that chain;
+// the Z row (padding included) leaves as whole 16-byte stores, so every
+// 128-byte line of Z is written completely.
 template <typename T, int E>
 __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
-  constexpr int KC = 64;
   constexpr int BLK = 3 * E;
-  __shared__ T lsh[KC][6];
-  __shared__ int act[KC];
-  const int k0 = blockIdx.x * KC;
-  if (threadIdx.x < KC) {
-    const int k = k0 + threadIdx.x;
-    int a = 0;
-    if (k < d.N) {
-      const int p = d.key_pt[k];
-      a = win_trial(d.ws[d.pt_win[p]]) ? 1 : 0;
-      if (a)
-#pragma unroll
-        for (int j = 0; j < 6; ++j) lsh[threadIdx.x][j] = d.Linv[6 * p + j];
-    }
-    act[threadIdx.x] = a;
-  }
-  __syncthreads();
   constexpr int ZS = z_stride(E, sizeof(T));
-  const int nk = min(KC, d.N - k0);
-  const T *w = d.W + (int64_t)k0 * BLK;
-  T *z = d.Z + (int64_t)k0 * ZS;
-  for (int e = threadIdx.x; e < nk * BLK; e += blockDim.x) {
-    const int kl = e / BLK, rem = e - kl * BLK;
-    if (!act[kl]) continue;
-    const int a = rem / E, j = rem - a * E;
-    const T *wk = w + kl * BLK;
-    T v;
-    if (a == 0) v = lsh[kl][0] * wk[j];
-    else if (a == 1) v = lsh[kl][1] * wk[j] + lsh[kl][2] * wk[E + j];
-    else v = lsh[kl][3] * wk[j] + lsh[kl][4] * wk[E + j] + lsh[kl][5] * wk[2 * E + j];
-    z[kl * ZS + rem] = v;
-    if (d.Zi) d.Zi[(int64_t)d.key_ipos[k0 + kl] * ZS + rem] = v;
+  constexpr int VW = 16 / sizeof(T);  // elements per 16-byte store
+  typedef T vec __attribute__((ext_vector_type(VW)));
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= d.N) return;
+  // the W row does not depend on the point: its loads are issued first and
+  // overlap the point / window lookups
+  T w[BLK];
+  const T *wk = d.W + (int64_t)k * BLK;
+#pragma unroll
+  for (int e = 0; e < BLK; ++e) w[e] = wk[e];
+  const int p = d.key_pt[k];
+  if (!win_trial(d.ws[d.pt_win[p]])) return;
+  T l[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) l[j] = d.Linv[6 * p + j];
+  // Z = L^-1 W (lower-triangular L^-1: l00 l10 l11 l20 l21 l22), padding zero,
+  // written as whole 16-byte vectors
+  vec out[ZS / VW];
+#pragma unroll
+  for (int e = 0; e < ZS; ++e) {
+    T v = 0;
+    if (e < E) v = l[0] * w[e];
+    else if (e < 2 * E) v = l[1] * w[e - E] + l[2] * w[e];
+    else if (e < BLK) v = l[3] * w[e - 2 * E] + l[4] * w[e - E] + l[5] * w[e];
+    out[e / VW][e % VW] = v;
+  }
+  vec *z = (vec *)(d.Z + (int64_t)k * ZS);
+#pragma unroll
+  for (int q = 0; q < ZS / VW; ++q) z[q] = out[q];
+  if (d.Zi) {
+    vec *zi = (vec *)(d.Zi + (int64_t)d.key_ipos[k] * ZS);
+#pragma unroll
+    for (int q = 0; q < ZS / VW; ++q) zi[q] = out[q];
   }
 }
 
@@ -855,37 +858,53 @@ __global__ void __launch_bounds__(256) k_grad_stats(Dev<T> d) {
 //   phase 3: after the candidate cost: gain ratio, accept / reject, mu update
 //   phase 4: after a relinearization: ||g||_inf <= eps1 -> stop
 template <typename T>
-__global__ void __launch_bounds__(64) k_lm_control(Dev<T> d, int phase) {
+__global__ void __launch_bounds__(256) k_lm_control(Dev<T> d, int phase) {
+  __shared__ double red[4][6];
   const int w = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   WinState &s = d.ws[w];
-  // chunk range of this window: chunks are sorted by window
-  int cb, ce;
-  {
-    int lo = 0, hi = d.nchunks;
-    while (lo < hi) { const int m = (lo + hi) >> 1; if (d.chunk_win[m] < w) lo = m + 1; else hi = m; }
-    cb = lo; hi = d.nchunks;
-    while (lo < hi) { const int m = (lo + hi) >> 1; if (d.chunk_win[m] <= w) lo = m + 1; else hi = m; }
-    ce = lo;
-  }
-  // lane partial sums / maxima over the window's chunks, fixed butterfly
   double F = 0.0, gm = 0.0, dm = -1e308, dn = 0.0, xn = 0.0, gd = 0.0;
-  for (int c = cb + lane; c < ce; c += 64) {
-    if (d.chunk_kind[c] == 3) F += d.partial[4 * c + 3];
-    else {
-      if (phase == 0 || phase == 4) { gm = fmax(gm, d.partial[4 * c]); dm = fmax(dm, d.partial[4 * c + 1]); }
-      if (phase == 2) { dn += d.partial[4 * c]; xn += d.partial[4 * c + 1]; gd += d.partial[4 * c + 2]; }
+  if (phase != 1) {  // phase 1 reads no sums
+    // chunk range of this window: chunks are sorted by window
+    int cb, ce;
+    {
+      int lo = 0, hi = d.nchunks;
+      while (lo < hi) { const int m = (lo + hi) >> 1; if (d.chunk_win[m] < w) lo = m + 1; else hi = m; }
+      cb = lo; hi = d.nchunks;
+      while (lo < hi) { const int m = (lo + hi) >> 1; if (d.chunk_win[m] <= w) lo = m + 1; else hi = m; }
+      ce = lo;
     }
+    // thread partial sums / maxima over the window's chunks; then a fixed
+    // butterfly in each wave and the four waves combined in order
+    for (int c = cb + t; c < ce; c += 256) {
+      if (d.chunk_kind[c] == 3) F += d.partial[4 * c + 3];
+      else {
+        if (phase == 0 || phase == 4) { gm = fmax(gm, d.partial[4 * c]); dm = fmax(dm, d.partial[4 * c + 1]); }
+        if (phase == 2) { dn += d.partial[4 * c]; xn += d.partial[4 * c + 1]; gd += d.partial[4 * c + 2]; }
+      }
+    }
+    for (int m = 1; m < 64; m <<= 1) {
+      F += __shfl_xor(F, m, 64);
+      gm = fmax(gm, __shfl_xor(gm, m, 64));
+      dm = fmax(dm, __shfl_xor(dm, m, 64));
+      dn += __shfl_xor(dn, m, 64);
+      xn += __shfl_xor(xn, m, 64);
+      gd += __shfl_xor(gd, m, 64);
+    }
+    if (lane == 0) {
+      red[wave][0] = F; red[wave][1] = gm; red[wave][2] = dm;
+      red[wave][3] = dn; red[wave][4] = xn; red[wave][5] = gd;
+    }
+    __syncthreads();
+    if (t != 0) return;
+    F = (red[0][0] + red[1][0]) + (red[2][0] + red[3][0]);
+    gm = fmax(fmax(red[0][1], red[1][1]), fmax(red[2][1], red[3][1]));
+    dm = fmax(fmax(red[0][2], red[1][2]), fmax(red[2][2], red[3][2]));
+    dn = (red[0][3] + red[1][3]) + (red[2][3] + red[3][3]);
+    xn = (red[0][4] + red[1][4]) + (red[2][4] + red[3][4]);
+    gd = (red[0][5] + red[1][5]) + (red[2][5] + red[3][5]);
   }
-  for (int m = 1; m < 64; m <<= 1) {
-    F += __shfl_xor(F, m, 64);
-    gm = fmax(gm, __shfl_xor(gm, m, 64));
-    dm = fmax(dm, __shfl_xor(dm, m, 64));
-    dn += __shfl_xor(dn, m, 64);
-    xn += __shfl_xor(xn, m, 64);
-    gd += __shfl_xor(gd, m, 64);
-  }
-  if (lane != 0) return;
+  if (t != 0) return;
   if (phase == 0) {
     if (!s.active) return;
     s.cost = F + s.cost_new;  // cost_new holds the constraint part
@@ -1525,7 +1544,7 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void key_z(const Dev<T> &d, hipStream_t st) override {
     if (!d.pts_free || E == 0 || d.N == 0) return;
-    hipLaunchKernelGGL((k_key_z<T, (E > 0 ? E : 1)>), dim3((d.N + 63) / 64), dim3(256), 0, st, d);
+    hipLaunchKernelGGL((k_key_z<T, (E > 0 ? E : 1)>), dim3((d.N + 255) / 256), dim3(256), 0, st, d);
   }
   bool has_pair_variant(int variant) const override { return variant == 0 ? E > 0 : (E >= 1 && E <= 15); }
   void pairs(const Dev<T> &d, hipStream_t st, int variant, const int32_t *plist, int count) override {
@@ -1561,7 +1580,7 @@ struct KernelSetImpl final : KernelSet<T> {
     hipLaunchKernelGGL((k_grad_stats<T, IMG, D>), dim3(d.nchunks), dim3(256), 0, st, d);
   }
   void lm_control(const Dev<T> &d, hipStream_t st, int phase) override {
-    hipLaunchKernelGGL((k_lm_control<T>), dim3(d.nwin), dim3(64), 0, st, d, phase);
+    hipLaunchKernelGGL((k_lm_control<T>), dim3(d.nwin), dim3(256), 0, st, d, phase);
   }
   void commit(const Dev<T> &d, hipStream_t st) override {
     const int64_t n = (d.pts_free ? 3LL * d.P : 0) + (d.imgs_free ? 6LL * d.I : 0) +
