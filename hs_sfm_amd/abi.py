@@ -28,7 +28,7 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 SOLVER_AUTO, SOLVER_DENSE, SOLVER_PCG = 0, 1, 2
 LOSS_TRIVIAL, LOSS_HUBER = 0, 1
-PAIRS_TUNED, PAIRS_VALU, PAIRS_MFMA = 0, 1, 2
+PAIRS_TUNED, PAIRS_VALU, PAIRS_MFMA, PAIRS_MFMA_WAVE = 0, 1, 2, 3
 COV_AUTO, COV_SCHUR, COV_DENSE = 0, 1, 2
 COV_DENSE_MAX = 4096
 
@@ -74,7 +74,7 @@ class hsba_options(C.Structure):
 
 class hsba_tune_entry(C.Structure):
     _fields_ = [("precision", C.c_int32), ("camera_block", C.c_int32), ("bucket", C.c_int32),
-                ("variant", C.c_int32), ("ms", C.c_double * 2), ("pairs", C.c_int64)]
+                ("variant", C.c_int32), ("ms", C.c_double * 3), ("pairs", C.c_int64)]
 
 
 class hsba_covariance_info(C.Structure):
@@ -329,8 +329,8 @@ def tuning_table():
     check(lib.hsba_tuning_table(None, 0, C.byref(n)), "hsba_tuning_table")
     arr = (hsba_tune_entry * max(1, n.value))()
     check(lib.hsba_tuning_table(arr, n.value, C.byref(n)), "hsba_tuning_table")
-    names = ["valu_wave_per_pair", "mfma_workgroup_per_pair"]
+    names = ["valu_wave_per_pair", "mfma_workgroup_per_pair", "mfma_wave_per_pair"]
     return [{"precision": "fp64" if e.precision == PRECISION_FP64 else "mixed", "camera_block": e.camera_block,
              "incidences_per_pair_bucket": f"[{2 ** e.bucket}, {2 ** (e.bucket + 1)})",
-             "chosen": names[e.variant], "ms": {names[0]: e.ms[0], names[1]: e.ms[1]}, "pairs_timed": e.pairs}
+             "chosen": names[e.variant], "ms": {names[v]: e.ms[v] for v in range(3)}, "pairs_timed": e.pairs}
             for e in arr[:n.value]]

@@ -103,7 +103,7 @@ int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_opti
     while ((1 << g) < o.point_group_lanes && g < 6) ++g;
     eng->glog2 = g;
   }
-  if (o.pair_kernel < HSBA_PAIRS_TUNED || o.pair_kernel > HSBA_PAIRS_MFMA) return HSBA_ERR_INVALID;
+  if (o.pair_kernel < HSBA_PAIRS_TUNED || o.pair_kernel > HSBA_PAIRS_MFMA_WAVE) return HSBA_ERR_INVALID;
   eng->pair_kernel = o.pair_kernel;
   eng->solver = pcg ? HSBA_SOLVER_PCG : HSBA_SOLVER_DENSE;
   eng->arena = arena;

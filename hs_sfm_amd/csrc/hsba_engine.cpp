@@ -265,12 +265,12 @@ struct Engine final : EngineBase {
   // ---- K4 kernel choice (autotuner) -------------------------------------
   // Pairs run by each pair kernel (index 0 VALU, 1 matrix cores); a list that
   // covers every pair is passed as null (pairs 0 .. npairs).
-  int32_t *plist_dev[2] = {nullptr, nullptr};
-  int plist_n[2] = {0, 0};
-  bool plist_all[2] = {false, false};
+  int32_t *plist_dev[kPairVariants] = {nullptr, nullptr, nullptr};
+  int plist_n[kPairVariants] = {0, 0, 0};
+  bool plist_all[kPairVariants] = {false, false, false};
 
   void enqueue_pairs() {
-    for (int v = 0; v < 2; ++v)
+    for (int v = 0; v < kPairVariants; ++v)
       if (plist_n[v]) ks->pairs(d, st, v, plist_all[v] ? nullptr : plist_dev[v], plist_n[v]);
   }
 
@@ -297,8 +297,8 @@ struct Engine final : EngineBase {
   // E, floor(log2(mean incidences per pair))); the first engine that meets a
   // class with two candidate kernels times both on that class's pairs (zeroed
   // Z rows, every window in a trial) and the process-wide table keeps the
-  // faster one.  The matrix-core kernel is a candidate for E <= 15 in mixed
-  // precision (the FP64 path keeps the register-tile kernel for E <= 6).
+  // fastest one.  The two matrix-core kernels are candidates for E <= 15 in
+  // mixed precision (the FP64 path keeps the register-tile kernel for E <= 6).
   int setup_pairs() {
     if (!L.pts_free || L.E == 0 || L.npairs == 0) return HSBA_OK;
     std::vector<int64_t> inc_w(L.nwin, 0), np_w(L.nwin, 0);
@@ -325,6 +325,8 @@ struct Engine final : EngineBase {
           var = 0;
         } else if (pair_kernel == HSBA_PAIRS_MFMA) {
           var = 1;
+        } else if (pair_kernel == HSBA_PAIRS_MFMA_WAVE) {
+          var = 2;
         } else if (!tune_lookup_locked(precision, L.E, kv.first, &var)) {
           std::vector<char> in(L.nwin, 0);
           for (int w : kv.second) in[w] = 1;
@@ -338,18 +340,20 @@ struct Engine final : EngineBase {
           if ((rc = set_state(nullptr, 0.0, 1, 0, 1))) return rc;
           HCHK(hipMemsetAsync(d.Z, 0, sizeof(T) * (size_t)z_stride(L.E, sizeof(T)) * (size_t)std::max<int64_t>(1, L.N), st));
           HCHK(hipMemsetAsync(d.q, 0, sizeof(T) * 3 * (size_t)std::max<int64_t>(1, L.P), st));
-          double ms[2] = {0.0, 0.0};
-          for (int v = 0; v < 2; ++v)
+          double ms[kPairVariants] = {0.0, 0.0, 0.0};
+          for (int v = 0; v < kPairVariants; ++v)
             if ((rc = time_pairs(v, dl, (int)cl.size(), 3, &ms[v]))) return rc;
-          var = ms[1] < ms[0] ? 1 : 0;
+          var = 0;
+          for (int v = 1; v < kPairVariants; ++v)
+            if (ms[v] < ms[var]) var = v;
           tune_store_locked(precision, L.E, kv.first, var, ms, (int64_t)cl.size());
         }
         for (int w : kv.second) wvar[w] = var;
       }
     }
-    std::vector<int32_t> lists[2];
+    std::vector<int32_t> lists[kPairVariants];
     for (int64_t q = 0; q < L.npairs; ++q) lists[wvar[L.pair_win[q]]].push_back((int32_t)q);
-    for (int v = 0; v < 2; ++v) {
+    for (int v = 0; v < kPairVariants; ++v) {
       plist_n[v] = (int)lists[v].size();
       plist_all[v] = plist_n[v] == L.npairs;
       if (plist_n[v] && !plist_all[v]) {

@@ -88,8 +88,10 @@ int guarded(F &&f) noexcept {
 // lookup, the timing of the candidates and the store, so one class is
 // measured once even when several engines are created concurrently.
 std::mutex &tune_mutex();
+constexpr int kPairVariants = 3;  // pair kernels the autotuner chooses between
 bool tune_lookup_locked(int precision, int E, int bucket, int *variant);
-void tune_store_locked(int precision, int E, int bucket, int variant, const double ms[2], int64_t pairs);
+void tune_store_locked(int precision, int E, int bucket, int variant, const double ms[kPairVariants],
+                       int64_t pairs);
 
 // First-fit sub-allocator over one device allocation, with coalescing of
 // freed neighbours (256-byte granules).  Thread-safe.  The serving loop
@@ -123,7 +125,7 @@ struct EngineBase {
   int loss = HSBA_LOSS_TRIVIAL;
   double loss_scale = 1.0;
   int glog2 = 3;
-  int pair_kernel = 0;  // 0 tuned, 1 wave-per-pair VALU, 2 matrix cores
+  int pair_kernel = 0;  // HSBA_PAIRS_*: 0 tuned, else the forced pair kernel
   virtual int init(hipStream_t stream) = 0;
   virtual int upload_x(const double *const *xs) = 0;
   virtual int upload_obs(const double *const *ys, const hsba_ycov_inverse *ycovs) = 0;

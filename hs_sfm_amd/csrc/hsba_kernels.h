@@ -18,9 +18,11 @@ struct KernelSet {
   virtual void constraints(const Dev<T> &d, hipStream_t st) = 0;
   virtual void point_factor(const Dev<T> &d, hipStream_t st) = 0;
   virtual void key_z(const Dev<T> &d, hipStream_t st) = 0;
-  // K4 over the pairs plist[0 .. count) (plist == nullptr: pairs 0 .. count).
-  // variant 0: one wavefront per pair, VALU register tiles; variant 1: one
-  // workgroup per pair, LDS-staged incidences on the matrix cores (E <= 15).
+  // K4 over the pairs plist[0 .. count) (plist == nullptr: pairs 0 .. count;
+  // entries < 0 are empty slots).  variant 0: one wavefront per pair, VALU
+  // register tiles; variant 1: one workgroup per pair, LDS-staged incidences
+  // on the matrix cores; variant 2: one wavefront per pair on the matrix
+  // cores (variants 1 and 2 for E <= 15).
   virtual void pairs(const Dev<T> &d, hipStream_t st, int variant, const int32_t *plist, int count) = 0;
   virtual bool has_pair_variant(int variant) const = 0;
   virtual void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) = 0;

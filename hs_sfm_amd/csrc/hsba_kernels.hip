@@ -474,6 +474,15 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d, const int32_t *plist, i
 // device).  The four wavefronts' partial products are summed in a fixed order.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+// Lanes of one wave see each other's earlier LDS writes after this point (the
+// LDS unit executes one wave's instructions in issue order; the fences keep
+// the compiler from moving LDS accesses across the call).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T>
 struct Mma;
 template <>
@@ -579,6 +588,99 @@ __global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d, const int32_t *plis
     if (row < E) {
       if (col < E) d.pair_out[(int64_t)pair * E * E + row * E + col] = v;
       else if (diag && col == E) d.rp[(int64_t)d.pair_img[pair] * E + row] = v;
+    }
+  }
+}
+
+// One wavefront per image pair on the matrix cores, four independent pairs
+// per workgroup and no workgroup barrier: for short incidence lists, where
+// k_pairs_mma leaves most of its loaders and waves idle and still pays the
+// cross-wave combine.  A stage is 16 incidences (4 lanes load each one's two
+// Z rows); while it is multiplied (48 rows of K = 3 per incidence, 12
+// v_mfma steps) the next stage's rows are in flight and the incidence codes
+// of the one after.  One accumulator per pair: the incidences are summed in
+// list order.
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_pairs_mma_wave(Dev<T> d, const int32_t *plist, int count) {
+  static_assert(E >= 1 && E <= 15, "one 16-wide tile holds the E columns and the rhs column");
+  constexpr int ZS = z_stride(E, sizeof(T));
+  constexpr int VW = 16 / sizeof(T), NV = ZS / VW;
+  constexpr int SI = 16;  // incidences per stage
+  typedef T vec __attribute__((ext_vector_type(VW)));
+  typedef typename Mma<T>::acc_t acc_t;
+  __shared__ vec sz[4][2][SI][NV];
+  __shared__ T sq[4][SI][4];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int slot = blockIdx.x * 4 + wave;
+  if (slot >= count) return;  // the whole wave
+  const int pair = plist ? plist[slot] : slot;
+  if (!win_trial(d.ws[d.pair_win[pair]])) return;
+  const bool diag = d.pair_img[pair] >= 0;
+  const int col = lane & 15, kr = lane >> 4;
+  const int ls = lane >> 2, lq = lane & 3;  // loader: incidence, quarter of its 2 NV vectors
+  const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
+  vec (*wz)[SI][NV] = sz[wave];
+  T (*wq)[4] = sq[wave];
+  acc_t acc = {0, 0, 0, 0};
+  vec v[2 * NV / 4];
+  T qv[3] = {0, 0, 0};
+  auto fetch = [&](uint64_t code) {
+    const int a = inc_a(code), b = inc_b(code);
+    const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
+    const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
+#pragma unroll
+    for (int u = 0; u < 2 * NV / 4; ++u) {
+      const int q = lq + 4 * u;
+      v[u] = q < NV ? ra[q] : rb[q - NV];
+    }
+    if (diag && lq == 0) {
+      const int p = d.key_pt[a];
+      qv[0] = d.q[3 * p]; qv[1] = d.q[3 * p + 1]; qv[2] = d.q[3 * p + 2];
+    }
+  };
+  uint64_t code_next = 0;
+  {
+    const int n0 = je - jb < SI ? (int)(je - jb) : SI;
+    if (ls < n0) fetch(d.inc[jb + ls]);
+    if (jb + SI + ls < je) code_next = d.inc[jb + SI + ls];
+  }
+  for (int64_t base = jb; base < je; base += SI) {
+    const int n = je - base < SI ? (int)(je - base) : SI;
+    if (ls < n) {
+#pragma unroll
+      for (int u = 0; u < 2 * NV / 4; ++u) {
+        const int q = lq + 4 * u;
+        wz[q < NV ? 0 : 1][ls][q < NV ? q : q - NV] = v[u];
+      }
+      if (diag && lq == 0) { wq[ls][0] = qv[0]; wq[ls][1] = qv[1]; wq[ls][2] = qv[2]; }
+    }
+    wave_lds_sync();
+    if (base + SI + ls < je) fetch(code_next);
+    code_next = (base + 2 * SI + ls < je) ? d.inc[base + 2 * SI + ls] : 0;
+    const int rows = 3 * n;
+    for (int g0 = 0; g0 < rows; g0 += 4) {
+      const int g = g0 + kr;
+      T av = 0, bv = 0;
+      if (g < rows) {
+        const int s = g / 3, r = g - 3 * s;
+        const T *za = (const T *)wz[0][s], *zb = (const T *)wz[1][s];
+        if (col < E) {
+          av = za[r * E + col];
+          bv = zb[r * E + col];
+        } else if (diag && col == E) {
+          bv = wq[s][r];
+        }
+      }
+      acc = Mma<T>::op(av, bv, acc);
+    }
+    wave_lds_sync();  // the stage's rows are read before the next stage overwrites them
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int row = Mma<T>::row(kr, e);
+    if (row < E) {
+      if (col < E) d.pair_out[(int64_t)pair * E * E + row * E + col] = acc[e];
+      else if (diag && col == E) d.rp[(int64_t)d.pair_img[pair] * E + row] = acc[e];
     }
   }
 }
@@ -1552,6 +1654,10 @@ struct KernelSetImpl final : KernelSet<T> {
     if constexpr (E >= 1 && E <= 15) {
       if (variant == 1) {
         hipLaunchKernelGGL((k_pairs_mma<T, E>), dim3(count), dim3(256), 0, st, d, plist);
+        return;
+      }
+      if (variant == 2) {
+        hipLaunchKernelGGL((k_pairs_mma_wave<T, E>), dim3((count + 3) / 4), dim3(256), 0, st, d, plist, count);
         return;
       }
     }

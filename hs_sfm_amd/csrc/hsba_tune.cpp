@@ -24,7 +24,7 @@ namespace {
 
 struct Entry {
   int variant;
-  double ms[2];
+  double ms[kPairVariants];
   int64_t pairs;
 };
 
@@ -42,10 +42,10 @@ void load_locked() {
   FILE *f = std::fopen(path, "r");
   if (!f) return;
   int prec, e, b, v;
-  double m0, m1;
+  double m0, m1, m2;
   long long np;
-  while (std::fscanf(f, "%d %d %d %d %lf %lf %lld", &prec, &e, &b, &v, &m0, &m1, &np) == 7)
-    table[Key(prec, e, b)] = Entry{v, {m0, m1}, (int64_t)np};
+  while (std::fscanf(f, "%d %d %d %d %lf %lf %lf %lld", &prec, &e, &b, &v, &m0, &m1, &m2, &np) == 8)
+    if (v >= 0 && v < kPairVariants) table[Key(prec, e, b)] = Entry{v, {m0, m1, m2}, (int64_t)np};
   std::fclose(f);
 }
 
@@ -55,8 +55,8 @@ void save_locked() {
   FILE *f = std::fopen(path, "w");
   if (!f) return;
   for (const auto &kv : table)
-    std::fprintf(f, "%d %d %d %d %.6g %.6g %lld\n", std::get<0>(kv.first), std::get<1>(kv.first),
-                 std::get<2>(kv.first), kv.second.variant, kv.second.ms[0], kv.second.ms[1],
+    std::fprintf(f, "%d %d %d %d %.6g %.6g %.6g %lld\n", std::get<0>(kv.first), std::get<1>(kv.first),
+                 std::get<2>(kv.first), kv.second.variant, kv.second.ms[0], kv.second.ms[1], kv.second.ms[2],
                  (long long)kv.second.pairs);
   std::fclose(f);
 }
@@ -73,8 +73,9 @@ bool tune_lookup_locked(int precision, int E, int bucket, int *variant) {
   return true;
 }
 
-void tune_store_locked(int precision, int E, int bucket, int variant, const double ms[2], int64_t pairs) {
-  table[Key(precision, E, bucket)] = Entry{variant, {ms[0], ms[1]}, pairs};
+void tune_store_locked(int precision, int E, int bucket, int variant, const double ms[kPairVariants],
+                       int64_t pairs) {
+  table[Key(precision, E, bucket)] = Entry{variant, {ms[0], ms[1], ms[2]}, pairs};
   save_locked();
 }
 
@@ -95,8 +96,7 @@ int hsba_tuning_table(hsba_tune_entry *entries, int32_t max_entries, int32_t *co
         o.camera_block = std::get<1>(kv.first);
         o.bucket = std::get<2>(kv.first);
         o.variant = kv.second.variant;
-        o.ms[0] = kv.second.ms[0];
-        o.ms[1] = kv.second.ms[1];
+        for (int v = 0; v < hsba::kPairVariants; ++v) o.ms[v] = kv.second.ms[v];
         o.pairs = kv.second.pairs;
       }
       ++n;

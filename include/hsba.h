@@ -162,6 +162,7 @@ typedef struct {
 #define HSBA_PAIRS_TUNED  0
 #define HSBA_PAIRS_VALU   1         /* one wavefront per pair, register tiles */
 #define HSBA_PAIRS_MFMA   2         /* one workgroup per pair, matrix cores (camera block <= 15) */
+#define HSBA_PAIRS_MFMA_WAVE 3      /* one wavefront per pair, matrix cores (camera block <= 15) */
 
 /* Optimizer settings: CameraSharedLevenbergMarquardtOptimizor ctor
  * (camera_shared_levenberg_marquardt_optimizor.hpp:36-41). */
@@ -414,8 +415,9 @@ typedef struct {
   int32_t precision;                /* HSBA_PRECISION_* */
   int32_t camera_block;             /* E: image + camera parameters of a reduced block row */
   int32_t bucket;                   /* floor(log2(mean incidences per camera pair)) of the windows */
-  int32_t variant;                  /* chosen pair kernel: 0 VALU, 1 matrix cores */
-  double  ms[2];                    /* measured time of each candidate (0 = not a candidate) */
+  int32_t variant;                  /* chosen pair kernel: 0 wavefront per pair (VALU), 1 workgroup per
+                                       pair (matrix cores), 2 wavefront per pair (matrix cores) */
+  double  ms[3];                    /* measured time of each candidate (0 = not a candidate) */
   int64_t pairs;                    /* camera pairs the candidates were timed on */
 } hsba_tune_entry;
 int hsba_tuning_table(hsba_tune_entry *entries, int32_t max_entries, int32_t *count);

@@ -79,18 +79,22 @@ def case_tiled_cholesky():
 
 
 def case_small_mma():
-    """E = 9 (matrix-core pair kernel), one-workgroup factorization, mixed."""
+    """E = 9: each of the three pair kernels (forced) against the oracle, then
+    the one-workgroup factorization, mixed."""
     s = syn.tiny_scene(comp_mask=syn.COMPUTE_RADIAL | syn.COMPUTE_INTRINSIC,
                        free_mask=(1 << syn.K1) | (1 << syn.K2) | (1 << syn.FOCAL), constraints=True,
                        random_cov=True, num_images=6, num_points=20, num_cameras=2)
-    o, p = Oracle(s), Problem(s, precision=abi.PRECISION_MIXED)
+    o = Oracle(s)
     o.build_normal(s.x0)
-    p.build_normal(s.x0)
     mu = 1e-3 * o.max_diagonal()
-    rc, S, rhs = p.reduced_system(mu)
     rc_o, So, ro = o.reduced_system(mu)
-    assert rc == 0 and rc_o == 0
-    assert rel(S, So) < 2e-5 and rel(rhs, ro) < 2e-5
+    assert rc_o == 0
+    for kernel in (abi.PAIRS_VALU, abi.PAIRS_MFMA_WAVE, abi.PAIRS_MFMA):
+        p = Problem(s, precision=abi.PRECISION_MIXED, pair_kernel=kernel)
+        p.build_normal(s.x0)
+        rc, S, rhs = p.reduced_system(mu)
+        assert rc == 0
+        assert rel(S, So) < 2e-5 and rel(rhs, ro) < 2e-5, (kernel, rel(S, So))
     rc, dg = p.solve(mu)
     assert rc == 0
     back = np.abs(S @ dg[-p.n_reduced:] - rhs).max() / (np.abs(S).max() * np.abs(dg[-p.n_reduced:]).max())
