@@ -104,18 +104,47 @@ def roofline_entry(kind, achieved, peak, unit, algorithmic, traffic=None, kernel
     return out
 
 
-def oracle_rmse(scene, iters):
-    """FP64 oracle's RMSE after `iters` LM iterations (checker only)."""
+def oracle_x(scene, iters):
+    """FP64 oracle's x after `iters` LM iterations (checker only)."""
     from oracle.oracle import Oracle
     _, xo, _, _ = Oracle(scene).lm(max_iterations=iters)
-    return syn.reprojection_rmse(scene, xo)
+    return xo
+
+
+def oracle_x_all(scenes, iters):
+    """Oracle x of every scene, on host threads (the oracle is a ctypes call,
+    which releases the GIL)."""
+    with cf.ThreadPoolExecutor(max_workers=CHECK_THREADS) as ex:
+        return list(ex.map(lambda s: oracle_x(s, iters), scenes))
 
 
 def oracle_rmse_all(scenes, iters):
-    """Oracle RMSE of every scene, on host threads (the oracle is a ctypes
-    call, which releases the GIL)."""
-    with cf.ThreadPoolExecutor(max_workers=CHECK_THREADS) as ex:
-        return list(ex.map(lambda s: oracle_rmse(s, iters), scenes))
+    return [syn.reprojection_rmse(s, xo) for s, xo in zip(scenes, oracle_x_all(scenes, iters))]
+
+
+def param_deviation(scene, x, xo):
+    """Deviation of the parameters x from the oracle's xo (reference x layout:
+    points | images | cameras, fixed blocks absent): max |dx| per block kind
+    and |dx| / |xo| over all of x.  Both runs start from the same x0 and their
+    LM trajectories agree, so they end in the same gauge and no similarity
+    alignment is applied."""
+    x, xo = np.asarray(x, np.float64), np.asarray(xo, np.float64)
+    d = np.abs(x - xo)
+    P, I = scene.num_points, scene.num_images
+    o = 0
+    out = {}
+    if not scene.fix_mask & syn.FIX_POINTS:
+        out["points_max_abs_m"] = float(d[o:o + 3 * P].max()) if P else 0.0
+        o += 3 * P
+    if not scene.fix_mask & syn.FIX_IMAGES:
+        blk = d[o:o + 6 * I].reshape(-1, 6)
+        out["rotation_max_abs_rad"] = float(blk[:, :3].max()) if I else 0.0
+        out["translation_max_abs_m"] = float(blk[:, 3:].max()) if I else 0.0
+        o += 6 * I
+    if o < len(d):
+        out["camera_params_max_abs"] = float(d[o:].max())
+    out["relative_l2"] = float(np.linalg.norm(x - xo) / max(np.linalg.norm(xo), 1e-300))
+    return out
 
 
 # ------------------------------------------------------------------ configs[1]
@@ -213,6 +242,9 @@ def run_mid(args):
         out["lm"]["fp64_oracle_rmse_px"] = ref["rmse_px"]
         out["lm"]["rmse_delta_px"] = abs(rmse - ref["rmse_px"])
         out["lm"]["final_cost_rel_err"] = abs(st.final_cost - ref["final_cost"]) / ref["final_cost"]
+        xo_path = os.path.join(ROOT, "tests", "golden", "mid_fp64_oracle_x.npy")
+        if os.path.exists(xo_path):
+            out["lm"]["param_deviation_vs_fp64_oracle"] = param_deviation(scene, x, np.load(xo_path))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_mid(scene, samples=args.cpu_samples)
     p.close()
@@ -244,6 +276,7 @@ def run_ring(args, K=20):
             "config": {"workload": f"ring: 10 cameras, 500 points, {scene.num_keys} keys, FP64"},
             "rmse_px": rm, "fp64_oracle_rmse_px": rmo, "rmse_delta_px": abs(rm - rmo),
             "final_cost_rel_err": abs(st.final_cost - sto.final_cost) / sto.final_cost,
+            "param_deviation_vs_fp64_oracle": param_deviation(scene, x, xo),
             "cpu_baseline": {"value": K / (c1 - c0), "unit": "iters/s", "cores": 1, "kind": "port",
                              "sample": f"{K} LM iterations of the same scene with the reference's literal "
                                        f"O(I^2 P) Schur loops, cpu={cpu_model()}"}}
@@ -297,8 +330,10 @@ def run_windows(args):
         p.close()
     one_at_a_time = n / single_s
     # accuracy: every window against the FP64 oracle
-    ro = oracle_rmse_all(scenes, K)
-    deltas = [abs(syn.reprojection_rmse(s, x) - r) for s, x, r in zip(scenes, xs, ro)]
+    xos = oracle_x_all(scenes, K)
+    deltas = [abs(syn.reprojection_rmse(s, x) - syn.reprojection_rmse(s, xo)) for s, x, xo in zip(scenes, xs, xos)]
+    devs = [param_deviation(s, x, xo) for s, x, xo in zip(scenes, xs, xos)]
+    worst_dev = {k: max(dv[k] for dv in devs if k in dv) for k in devs[0]}
     # CPU baseline: the oracle solving a sample of windows on one core
     from oracle.oracle import Oracle  # CPU baseline leg only
     sample = scenes[:args.cpu_windows]
@@ -328,6 +363,7 @@ def run_windows(args):
                                    f"{keys} keys, {K} LM iterations each"},
             "one_at_a_time_solves_per_s": one_at_a_time, "speedup_vs_one_at_a_time": batched / one_at_a_time,
             "rmse_worst_delta_vs_fp64_oracle_px": max(deltas), "oracle_checked_windows": len(deltas),
+            "param_deviation_worst_vs_fp64_oracle": worst_dev,
             "statuses": sorted(set(int(s.status) for s in sts)),
             "stage_ms": stages, "dominant_stage": dominant, "autotuner": abi.tuning_table(),
             "roofline": dict(kernels[dom_kernel], kernel=names[dom_kernel]), "kernels": kernels,

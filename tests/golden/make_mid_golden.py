@@ -1,13 +1,16 @@
 """Writes tests/golden/mid_fp64_oracle.json: the FP64 oracle's 20-iteration LM
-run on the 200-camera / 50k-point / ~1M-key scene (about 200 s on one core).
+run on the 200-camera / 50k-point / ~1M-key scene (about 200 s on one core),
+and tests/golden/mid_fp64_oracle_x.npy: its final x (reference x layout).
 
-The bench compares the GPU run's final cost and RMSE with these figures.
+The bench compares the GPU run's final cost, RMSE and parameters with these.
 Run: python tests/golden/make_mid_golden.py
 """
 import json
 import os
 import sys
 import time
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
@@ -28,6 +31,7 @@ def main():
            "mean_error_px": syn.mean_reprojection_error(s, x), "oracle_seconds": dt}
     with open(os.path.join(HERE, "mid_fp64_oracle.json"), "w") as f:
         json.dump(out, f, indent=1)
+    np.save(os.path.join(HERE, "mid_fp64_oracle_x.npy"), np.asarray(x, np.float64))
     print(out)
 
 
