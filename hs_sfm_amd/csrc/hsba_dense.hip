@@ -301,7 +301,6 @@ int dense_pinv(double *A, double *Vt, int64_t n, int64_t ld, double thr, double 
   const int64_t cells = ld * ld;
   hipLaunchKernelGGL(k_identity, dim3((unsigned)std::min<int64_t>((cells + 255) / 256, 65536)), dim3(256), 0, st, Vt,
                      ld, n);
-  hipLaunchKernelGGL(k_fill_int, dim3((max_sweeps + 255) / 256), dim3(256), 0, st, rotations_dev, max_sweeps, 0);
   hipLaunchKernelGGL(k_row_sq, dim3(nn), dim3(256), 0, st, A, ld, nn, aux);
   std::vector<double> sq((size_t)n);
   if (hipMemcpyAsync(sq.data(), aux, sizeof(double) * sq.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
@@ -311,24 +310,39 @@ int dense_pinv(double *A, double *Vt, int64_t n, int64_t ld, double thr, double 
   for (double v : sq) fro2 += v;
   const double eps = 2.220446049250313e-16;
   const double floor_ = eps * std::sqrt(fro2);
-  int done = 0;
-  for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    for (int r = 0; r < m - 1; ++r) {
-      hipLaunchKernelGGL(k_jac_angles, dim3((m / 2 + 255) / 256), dim3(256), 0, st, A, ld, nn, m, r, eps, floor_,
-                         rot, rotations_dev + sweep);
-      hipLaunchKernelGGL(k_jac_apply, dim3(m / 2), dim3(256), 0, st, A, Vt, ld, nn, m, r, rot);
-    }
+  // one sweep (the rotation counter cleared by a kernel, then every round's
+  // two launches) captured as a graph and replayed until a sweep rotates
+  // nothing
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) return HSBA_ERR_DEVICE;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, rotations_dev, 1, 0);
+  for (int r = 0; r < m - 1; ++r) {
+    hipLaunchKernelGGL(k_jac_angles, dim3((m / 2 + 255) / 256), dim3(256), 0, st, A, ld, nn, m, r, eps, floor_, rot,
+                       rotations_dev);
+    hipLaunchKernelGGL(k_jac_apply, dim3(m / 2), dim3(256), 0, st, A, Vt, ld, nn, m, r, rot);
+  }
+  const hipError_t ec = hipStreamEndCapture(st, &graph);
+  if (ec != hipSuccess || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return HSBA_ERR_DEVICE;
+  }
+  int done = 0, rc = HSBA_OK;
+  for (int sweep = 0; sweep < max_sweeps && rc == HSBA_OK; ++sweep) {
     int count = 0;
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(&count, rotations_dev + sweep, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-      return HSBA_ERR_DEVICE;
-    *sweeps = sweep + 1;
-    if (count == 0) {
-      done = 1;
+    if (hipGraphLaunch(exec, st) != hipSuccess ||
+        hipMemcpyAsync(&count, rotations_dev, sizeof(int), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      rc = HSBA_ERR_DEVICE;
       break;
     }
+    *sweeps = sweep + 1;
+    if (count == 0) done = 1;
+    if (done) break;
   }
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  if (rc) return rc;
   if (!done) return HSBA_ERR_GENERIC;
   hipLaunchKernelGGL(k_eig_scale, dim3(nn), dim3(256), 0, st, A, Vt, ld, nn, thr, aux);
   // out = Vt^T (w Vt): element (a, b) = sum_j Vt[j][a] w_j Vt[j][b]

@@ -64,11 +64,13 @@ struct Contrib {        // one term of a reduced-system block
   int32_t trans;        // pair block used transposed
 };
 
-// Row stride of the per-key Z blocks (3 x E values of `tbytes` each): rounded
-// up to whole 128-byte cache lines, so a gathered block touches as few lines
-// as possible and loads as 16-byte vectors.
+// Row stride of the per-key Z blocks: the 3 x E values of Z_k, then the
+// point's q_p (3 values, read by the matrix-core pair kernels as the
+// right-hand-side column), rounded up to whole 128-byte cache lines, so a
+// gathered block touches as few lines as possible and loads as 16-byte
+// vectors.
 __host__ __device__ constexpr int z_stride(int e, int tbytes) {
-  return (3 * e * tbytes + 127) / 128 * (128 / tbytes);
+  return ((3 * e + 3) * tbytes + 127) / 128 * (128 / tbytes);
 }
 
 // Keys per image chunk of the image-block reduction.

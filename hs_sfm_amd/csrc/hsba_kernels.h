@@ -89,8 +89,8 @@ void dense_copy(const double *src, int64_t lds, double *dst, int64_t ldd, int64_
 // inverted, the others dropped (for a symmetric matrix this is the SVD
 // pseudo-inverse with singular values |lambda|); out (pitch ldo) receives the
 // result, lam the eigenvalues.  A is overwritten; Vt (ld x ld),
-// rotations_dev (max_sweeps ints) and work_dev (n + 1 + n doubles) are work
-// space.  HSBA_ERR_GENERIC when max_sweeps sweeps do not converge.
+// rotations_dev (one int) and work_dev (n + 1 + n doubles) are work space.
+// HSBA_ERR_GENERIC when max_sweeps sweeps do not converge.
 int dense_pinv(double *A, double *Vt, int64_t n, int64_t ld, double thr, double *out, int64_t ldo, int max_sweeps,
                int *rotations_dev, double *work_dev, std::vector<double> &lam, int *sweeps, hipStream_t st);
 

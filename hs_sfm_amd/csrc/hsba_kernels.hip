@@ -330,7 +330,7 @@ __global__ void k_point_factor(Dev<T> d) {
   d.q[3 * p + 2] = T(i20 * g0 + i21 * g1 + i22 * g2);
 }
 
-// Z_k = L_p^-1 W_k, one thread per key.  The thread's W loads go out before
+// Z_k = L_p^-1 W_k and q_p, one thread per key.  The thread's W loads go out before
 // the dependent point / window / factor lookups, so they overlap that chain;
 // the Z row (padding included) leaves as whole 16-byte stores, so every
 // 128-byte line of Z is written completely.
@@ -350,11 +350,13 @@ __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
   for (int e = 0; e < BLK; ++e) w[e] = wk[e];
   const int p = d.key_pt[k];
   if (!win_trial(d.ws[d.pt_win[p]])) return;
-  T l[6];
+  T l[6], qp[3];
 #pragma unroll
   for (int j = 0; j < 6; ++j) l[j] = d.Linv[6 * p + j];
-  // Z = L^-1 W (lower-triangular L^-1: l00 l10 l11 l20 l21 l22), padding zero,
-  // written as whole 16-byte vectors
+#pragma unroll
+  for (int j = 0; j < 3; ++j) qp[j] = d.q[3 * p + j];
+  // Z = L^-1 W (lower-triangular L^-1: l00 l10 l11 l20 l21 l22), then q_p,
+  // padding zero, written as whole 16-byte vectors
   vec out[ZS / VW];
 #pragma unroll
   for (int e = 0; e < ZS; ++e) {
@@ -362,6 +364,7 @@ __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
     if (e < E) v = l[0] * w[e];
     else if (e < 2 * E) v = l[1] * w[e - E] + l[2] * w[e];
     else if (e < BLK) v = l[3] * w[e - 2 * E] + l[4] * w[e - E] + l[5] * w[e];
+    else if (e < BLK + 3) v = qp[e - BLK];
     out[e / VW][e % VW] = v;
   }
   vec *z = (vec *)(d.Z + (int64_t)k * ZS);
@@ -463,15 +466,11 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d, const int32_t *plist, i
 
 // K4 on the matrix cores (camera blocks up to E = 15).  For a pair
 // (a, b), P_ab = A^T B where A and B stack the Z rows of the shared points (3
-// per point); for a diagonal pair column E of B carries q_p, so column E of
-// the product is rp_i.  One workgroup per pair walks its incidences 64 at a
-// time: the 256 threads gather the 128 Z rows of a stage into LDS with
-// 16-byte loads (4 threads per incidence), then each wavefront multiplies 16
-// incidences (48 stacked rows) with v_mfma_{f32,f64}_16x16x4, lane i + 16k
-// supplying A^T[i][k] = Z_a[row k][i] and B[k][i].  The product P[i][j] lands
-// in lane j + 16 (i / 4), element i % 4 for the f32 form and in lane
-// j + 16 (i % 4), element i / 4 for the f64 form (both checked on the
-// device).  The four wavefronts' partial products are summed in a fixed order.
+// per point); column E of B carries q_p, so for a diagonal pair column E of
+// the product is rp_i.  In v_mfma_{f32,f64}_16x16x4 lane i + 16k supplies
+// A^T[i][k] = Z_a[row k][i] and B[k][i]; the product P[i][j] lands in lane
+// j + 16 (i / 4), element i % 4 for the f32 form and in lane j + 16 (i % 4),
+// element i / 4 for the f64 form (both checked on the device).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 // Lanes of one wave see each other's earlier LDS writes after this point (the
@@ -502,77 +501,102 @@ struct Mma<double> {
   }
 };
 
+// Destination in the K-major LDS stage (see k_pairs_mma) of element e of the
+// concatenated [Z_a row | Z_b row] of one incidence (each row z_stride(E)
+// values: Z's 3 x E, then q_p): row r = e' / E, column e' % E for Z values; q_p
+// of the b row goes to column E of rows 0..2; everything else to the dump row.
+// Offsets are in elements of an array [2][rows + 1][16] and relative to the
+// incidence's first row, except the dump row, which is absolute (the caller
+// passes the incidence base to add).
+template <int E, int ZS>
+__device__ __forceinline__ int kmajor_dst(int e, int rows, int base) {
+  const int side = e < ZS ? 0 : 1;
+  const int f = e - side * ZS;
+  const int plane = side * (rows + 1) * 16;
+  if (f < 3 * E) return plane + base + (f / E) * 16 + (f % E);
+  if (side == 1 && f < 3 * E + 3) return plane + base + (f - 3 * E) * 16 + E;
+  return plane + rows * 16;
+}
+
+// One workgroup per image pair (ia <= ib) on the matrix cores
+// (v_mfma_f32_16x16x4_f32 / v_mfma_f64_16x16x4_f64): P = sum over incidences
+// of Z_a^T Z_b (E x E) and, in column E of the same 16 x 16 tile, Z_a^T q_p
+// (the pair's rhs part when ia == ib).  A stage of 64 incidences is laid out
+// in LDS K-major: every incidence gives 3 rows of 16 values per side,
+// [Z_a row r | 0...] and [Z_b row r | q_p[r] | 0...], so each v_mfma step is
+// two LDS reads at a fixed offset from the lane's base and the product.  Wave
+// w takes the stage's incidences 16 w .. 16 w + 15 (48 rows, 12 steps); its
+// partial tile is added to the others at the end.  Loading: 4 lanes per
+// incidence, each moving a fixed quarter of the two Z rows (16-byte loads,
+// element stores to destinations computed once per lane); the next stage's
+// rows are in flight while this one is multiplied, and the incidence codes
+// of the stage after.  Replaces the O(I^2 P) loops of
+// camera_shared_normal_equation_solver.hpp:198-443.
 template <typename T, int E>
 __global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d, const int32_t *plist) {
   static_assert(E >= 1 && E <= 15, "one 16-wide tile holds the E columns and the rhs column");
   constexpr int ZS = z_stride(E, sizeof(T));
   constexpr int VW = 16 / sizeof(T), NV = ZS / VW;
-  constexpr int SI = 64;  // incidences per stage
+  constexpr int SI = 64;            // incidences per stage
+  constexpr int KR = 3 * SI;        // K rows per stage
+  constexpr int NU = 2 * NV / 4;    // vectors a lane moves per incidence
   typedef T vec __attribute__((ext_vector_type(VW)));
   typedef typename Mma<T>::acc_t acc_t;
-  __shared__ vec sz[2][SI][NV];
-  __shared__ T sq[SI][4];
+  __shared__ T lds[2][KR + 1][16];  // K-major a and b sides; row KR takes the unread elements
   __shared__ T part[3][64][4];
   const int pair = plist ? plist[blockIdx.x] : blockIdx.x;
+  if (pair < 0) return;  // an empty slot of the launch order (the whole workgroup)
   if (!win_trial(d.ws[d.pair_win[pair]])) return;
   const bool diag = d.pair_img[pair] >= 0;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int col = lane & 15, kr = lane >> 4;
   const int ls = t >> 2, lq = t & 3;  // loader: incidence, quarter of its 2 NV vectors
+  T *flat = &lds[0][0][0];
+  for (int e = t; e < 2 * (KR + 1) * 16; e += 256) flat[e] = 0;  // columns no stage writes stay zero
+  int dst[NU * VW];
+#pragma unroll
+  for (int u = 0; u < NU; ++u)
+#pragma unroll
+    for (int c = 0; c < VW; ++c) dst[u * VW + c] = kmajor_dst<E, ZS>((lq + 4 * u) * VW + c, KR, 3 * ls * 16);
   const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
   acc_t acc = {0, 0, 0, 0};
-  // software pipeline: while stage s is multiplied, the Z rows of stage s + 1
-  // are in flight (registers) and so is the incidence code of stage s + 2
-  vec v[2 * NV / 4];
-  T qv[3] = {0, 0, 0};
+  vec v[NU];
   auto fetch = [&](uint64_t code) {
-    const int a = inc_a(code), b = inc_b(code);
-    const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
-    const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
+    const vec *ra = (const vec *)(d.Z + (int64_t)inc_a(code) * ZS);
+    const vec *rb = (const vec *)(d.Z + (int64_t)inc_b(code) * ZS);
 #pragma unroll
-    for (int u = 0; u < 2 * NV / 4; ++u) {
+    for (int u = 0; u < NU; ++u) {
       const int q = lq + 4 * u;
       v[u] = q < NV ? ra[q] : rb[q - NV];
     }
-    if (diag && lq == 0) {
-      const int p = d.key_pt[a];
-      qv[0] = d.q[3 * p]; qv[1] = d.q[3 * p + 1]; qv[2] = d.q[3 * p + 2];
-    }
+  };
+  auto clear = [&]() {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) v[u] = vec{0};
   };
   uint64_t code_next = 0;
-  {
-    const int n0 = je - jb < SI ? (int)(je - jb) : SI;
-    if (ls < n0) fetch(d.inc[jb + ls]);
-    if (jb + SI + ls < je) code_next = d.inc[jb + SI + ls];
-  }
+  if (jb + ls < je) fetch(d.inc[jb + ls]);
+  else clear();
+  if (jb + SI + ls < je) code_next = d.inc[jb + SI + ls];
+  const T *pa = &lds[0][48 * wave + kr][col], *pb = &lds[1][48 * wave + kr][col];
+  __syncthreads();  // the zeroed rows before the first stage's stores
   for (int64_t base = jb; base < je; base += SI) {
     const int n = je - base < SI ? (int)(je - base) : SI;
-    if (ls < n) {
+    // stage the rows (zeros for the slots past the pair's last incidence)
 #pragma unroll
-      for (int u = 0; u < 2 * NV / 4; ++u) {
-        const int q = lq + 4 * u;
-        sz[q < NV ? 0 : 1][ls][q < NV ? q : q - NV] = v[u];
-      }
-      if (diag && lq == 0) { sq[ls][0] = qv[0]; sq[ls][1] = qv[1]; sq[ls][2] = qv[2]; }
-    }
+    for (int u = 0; u < NU; ++u)
+#pragma unroll
+      for (int c = 0; c < VW; ++c) flat[dst[u * VW + c]] = v[u][c];
     __syncthreads();
     if (base + SI + ls < je) fetch(code_next);
+    else clear();
     code_next = (base + 2 * SI + ls < je) ? d.inc[base + 2 * SI + ls] : 0;
-    const int rows = 3 * max(0, min(16, n - 16 * wave));
-    for (int g0 = 0; g0 < rows; g0 += 4) {
-      const int g = g0 + kr;
-      T av = 0, bv = 0;
-      if (g < rows) {
-        const int s = 16 * wave + g / 3, r = g - 3 * (g / 3);
-        const T *za = (const T *)sz[0][s], *zb = (const T *)sz[1][s];
-        if (col < E) {
-          av = za[r * E + col];
-          bv = zb[r * E + col];
-        } else if (diag && col == E) {
-          bv = sq[s][r];
-        }
-      }
-      acc = Mma<T>::op(av, bv, acc);
+    const int m = max(0, min(16, n - 16 * wave));  // this wave's incidences
+    if (m == 16) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) acc = Mma<T>::op(pa[64 * k], pb[64 * k], acc);
+    } else {
+      for (int k = 0; 4 * k < 3 * m; ++k) acc = Mma<T>::op(pa[64 * k], pb[64 * k], acc);
     }
     __syncthreads();
   }
@@ -595,83 +619,75 @@ __global__ void __launch_bounds__(256) k_pairs_mma(Dev<T> d, const int32_t *plis
 // One wavefront per image pair on the matrix cores, four independent pairs
 // per workgroup and no workgroup barrier: for short incidence lists, where
 // k_pairs_mma leaves most of its loaders and waves idle and still pays the
-// cross-wave combine.  A stage is 16 incidences (4 lanes load each one's two
-// Z rows); while it is multiplied (48 rows of K = 3 per incidence, 12
-// v_mfma steps) the next stage's rows are in flight and the incidence codes
-// of the one after.  One accumulator per pair: the incidences are summed in
-// list order.
+// cross-wave combine.  A stage is 16 incidences in the wave's own K-major LDS
+// rows (as in k_pairs_mma; 4 lanes load each incidence); while it is
+// multiplied (48 rows, 12 v_mfma steps) the next stage's rows are in flight
+// and the incidence codes of the one after.  One accumulator per pair: the
+// incidences are summed in list order.
 template <typename T, int E>
 __global__ void __launch_bounds__(256) k_pairs_mma_wave(Dev<T> d, const int32_t *plist, int count) {
   static_assert(E >= 1 && E <= 15, "one 16-wide tile holds the E columns and the rhs column");
   constexpr int ZS = z_stride(E, sizeof(T));
   constexpr int VW = 16 / sizeof(T), NV = ZS / VW;
-  constexpr int SI = 16;  // incidences per stage
+  constexpr int SI = 16;            // incidences per stage
+  constexpr int KR = 3 * SI;        // K rows per stage
+  constexpr int NU = 2 * NV / 4;    // vectors a lane moves per incidence
   typedef T vec __attribute__((ext_vector_type(VW)));
   typedef typename Mma<T>::acc_t acc_t;
-  __shared__ vec sz[4][2][SI][NV];
-  __shared__ T sq[4][SI][4];
+  __shared__ T lds[4][2][KR + 1][16];  // per wave: K-major a and b sides, row KR for unread elements
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int slot = blockIdx.x * 4 + wave;
   if (slot >= count) return;  // the whole wave
   const int pair = plist ? plist[slot] : slot;
+  if (pair < 0) return;  // an empty slot of the launch order
   if (!win_trial(d.ws[d.pair_win[pair]])) return;
   const bool diag = d.pair_img[pair] >= 0;
   const int col = lane & 15, kr = lane >> 4;
   const int ls = lane >> 2, lq = lane & 3;  // loader: incidence, quarter of its 2 NV vectors
-  const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
-  vec (*wz)[SI][NV] = sz[wave];
-  T (*wq)[4] = sq[wave];
-  acc_t acc = {0, 0, 0, 0};
-  vec v[2 * NV / 4];
-  T qv[3] = {0, 0, 0};
-  auto fetch = [&](uint64_t code) {
-    const int a = inc_a(code), b = inc_b(code);
-    const vec *ra = (const vec *)(d.Z + (int64_t)a * ZS);
-    const vec *rb = (const vec *)(d.Z + (int64_t)b * ZS);
+  T *flat = &lds[wave][0][0][0];
+  for (int e = lane; e < 2 * (KR + 1) * 16; e += 64) flat[e] = 0;
+  int dst[NU * VW];
 #pragma unroll
-    for (int u = 0; u < 2 * NV / 4; ++u) {
+  for (int u = 0; u < NU; ++u)
+#pragma unroll
+    for (int c = 0; c < VW; ++c) dst[u * VW + c] = kmajor_dst<E, ZS>((lq + 4 * u) * VW + c, KR, 3 * ls * 16);
+  const int64_t jb = d.pair_ptr[pair], je = d.pair_ptr[pair + 1];
+  acc_t acc = {0, 0, 0, 0};
+  vec v[NU];
+  auto fetch = [&](uint64_t code) {
+    const vec *ra = (const vec *)(d.Z + (int64_t)inc_a(code) * ZS);
+    const vec *rb = (const vec *)(d.Z + (int64_t)inc_b(code) * ZS);
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
       const int q = lq + 4 * u;
       v[u] = q < NV ? ra[q] : rb[q - NV];
     }
-    if (diag && lq == 0) {
-      const int p = d.key_pt[a];
-      qv[0] = d.q[3 * p]; qv[1] = d.q[3 * p + 1]; qv[2] = d.q[3 * p + 2];
-    }
+  };
+  auto clear = [&]() {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) v[u] = vec{0};
   };
   uint64_t code_next = 0;
-  {
-    const int n0 = je - jb < SI ? (int)(je - jb) : SI;
-    if (ls < n0) fetch(d.inc[jb + ls]);
-    if (jb + SI + ls < je) code_next = d.inc[jb + SI + ls];
-  }
+  if (jb + ls < je) fetch(d.inc[jb + ls]);
+  else clear();
+  if (jb + SI + ls < je) code_next = d.inc[jb + SI + ls];
+  const T *pa = &lds[wave][0][kr][col], *pb = &lds[wave][1][kr][col];
+  wave_lds_sync();  // the zeroed rows before the first stage's stores
   for (int64_t base = jb; base < je; base += SI) {
     const int n = je - base < SI ? (int)(je - base) : SI;
-    if (ls < n) {
 #pragma unroll
-      for (int u = 0; u < 2 * NV / 4; ++u) {
-        const int q = lq + 4 * u;
-        wz[q < NV ? 0 : 1][ls][q < NV ? q : q - NV] = v[u];
-      }
-      if (diag && lq == 0) { wq[ls][0] = qv[0]; wq[ls][1] = qv[1]; wq[ls][2] = qv[2]; }
-    }
+    for (int u = 0; u < NU; ++u)
+#pragma unroll
+      for (int c = 0; c < VW; ++c) flat[dst[u * VW + c]] = v[u][c];
     wave_lds_sync();
     if (base + SI + ls < je) fetch(code_next);
+    else clear();
     code_next = (base + 2 * SI + ls < je) ? d.inc[base + 2 * SI + ls] : 0;
-    const int rows = 3 * n;
-    for (int g0 = 0; g0 < rows; g0 += 4) {
-      const int g = g0 + kr;
-      T av = 0, bv = 0;
-      if (g < rows) {
-        const int s = g / 3, r = g - 3 * s;
-        const T *za = (const T *)wz[0][s], *zb = (const T *)wz[1][s];
-        if (col < E) {
-          av = za[r * E + col];
-          bv = zb[r * E + col];
-        } else if (diag && col == E) {
-          bv = wq[s][r];
-        }
-      }
-      acc = Mma<T>::op(av, bv, acc);
+    if (n == SI) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) acc = Mma<T>::op(pa[64 * k], pb[64 * k], acc);
+    } else {
+      for (int k = 0; 4 * k < 3 * n; ++k) acc = Mma<T>::op(pa[64 * k], pb[64 * k], acc);
     }
     wave_lds_sync();  // the stage's rows are read before the next stage overwrites them
   }
