@@ -88,9 +88,13 @@ int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_opti
   // FP64 dense-Schur problems whose camera block is 7 or more wide are switched
   // off: an FP64 LM on the 200-camera x 9-parameter scene faulted the GPU once
   // (with an earlier pair kernel that spilled ~1350 SGPRs), and no cause in
-  // this code has been shown.  Kept off, with no override, until one is.
+  // this code has been shown.  Kept off on the GPU, with no override, until
+  // one is; only the bounds-checked CPU emulation of the kernels
+  // (HSBA_CPU_EMULATION, a host build) runs this path.
+#ifndef HSBA_CPU_EMULATION
   if (o.precision == HSBA_PRECISION_FP64 && eng->L.pts_free && eng->L.E >= 7)
     return HSBA_ERR_UNSUPPORTED;
+#endif
   eng->precision = o.precision;
   eng->pcg_rtol = o.pcg_rtol;
   eng->pcg_max_iterations = o.pcg_max_iterations;

@@ -8,7 +8,7 @@ SRC=$HERE/../../hs_sfm_amd/csrc
 OUT=$HERE/_build
 mkdir -p "$OUT"
 CXX=/opt/rocm/llvm/bin/clang++
-FLAGS="-std=c++20 -O1 -g -fPIC -fsanitize=address -fno-omit-frame-pointer -I$HERE/include -Wno-unused-result -Wno-psabi -Wno-unknown-attributes"
+FLAGS="-std=c++20 -O1 -g -fPIC -DHSBA_CPU_EMULATION -fsanitize=address -fno-omit-frame-pointer -I$HERE/include -Wno-unused-result -Wno-psabi -Wno-unknown-attributes"
 pids=()
 for f in hsba_kernels.hip hsba_cholesky.hip hsba_scratch.hip hsba_dense.hip hsba_layout.cpp hsba_engine.cpp hsba_server.cpp hsba_naive.cpp hsba_io.cpp hsba_api.cpp hsba_tune.cpp; do
   if [ "$SRC/$f" -nt "$OUT/$f.o" ] || [ "$HERE/include/hip/hip_runtime.h" -nt "$OUT/$f.o" ] || [ "$SRC/hsba_device.h" -nt "$OUT/$f.o" ] || [ "$SRC/hsba_kernels.h" -nt "$OUT/$f.o" ] || [ "$SRC/hsba_host.h" -nt "$OUT/$f.o" ]; then
