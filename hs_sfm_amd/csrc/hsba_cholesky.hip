@@ -27,6 +27,8 @@
 //
 // A non-positive pivot marks the window (WinState::chol_fail); the reference
 // reports the same condition as SimplicialLDLT's info() != Success.
+#include <algorithm>
+
 #include "hsba_kernels.h"
 
 namespace hsba {
@@ -428,13 +430,21 @@ __device__ __forceinline__ void tile_gemv(const double *M, int64_t ld, const dou
 __device__ __forceinline__ bool chol_active(const WinState &s) { return s.active && s.trial && !s.chol_fail; }
 
 // ---------------------------------------------------------------- large path
+// Every large-path launch covers all large windows of the batch: the window
+// of a workgroup is large_list[blockIdx.y], and each window runs exactly the
+// work (and order of work) it runs alone, so its results do not depend on
+// the batch.  Windows with fewer tiles finish their steps earlier; their
+// workgroups of later launches return at once.
+
 // Diagonal tile 0 (before the first step).
-__global__ void __launch_bounds__(256) k_chol_first(WinInfo wi, WinState *ws, int w, const double *S, double *Lf,
-                                                    double *Dinv) {
+__global__ void __launch_bounds__(256) k_chol_first(const WinInfo *wis, WinState *ws, const int *large_list,
+                                                    const double *S, double *Lf, double *Dinv) {
   __shared__ double wa[NB][LDT];
   __shared__ double wx[NB][LDT];
   __shared__ FactorScratch fs;
   __shared__ int fail;
+  const int w = large_list[blockIdx.y];
+  const WinInfo wi = wis[w];
   WinState &s = ws[w];
   if (!chol_active(s)) return;
   if (threadIdx.x == 0) fail = 0;
@@ -444,10 +454,11 @@ __global__ void __launch_bounds__(256) k_chol_first(WinInfo wi, WinState *ws, in
   if (threadIdx.x == 0 && fail) s.chol_fail = 1;
 }
 
-// Step k: trailing tiles (i, j), k < j <= i < nb, in row order, then the
-// right-hand-side workgroups j = k+1 .. nb-1.
-__global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int w, double *S, double *Lf,
-                                                   double *Dinv, double *b, double *y, int k, int ntiles) {
+// Step k of window large_list[blockIdx.y]: trailing tiles (i, j),
+// k < j <= i < nb, in row order, then the right-hand-side workgroups
+// j = k+1 .. nb-1.
+__global__ void __launch_bounds__(256) k_chol_step(const WinInfo *wis, WinState *ws, const int *large_list,
+                                                   double *S, double *Lf, double *Dinv, double *b, double *y, int k) {
   __shared__ double li[NB][LDT];
   __shared__ double pa[NB][LDT];
   __shared__ double pb[NB][LDT];
@@ -455,6 +466,11 @@ __global__ void __launch_bounds__(256) k_chol_step(WinInfo wi, WinState *ws, int
   __shared__ double red[4][NB];
   __shared__ FactorScratch fs;
   __shared__ int fail;
+  const int w = large_list[blockIdx.y];
+  const WinInfo wi = wis[w];
+  if (k + 1 >= wi.nb) return;  // this window's factorization is complete
+  const int rest = wi.nb - k - 1, ntiles = rest * (rest + 1) / 2;
+  if ((int)blockIdx.x >= ntiles + rest) return;
   WinState &s = ws[w];
   if (!chol_active(s)) return;
   const int64_t ld = wi.ld;
@@ -548,12 +564,19 @@ __device__ __forceinline__ double reduce16(double part, double (*red)[NB]) {
 // with every tile of the chain fetched into registers before the first
 // product (the chain is then bound by its reductions, not by memory latency);
 // workgroup ib < k_lo then applies y_ib -= sum_{m = k_lo..k_hi} L_m,ib^T x_m.
-__global__ void __launch_bounds__(BT) k_chol_back(WinInfo wi, WinState *ws, int w, const double *Lf,
-                                                  const double *Dinv, const double *b, double *y, double *x,
-                                                  int k_hi, int k_lo) {
+__global__ void __launch_bounds__(BT) k_chol_back(const WinInfo *wis, WinState *ws, const int *large_list,
+                                                  const double *Lf, const double *Dinv, const double *b, double *y,
+                                                  double *x, int q) {
   __shared__ double xs[4][NB];
   __shared__ double vs[NB];
   __shared__ double red[16][NB];
+  const int w = large_list[blockIdx.y];
+  const WinInfo wi = wis[w];
+  // launch q solves block rows nb - 1 - 4 q down to (at most) nb - 4 - 4 q
+  const int k_hi = wi.nb - 1 - 4 * q;
+  if (k_hi < 0) return;
+  const int k_lo = k_hi >= 3 ? k_hi - 3 : 0;
+  if ((int)blockIdx.x >= (k_lo > 0 ? k_lo : 1)) return;
   const WinState &s = ws[w];
   if (!chol_active(s)) return;
   const int64_t ld = wi.ld;
@@ -833,7 +856,9 @@ __global__ void k_pad_identity(const WinInfo *wis, int nwin, double *S) {
 }
 
 // b = rhs with zero padding (large windows: the step kernels update b in place).
-__global__ void k_rhs_copy(WinInfo wi, WinState *ws, int w, const double *rhs, double *b) {
+__global__ void k_rhs_copy(const WinInfo *wis, WinState *ws, const int *large_list, const double *rhs, double *b) {
+  const int w = large_list[blockIdx.y];
+  const WinInfo wi = wis[w];
   const WinState &s = ws[w];
   if (!chol_active(s)) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wi.ld; i += gridDim.x * blockDim.x)
@@ -850,21 +875,33 @@ void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int
   if (num_small > 0)
     hipLaunchKernelGGL(k_chol_small, dim3(num_small), dim3(256), 0, st, plan.wi, plan.ws, small_list_dev,
                        plan.S, plan.Dinv, plan.rhs, plan.dred);
-  for (int q = 0; q < num_large; ++q) {
-    const int w = large_host[q];
-    const WinInfo wi = plan.wi_host[w];
-    hipLaunchKernelGGL(k_rhs_copy, dim3(8), dim3(256), 0, st, wi, plan.ws, w, plan.rhs, plan.bwork);
-    hipLaunchKernelGGL(k_chol_first, dim3(1), dim3(256), 0, st, wi, plan.ws, w, plan.S, plan.Lf, plan.Dinv);
-    for (int k = 0; k + 1 < wi.nb; ++k) {
-      const int rest = wi.nb - k - 1;
-      const int ntiles = rest * (rest + 1) / 2;
-      hipLaunchKernelGGL(k_chol_step, dim3(ntiles + rest), dim3(256), 0, st, wi, plan.ws, w, plan.S, plan.Lf,
-                         plan.Dinv, plan.bwork, plan.ywork, k, ntiles);
+  if (num_large > 0) {
+    // every launch covers all large windows (blockIdx.y); the grid is sized
+    // for the window with the most work in that launch
+    int max_nb = 0;
+    for (int q = 0; q < num_large; ++q) max_nb = std::max(max_nb, plan.wi_host[large_host[q]].nb);
+    const dim3 win_y(1, num_large);
+    hipLaunchKernelGGL(k_rhs_copy, dim3(8, num_large), dim3(256), 0, st, plan.wi, plan.ws, plan.large_list,
+                       plan.rhs, plan.bwork);
+    hipLaunchKernelGGL(k_chol_first, win_y, dim3(256), 0, st, plan.wi, plan.ws, plan.large_list, plan.S, plan.Lf,
+                       plan.Dinv);
+    for (int k = 0; k + 1 < max_nb; ++k) {
+      int slots = 0;
+      for (int q = 0; q < num_large; ++q) {
+        const int rest = plan.wi_host[large_host[q]].nb - k - 1;
+        if (rest > 0) slots = std::max(slots, rest * (rest + 1) / 2 + rest);
+      }
+      hipLaunchKernelGGL(k_chol_step, dim3(slots, num_large), dim3(256), 0, st, plan.wi, plan.ws, plan.large_list,
+                         plan.S, plan.Lf, plan.Dinv, plan.bwork, plan.ywork, k);
     }
-    for (int k_hi = wi.nb - 1; k_hi >= 0; k_hi -= 4) {
-      const int k_lo = k_hi >= 3 ? k_hi - 3 : 0;
-      hipLaunchKernelGGL(k_chol_back, dim3(k_lo > 0 ? k_lo : 1), dim3(BT), 0, st, wi, plan.ws, w, plan.Lf,
-                         plan.Dinv, plan.bwork, plan.ywork, plan.dred, k_hi, k_lo);
+    for (int q = 0; 4 * q < max_nb; ++q) {
+      int blocks = 1;
+      for (int u = 0; u < num_large; ++u) {
+        const int k_hi = plan.wi_host[large_host[u]].nb - 1 - 4 * q;
+        if (k_hi >= 4) blocks = std::max(blocks, k_hi - 3);
+      }
+      hipLaunchKernelGGL(k_chol_back, dim3(blocks, num_large), dim3(BT), 0, st, plan.wi, plan.ws, plan.large_list,
+                         plan.Lf, plan.Dinv, plan.bwork, plan.ywork, plan.dred, q);
     }
   }
 }

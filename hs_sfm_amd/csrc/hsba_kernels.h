@@ -64,6 +64,7 @@ struct CholPlan {
   double *Lf;                // factor of large windows (same layout as S)
   double *bwork, *ywork;     // right-hand side being reduced / forward solution
   const WinInfo *wi_host;    // host copy (large-window launch loop)
+  const int *large_list;     // device copy of the large windows' ids
 };
 void chol_init_padding(const CholPlan &plan, hipStream_t st);
 void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int num_small,

@@ -182,7 +182,31 @@ def case_x_covariance():
             f"{e_free:.1e}; fixed gauge (Schur) {e_s:.1e}")
 
 
+def case_fp64_wide(num_cameras=None, num_points=None, iterations=None):
+    """The FP64 LM on 9-parameter cameras (camera block E = 9, free points):
+    the path switched off on the GPU.  Default: a 40-camera, 2000-point
+    scene of the 1M-key scene's kind for two iterations; HSBA_EMU_WIDE =
+    "cameras,points,iterations" sets the size (200,50000,1 is the full
+    1M-key scene).  Every device index is bounds-checked by AddressSanitizer;
+    the final cost is compared with the FP64 oracle."""
+    spec = os.environ.get("HSBA_EMU_WIDE", "40,2000,2").split(",")
+    C = num_cameras or int(spec[0])
+    P = num_points or int(spec[1])
+    K = iterations or int(spec[2])
+    s = syn.mid_scene(num_cameras=C, num_points=P, kmax=min(40, C))
+    p = Problem(s, precision=abi.PRECISION_FP64)
+    rc, x, st = p.lm(max_iterations=K)
+    assert rc == 0, rc
+    rc_o, xo, sto, _ = Oracle(s).lm(max_iterations=K)
+    assert rc_o == 0
+    err = abs(st.final_cost - sto.final_cost) / sto.final_cost
+    assert err < 1e-9, err
+    return (f"FP64 E=9 LM x{K}: {s.num_keys} keys, n = {p.n_reduced}, final cost rel err {err:.1e}, "
+            f"rmse {syn.reprojection_rmse(s, x):.6f} px")
+
+
 CASES = {"tiled_cholesky": case_tiled_cholesky, "small_mma": case_small_mma, "lm_tiled": case_lm_tiled,
+         "fp64_wide": case_fp64_wide,
          "batch": case_batch, "batch_xcd": case_batch_xcd, "covariance_static": case_covariance_static,
          "x_covariance": case_x_covariance}
 

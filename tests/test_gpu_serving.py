@@ -37,6 +37,26 @@ def test_batch_bitwise_matches_single(gpu):
         assert st.final_cost == sta.final_cost and st.iterations == sta.iterations
 
 
+def test_large_windows_batch_bitwise_matches_single(gpu):
+    """Windows whose reduced systems take the tiled factorization (more than
+    four 64-row tiles) share each launch of the large path; tile counts
+    differ (6, 8 and 5 tiles), so windows finish their steps at different
+    launches.  Each result equals the window solved alone, bit for bit, and
+    the FP64 oracle's RMSE within 1e-3 px."""
+    scenes = [syn.mid_scene(seed=21, num_cameras=40, num_points=1500, kmax=20, name="L40"),
+              syn.mid_scene(seed=22, num_cameras=53, num_points=1800, kmax=20, name="L53"),
+              syn.mid_scene(seed=23, num_cameras=33, num_points=1200, kmax=20, name="L33")]
+    assert [(9 * s.num_images + 63) // 64 for s in scenes] == [6, 8, 5]
+    rc, xs, sts = Batch(scenes).lm(max_iterations=5)
+    assert rc == 0
+    for s, x, st in zip(scenes, xs, sts):
+        xa, sta = alone(s, 5)
+        assert np.array_equal(x, xa), s.name
+        assert st.final_cost == sta.final_cost and st.iterations == sta.iterations
+        _, xo, _, _ = Oracle(s).lm(max_iterations=5)
+        assert abs(syn.reprojection_rmse(s, x) - syn.reprojection_rmse(s, xo)) < 1e-3, s.name
+
+
 def mixed_jobs():
     rng = np.random.default_rng(7)
     jobs = []
