@@ -234,9 +234,10 @@ def run_mid(args):
                "initial_cost": st.initial_cost, "final_cost": st.final_cost, "rmse_px": rmse,
                "mean_error_px": syn.mean_reprojection_error(scene, x),
                "rmse_x0_px": syn.reprojection_rmse(scene, scene.x0)},
-        "fp64_gpu_plain": {"status": "switched off (HSBA_ERR_UNSUPPORTED)",
+        "fp64_gpu_plain": {"status": "switched off (HSBA_ERR_UNSUPPORTED) for this scene's 9-parameter cameras",
                            "why": "an FP64 LM on this scene faulted the GPU once and no cause in the code has "
-                                  "been shown; the FP64 path with camera blocks >= 7 stays off"},
+                                  "been shown; the FP64 path with camera blocks >= 7 stays off",
+                           "calibrated_cameras": fp64_vs_mixed_calibrated(scene, K)},
     }
     if ref and ref.get("iterations") == K:
         out["lm"]["fp64_oracle_rmse_px"] = ref["rmse_px"]
@@ -248,6 +249,37 @@ def run_mid(args):
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_mid(scene, samples=args.cpu_samples)
     p.close()
+    return out
+
+
+def fp64_vs_mixed_calibrated(scene, K):
+    """The same 1M-key problem with every camera held at its true value
+    (calibrated cameras, camera block E = 6, which the FP64 path serves):
+    K LM iterations from x0 in FP64 end to end and in mixed precision, each
+    after one untimed run, with the FP64 result as the accuracy reference."""
+    s = syn.hold_cameras(scene)
+    out = {"workload": f"the 1M-key scene with cameras held (camera block 6): {s.num_keys} keys, "
+                       f"reduced size {6 * s.num_images}, {K} LM iterations"}
+    xs = {}
+    for name, prec in (("fp64", abi.PRECISION_FP64), ("mixed", abi.PRECISION_MIXED)):
+        p = Problem(s, precision=prec)
+        abi.check(p.lm_begin(max_iterations=K), "lm_begin")
+        abi.check(p.lm_iterate(K)[0], "lm_iterate")
+        p.lm_end()
+        abi.check(p.lm_begin(max_iterations=K), "lm_begin")
+        t0 = time.perf_counter()
+        rc, _ = p.lm_iterate(K)
+        dt = time.perf_counter() - t0
+        abi.check(rc, "lm_iterate")
+        _, x, st = p.lm_end()
+        p.close()
+        xs[name] = x
+        out[name] = {"iters_per_s": K / dt, "ms_per_iter": 1e3 * dt / K, "accepted": st.accepted,
+                     "final_cost": st.final_cost, "rmse_px": syn.reprojection_rmse(s, x)}
+    out["mixed_over_fp64"] = out["mixed"]["iters_per_s"] / out["fp64"]["iters_per_s"]
+    out["mixed_rmse_delta_vs_fp64_px"] = abs(out["mixed"]["rmse_px"] - out["fp64"]["rmse_px"])
+    out["mixed_final_cost_rel_err_vs_fp64"] = (abs(out["mixed"]["final_cost"] - out["fp64"]["final_cost"]) /
+                                               out["fp64"]["final_cost"])
     return out
 
 

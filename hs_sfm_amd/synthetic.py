@@ -368,6 +368,17 @@ def mid_scene(seed: int = SEED_MID, num_cameras: int = 200, num_points: int = 50
     return scene
 
 
+def hold_cameras(scene: Scene, name: Optional[str] = None) -> Scene:
+    """The same problem with every camera block held at its true value
+    (FIX_CAMERAS, calibrated cameras): same keys and observations, camera
+    block E = 6 (the image extrinsics)."""
+    p, i, _ = scene.x_layout()
+    return dataclasses.replace(scene, fix_mask=scene.fix_mask | FIX_CAMERAS,
+                               fix_cameras=scene.x_true[p + i:].reshape(scene.num_cameras, -1).copy(),
+                               x_true=scene.x_true[:p + i].copy(), x0=scene.x0[:p + i].copy(),
+                               name=name or scene.name + "_calibrated")
+
+
 def large_scene(seed: int = SEED_LARGE, num_cameras: int = 2000, num_points: int = 600000,
                 kmin: int = 2, kmax: int = 15, altitude: float = 40.0, step: float = 2.0,
                 focal: float = 1000.0, image_size: float = 1000.0, name="large") -> Scene:
