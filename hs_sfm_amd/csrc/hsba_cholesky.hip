@@ -7,10 +7,11 @@
 // the identity once.  64 x 64 tiles are staged through LDS.
 //
 // Diagonal tiles: a blocked factorization with 16-wide panels (each 16 x 16
-// diagonal block eliminated by the whole workgroup, one element per thread,
-// four columns per barrier; the block products on the matrix cores) yields
-// L_kk^-1, the only form of the diagonal factor tiles that the solves and the
-// inverse read; panel tiles are then GEMMs with L_kk^-1.
+// diagonal block eliminated by one wavefront, four columns per step on the
+// matrix cores, while the other three waves form the panel, trailing and
+// inverse blocks around it) yields L_kk^-1, the only form of the diagonal
+// factor tiles that the solves and the inverse read; panel tiles are then
+// GEMMs with L_kk^-1.
 //
 // Large windows (more than 4 tiles): one fused launch per panel step k.
 // Every workgroup of step k owns one trailing tile (i, j), k < j <= i: it forms
@@ -86,93 +87,6 @@ __device__ __forceinline__ void wave_sync() {
 
 constexpr int PB = 16;  // panel width inside a 64 x 64 tile
 
-struct alignas(32) DQuad {
-  double v[4];
-};
-
-struct FactorScratch {
-  DQuad col[2][PB];   // a[i][j .. j+3]: the four columns of the pivot block being eliminated
-  DQuad xrow[2][PB];  // x[j .. j+3][c]: the four rows of the inverse at that block
-  double d[PB];       // pivots
-};
-
-// The whole workgroup (256 threads, one element each): factor the 16 x 16
-// diagonal block of wa at (c0, c0) (lower triangle, SPD) in place into its
-// Cholesky factor L_p, and write L_p^-1 (lower, zero upper) to the same block
-// of wx.  Elimination of [A | I] by 4 x 4 pivot blocks, one barrier per
-// block: the block's four columns of a and four rows of x are published as
-// they stand; every thread forms P = L_P D_P L_P^T (unit lower L_P; four
-// reciprocals, computed redundantly in registers) and, by forward
-// substitution with L_P,
-//   y_i = a_iJ L_P^-T,  y_c = a_cJ L_P^-T,  w_c = L_P^-1 x_J[:, c],  z_i = y_i D_P^-1
-// (a_iJ: row i of the block's columns, x_J: the block's rows of x), then
-//   a[i][c] = y_i[c - j]               (c in the block, i >= c)
-//   a[i][c] -= z_i . y_c               (c beyond the block, i >= c)
-//   x[i][c] = w_c[i - j]               (i in the block)
-//   x[i][c] -= z_i . w_c               (i beyond the block)
-// which is four single-column steps of LDL^T at once.  At the end a = Lt D
-// and x = Lt^-1 (Lt unit lower), so L = Lt D^1/2 and L^-1 = D^-1/2 x.
-// Thread t owns element (t / 16, t % 16).
-__device__ __forceinline__ void diag16(double (*wa)[LDT], double (*wx)[LDT], int c0, FactorScratch &fs,
-                                       bool &bad) {
-  const int tid = threadIdx.x;
-  const int i = tid >> 4, c = tid & 15;
-  double a = (i >= c) ? wa[c0 + i][c0 + c] : 0.0;
-  double x = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-  for (int j = 0; j < PB; j += 4) {
-    const int b = (j >> 2) & 1;
-    if (c >= j && c < j + 4) fs.col[b][i].v[c - j] = a;
-    if (i >= j && i < j + 4) fs.xrow[b][c].v[i - j] = x;
-    __syncthreads();
-    const DQuad ai = fs.col[b][i], ac = fs.col[b][c], xr = fs.xrow[b][c];
-    const DQuad p1 = fs.col[b][j + 1], p2 = fs.col[b][j + 2], p3 = fs.col[b][j + 3];
-    const double d0 = fs.col[b][j].v[0];
-    // P = L_P D_P L_P^T (t_rs = l_rs d_s)
-    const double r0 = d0 > 0.0 ? recip_nr(d0) : 0.0;
-    const double l10 = p1.v[0] * r0, l20 = p2.v[0] * r0, l30 = p3.v[0] * r0;
-    const double d1 = p1.v[1] - l10 * p1.v[0];
-    const double r1 = d1 > 0.0 ? recip_nr(d1) : 0.0;
-    const double t21 = p2.v[1] - l20 * p1.v[0], t31 = p3.v[1] - l30 * p1.v[0];
-    const double l21 = t21 * r1, l31 = t31 * r1;
-    const double d2 = p2.v[2] - l20 * p2.v[0] - l21 * t21;
-    const double r2 = d2 > 0.0 ? recip_nr(d2) : 0.0;
-    const double t32 = p3.v[2] - l30 * p2.v[0] - l31 * t21;
-    const double l32 = t32 * r2;
-    const double d3 = p3.v[3] - l30 * p3.v[0] - l31 * t31 - l32 * t32;
-    const double r3 = d3 > 0.0 ? recip_nr(d3) : 0.0;
-    bad = bad || !(d0 > 0.0 && d1 > 0.0 && d2 > 0.0 && d3 > 0.0);
-    // forward substitutions with L_P
-    double yi[4], yc[4], wc[4];
-    yi[0] = ai.v[0];
-    yi[1] = ai.v[1] - l10 * yi[0];
-    yi[2] = ai.v[2] - l20 * yi[0] - l21 * yi[1];
-    yi[3] = ai.v[3] - l30 * yi[0] - l31 * yi[1] - l32 * yi[2];
-    yc[0] = ac.v[0];
-    yc[1] = ac.v[1] - l10 * yc[0];
-    yc[2] = ac.v[2] - l20 * yc[0] - l21 * yc[1];
-    yc[3] = ac.v[3] - l30 * yc[0] - l31 * yc[1] - l32 * yc[2];
-    wc[0] = xr.v[0];
-    wc[1] = xr.v[1] - l10 * wc[0];
-    wc[2] = xr.v[2] - l20 * wc[0] - l21 * wc[1];
-    wc[3] = xr.v[3] - l30 * wc[0] - l31 * wc[1] - l32 * wc[2];
-    const double z0 = yi[0] * r0, z1 = yi[1] * r1, z2 = yi[2] * r2, z3 = yi[3] * r3;
-    if (tid == 0) { fs.d[j] = d0; fs.d[j + 1] = d1; fs.d[j + 2] = d2; fs.d[j + 3] = d3; }
-    const double a_upd = a - ((z0 * yc[0] + z1 * yc[1]) + (z2 * yc[2] + z3 * yc[3]));
-    const double x_upd = x - ((z0 * wc[0] + z1 * wc[1]) + (z2 * wc[2] + z3 * wc[3]));
-    const int cq = c - j, iq = i - j;
-    const double a_blk = cq == 0 ? yi[0] : (cq == 1 ? yi[1] : (cq == 2 ? yi[2] : yi[3]));
-    const double x_blk = iq == 0 ? wc[0] : (iq == 1 ? wc[1] : (iq == 2 ? wc[2] : wc[3]));
-    if (i >= c) a = (c >= j + 4) ? a_upd : (c >= j ? a_blk : a);
-    x = (i >= j + 4) ? x_upd : (i >= j ? x_blk : x);
-  }
-  __syncthreads();
-  const double dc = fs.d[c] > 0.0 ? fs.d[c] : 1.0, di = fs.d[i] > 0.0 ? fs.d[i] : 1.0;
-  const double sdc = sqrt(dc);
-  if (i >= c) wa[c0 + i][c0 + c] = (i > c) ? a / sdc : sdc;
-  wx[c0 + i][c0 + c] = (c <= i) ? x / sqrt(di) : 0.0;
-}
-
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // 16 x 16 x 16 block products of one wave on the matrix cores
@@ -210,66 +124,233 @@ __device__ __forceinline__ void blk_sub(double (*out)[LDT], int r, int c, dbl4 a
   for (int e = 0; e < 4; ++e) out[r + (l >> 4) + 4 * e][c + (l & 15)] -= acc[e];
 }
 
-// Factor a 64 x 64 SPD tile (lower triangle of src, row pitch lds) and invert
-// the factor: writes L (zero upper) to dstL unless it is null and L^-1 (zero
-// upper) to dstI; a non-positive pivot sets *fail.  The factorizations below
-// pass a null dstL: the solves and the inverse use L_kk^-1 for diagonal tiles
-// and never read L_kk.  wa / wx are two LDS tiles of scratch.
-// Right-looking blocked Cholesky with 16-wide panels p = 0..3:
-//   L_pp = chol(A_pp), M_p = L_pp^-1      (whole workgroup, diag16)
-//   L_rp = A_rp M_p^T                     (rows below the panel; wave w: block row p+1+w)
-//   A_ij -= L_ip L_jp^T                   (trailing lower triangle, blocks dealt to the waves)
-// then the off-diagonal blocks of L^-1 by block diagonals s = i - j = 1..3,
-// wave q forming block (q + s, q):
-//   T = sum_{k=j}^{i-1} L_ik X_kj,  X_ij = -M_i T
-// (T is staged in the unused upper block (j, i) of wa and read back by the
-// same wave).  Every block product is a 16 x 16 x 16 matrix-core product.
-// 256 threads.  tile_factor_inverse_lds: the tile already in wa (synchronised).
+// One wavefront factors the 16 x 16 diagonal block of wa at (c0, c0) (lower
+// triangle, SPD) in place into its Cholesky factor L_p (zero upper) and
+// writes L_p^-1 (zero upper) to the same block of wx.  Elimination of [A | I]
+// by 4 x 4 pivot blocks J: the block's four columns of A and four rows of X
+// are published through LDS; every lane forms P = L_P D_P L_P^T (unit lower
+// L_P, four reciprocals, the same in every lane) and by forward substitution
+//   y_i = A_iJ L_P^-T,  w_c = L_P^-1 X_J[:, c],  z_i = y_i D_P^-1,
+// then the trailing update A_ic -= z_i . y_c and X_ic -= z_i . w_c (rows
+// below the block) is one v_mfma_f64_16x16x4_f64 each, with the 4-wide k
+// dimension the pivot block's columns; the block's rows of X become w.  This
+// is four steps of LDL^T at once.  A and X live in the matrix-core result
+// layout: lane jj + 16 q, element e holds row q + 4 e, column jj.  At the
+// end L = Lt D^1/2 (written column by column as the blocks finish) and
+// L^-1 = D^-1/2 (Lt^-1).  A non-positive (or NaN) pivot sets bad and
+// continues with pivot 1.
+struct WaveScratch16 {
+  double pan[PB][4];  // A[i][p .. p+3]: the pivot block's columns
+  double xr[4][PB];   // X[p .. p+3][c]: the pivot block's rows of X
+};
+// a, b, c or d by q = 0..3 as selects; the operands are pinned to registers
+// first so that the compiler does not turn the choice into branches.
+__device__ __forceinline__ double sel4(int q, double a, double b, double c, double d) {
+  asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d));
+  double v = a;
+  v = (q == 1) ? b : v;
+  v = (q == 2) ? c : v;
+  v = (q == 3) ? d : v;
+  return v;
+}
+__device__ __forceinline__ void diag16_wave(double (*wa)[LDT], double (*wx)[LDT], int c0, WaveScratch16 &sc,
+                                            bool &bad) {
+  const int l = threadIdx.x & 63, jj = l & 15, q = l >> 4;
+  dbl4 accA, accX;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int i = q + 4 * e;
+    accA[e] = (i >= jj) ? wa[c0 + i][c0 + jj] : wa[c0 + jj][c0 + i];
+    accX[e] = (i == jj) ? 1.0 : 0.0;
+  }
+  double dv[PB];
+#pragma unroll
+  for (int J = 0; J < 4; ++J) {
+    const int p = 4 * J;
+    if (jj >= p && jj < p + 4) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) sc.pan[q + 4 * e][jj - p] = accA[e];
+    }
+    sc.xr[q][jj] = accX[J];
+    wave_sync();
+    const double p00 = sc.pan[p][0], p10 = sc.pan[p + 1][0], p11 = sc.pan[p + 1][1];
+    const double p20 = sc.pan[p + 2][0], p21 = sc.pan[p + 2][1], p22 = sc.pan[p + 2][2];
+    const double p30 = sc.pan[p + 3][0], p31 = sc.pan[p + 3][1], p32 = sc.pan[p + 3][2], p33 = sc.pan[p + 3][3];
+    double ai[4], xc[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      ai[r] = sc.pan[jj][r];
+      xc[r] = sc.xr[r][jj];
+    }
+    wave_sync();  // the next block's writes come after every lane's reads
+    // P = L_P D_P L_P^T (t_rs = l_rs d_s)
+    const double d0 = p00;
+    const double r0 = recip_nr(d0 > 0.0 ? d0 : 1.0);
+    const double l10 = p10 * r0, l20 = p20 * r0, l30 = p30 * r0;
+    const double d1 = p11 - l10 * p10;
+    const double r1 = recip_nr(d1 > 0.0 ? d1 : 1.0);
+    const double t21 = p21 - l20 * p10, t31 = p31 - l30 * p10;
+    const double l21 = t21 * r1, l31 = t31 * r1;
+    const double d2 = p22 - l20 * p20 - l21 * t21;
+    const double r2 = recip_nr(d2 > 0.0 ? d2 : 1.0);
+    const double t32 = p32 - l30 * p20 - l31 * t21;
+    const double l32 = t32 * r2;
+    const double d3 = p33 - l30 * p30 - l31 * t31 - l32 * t32;
+    const double r3 = recip_nr(d3 > 0.0 ? d3 : 1.0);
+    bad = bad || !(d0 > 0.0 && d1 > 0.0 && d2 > 0.0 && d3 > 0.0);
+    dv[p] = d0;
+    dv[p + 1] = d1;
+    dv[p + 2] = d2;
+    dv[p + 3] = d3;
+    double y[4], w[4];
+    y[0] = ai[0];
+    y[1] = ai[1] - l10 * y[0];
+    y[2] = ai[2] - l20 * y[0] - l21 * y[1];
+    y[3] = ai[3] - l30 * y[0] - l31 * y[1] - l32 * y[2];
+    w[0] = xc[0];
+    w[1] = xc[1] - l10 * w[0];
+    w[2] = xc[2] - l20 * w[0] - l21 * w[1];
+    w[3] = xc[3] - l30 * w[0] - l31 * w[1] - l32 * w[2];
+    // this lane's k = q entries: operand A[i = jj][k] = -z_jj[q], B[k][c = jj] = y_jj[q]
+    const double yk = sel4(q, y[0], y[1], y[2], y[3]);
+    const double wk = sel4(q, w[0], w[1], w[2], w[3]);
+    const double rk = sel4(q, r0, r1, r2, r3);
+    const double dk = sel4(q, d0, d1, d2, d3);
+    const bool below = jj >= p + 4;
+    const double zop = below ? -(yk * rk) : 0.0, yop = below ? yk : 0.0;
+    accA = __builtin_amdgcn_mfma_f64_16x16x4f64(zop, yop, accA, 0, 0, 0);
+    accX[J] = wk;  // rows p .. p+3 of X (element J of lane (c, q) is row p + q)
+    accX = __builtin_amdgcn_mfma_f64_16x16x4f64(zop, wk, accX, 0, 0, 0);
+    // column p + q of L: y / sqrt(d) on and below the diagonal
+    wa[c0 + jj][c0 + p + q] = (jj >= p + q) ? yk * rsqrt(dk > 0.0 ? dk : 1.0) : 0.0;
+  }
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int i = q + 4 * e;
+    const double di = sel4(q, dv[4 * e], dv[4 * e + 1], dv[4 * e + 2], dv[4 * e + 3]);
+    wx[c0 + i][c0 + jj] = (jj <= i) ? accX[e] * rsqrt(di > 0.0 ? di : 1.0) : 0.0;
+  }
+}
+
+// Factor a 64 x 64 SPD tile and invert the factor, in LDS: wa holds the tile
+// (lower 16 x 16 blocks valid, synchronised); with pa != null the tile is
+// first updated, T = wa - pa pa^T, fused into the schedule below.  Writes L
+// (zero upper) to dstL unless it is null and L^-1 (zero upper) to dstI; a
+// non-positive pivot sets *fail.  The factorizations pass a null dstL: the
+// solves and the inverse use L_kk^-1 for diagonal tiles and never read L_kk.
+// wa ends with L in its lower blocks, wx (scratch, must not alias wa or pa)
+// with L^-1 in its lower blocks.  256 threads.
+//
+// Right-looking blocked Cholesky with 16-wide panels p = 0..3 and blocked
+// inversion of the factor:
+//   L_pp = chol(T_pp), M_p = L_pp^-1        (wave 0, diag16_wave)
+//   L_rp = T_rp M_p^T                       (r > p)
+//   T_rc -= L_rp L_cp^T                     (p < c <= r)
+//   X_rc = -M_r S_rc, S_rc = sum_{k=c}^{r-1} L_rk X_kc   (X = L^-1, X_pp = M_p)
+// Wave 0 runs the chain of diagonal blocks (update of T_pp, then its factor);
+// waves 1-3 do every other block product (16 x 16 x 16 on the matrix cores)
+// in the stage where its operands are ready, so that between two diagonal
+// blocks wave 0 waits for one panel block only.  Stages (barriers between):
+//   S1   w0: T_00 update, diag(0)      w1-3: T updates of the other 9 blocks
+//   Sa1  w1-3: L_10 L_20 L_30
+//   Sb1  w0: T_11 -= L_10 L_10^T, diag(1)
+//        w1: T_21 T_31   w2: T_22 T_32   w3: T_33, S_10 = L_10 M_0
+//   Sa2  w1: L_21   w2: L_31   w3: X_10
+//   Sb2  w0: T_22 -= L_21 L_21^T, diag(2)
+//        w1: T_32, S_21 = L_21 M_1   w2: T_33   w3: S_20 = L_20 M_0 + L_21 X_10
+//   Sa3  w1: L_32   w2: X_21   w3: X_20
+//   Sb3  w0: T_33 -= L_32 L_32^T, diag(3)   w1-3: S_32, S_31, S_30
+//   end  w1-3: X_32, X_31, X_30
+// The S blocks are staged in the unused upper blocks (c, r) of wx.
 __device__ __forceinline__ void tile_factor_inverse_lds(double *dstL, int64_t ldl, double *dstI, int64_t ldi,
-                                                        int *fail, double (*wa)[LDT], double (*wx)[LDT],
-                                                        FactorScratch &fs) {
+                                                        int *fail, double (*wa)[LDT], const double (*pa)[LDT],
+                                                        double (*wx)[LDT], WaveScratch16 &ws) {
   const int tid = threadIdx.x, w = tid >> 6;
+  const dbl4 zero = dbl4{0.0, 0.0, 0.0, 0.0};
   bool bad = false;
+  auto sub_pp = [&](int r, int c) {  // T_rc -= sum_k P_rk P_ck^T
+    dbl4 acc = zero;
 #pragma unroll
-  for (int p = 0; p < 4; ++p) {
-    const int c0 = PB * p;
-    diag16(wa, wx, c0, fs, bad);
-    __syncthreads();
-    const int nu = 3 - p;  // 16-row blocks below the panel
-    if (nu > 0) {
-      // panel: L_rp = A_rp M_p^T, in place (the wave's operand reads precede its writes)
-      if (w < nu) {
-        const int r = c0 + PB * (1 + w);
-        const dbl4 acc = blk_nt(wa, r, c0, wx, c0, c0, dbl4{0.0, 0.0, 0.0, 0.0});
-        blk_put(wa, r, c0, acc, 1.0);
-      }
-      __syncthreads();
-      // trailing blocks (u, v), v <= u < nu, numbered row by row
-      const int nblk = nu * (nu + 1) / 2;
-      for (int bk = w; bk < nblk; bk += 4) {
-        const int u = bk >= 3 ? 2 : (bk >= 1 ? 1 : 0);
-        const int v = bk - u * (u + 1) / 2;
-        const int ri = c0 + PB * (1 + u), cj = c0 + PB * (1 + v);
-        const dbl4 acc = blk_nt(wa, ri, c0, wa, cj, c0, dbl4{0.0, 0.0, 0.0, 0.0});
-        blk_sub(wa, ri, cj, acc);
-      }
-      __syncthreads();
-    }
-  }
-  // off-diagonal blocks of X = L^-1, one block diagonal at a time
-#pragma unroll
-  for (int s = 1; s < 4; ++s) {
-    if (w + s < 4) {
-      const int bj = w, bi = w + s;
-      dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
-      for (int k = bj; k < bi; ++k) acc = blk_nn(wa, PB * bi, PB * k, wx, PB * k, PB * bj, acc);
-      blk_put(wa, PB * bj, PB * bi, acc, 1.0);  // upper block (bj, bi) is free
+    for (int kb = 0; kb < 4; ++kb) acc = blk_nt(pa, PB * r, PB * kb, pa, PB * c, PB * kb, acc);
+    blk_sub(wa, PB * r, PB * c, acc);
+  };
+  auto lpanel = [&](int r, int c) {  // L_rc = T_rc M_c^T, in place (reads precede the writes)
+    blk_put(wa, PB * r, PB * c, blk_nt(wa, PB * r, PB * c, wx, PB * c, PB * c, zero), 1.0);
+  };
+  auto sub_ll = [&](int r, int c, int k) {  // T_rc -= L_rk L_ck^T
+    blk_sub(wa, PB * r, PB * c, blk_nt(wa, PB * r, PB * k, wa, PB * c, PB * k, zero));
+  };
+  auto xblk = [&](int r, int c) {  // X_rc = -M_r S_rc (S_rc in upper block (c, r) of wx)
+    blk_put(wx, PB * r, PB * c, blk_nn(wx, PB * r, PB * r, wx, PB * c, PB * r, zero), -1.0);
+  };
+  // S1
+  if (w == 0) {
+    if (pa) {
+      sub_pp(0, 0);
       wave_sync();
-      const dbl4 xv = blk_nn(wx, PB * bi, PB * bi, wa, PB * bj, PB * bi, dbl4{0.0, 0.0, 0.0, 0.0});
-      blk_put(wx, PB * bi, PB * bj, xv, -1.0);
     }
-    __syncthreads();
+    diag16_wave(wa, wx, 0, ws, bad);
+  } else if (pa) {
+    if (w == 1) { sub_pp(1, 0); sub_pp(1, 1); sub_pp(2, 0); }
+    else if (w == 2) { sub_pp(2, 1); sub_pp(2, 2); sub_pp(3, 0); }
+    else { sub_pp(3, 1); sub_pp(3, 2); sub_pp(3, 3); }
   }
+  __syncthreads();
+  // Sa1
+  if (w >= 1) lpanel(w, 0);
+  __syncthreads();
+  // Sb1
+  if (w == 0) {
+    sub_ll(1, 1, 0);
+    wave_sync();
+    diag16_wave(wa, wx, PB, ws, bad);
+  } else if (w == 1) { sub_ll(2, 1, 0); sub_ll(3, 1, 0); }
+  else if (w == 2) { sub_ll(2, 2, 0); sub_ll(3, 2, 0); }
+  else {
+    sub_ll(3, 3, 0);
+    blk_put(wx, 0, PB, blk_nn(wa, PB, 0, wx, 0, 0, zero), 1.0);  // S_10
+  }
+  __syncthreads();
+  // Sa2
+  if (w == 1) lpanel(2, 1);
+  else if (w == 2) lpanel(3, 1);
+  else if (w == 3) xblk(1, 0);
+  __syncthreads();
+  // Sb2
+  if (w == 0) {
+    sub_ll(2, 2, 1);
+    wave_sync();
+    diag16_wave(wa, wx, 2 * PB, ws, bad);
+  } else if (w == 1) {
+    sub_ll(3, 2, 1);
+    blk_put(wx, PB, 2 * PB, blk_nn(wa, 2 * PB, PB, wx, PB, PB, zero), 1.0);  // S_21
+  } else if (w == 2) {
+    sub_ll(3, 3, 1);
+  } else {
+    dbl4 t = blk_nn(wa, 2 * PB, 0, wx, 0, 0, zero);
+    t = blk_nn(wa, 2 * PB, PB, wx, PB, 0, t);
+    blk_put(wx, 0, 2 * PB, t, 1.0);  // S_20
+  }
+  __syncthreads();
+  // Sa3
+  if (w == 1) lpanel(3, 2);
+  else if (w == 2) xblk(2, 1);
+  else if (w == 3) xblk(2, 0);
+  __syncthreads();
+  // Sb3
+  if (w == 0) {
+    sub_ll(3, 3, 2);
+    wave_sync();
+    diag16_wave(wa, wx, 3 * PB, ws, bad);
+  } else {
+    const int c = 3 - w;  // w1: S_32, w2: S_31, w3: S_30
+    dbl4 t = zero;
+    for (int k = c; k < 3; ++k) t = blk_nn(wa, 3 * PB, PB * k, wx, PB * k, PB * c, t);
+    blk_put(wx, PB * c, 3 * PB, t, 1.0);
+  }
+  __syncthreads();
+  if (w >= 1) xblk(3, 3 - w);
+  __syncthreads();
   if (bad && tid == 0) *fail = 1;
 #pragma unroll 4
   for (int q = 0; q < 16; ++q) {
@@ -281,10 +362,10 @@ __device__ __forceinline__ void tile_factor_inverse_lds(double *dstL, int64_t ld
 
 __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dstL, int64_t ldl,
                                                     double *dstI, int64_t ldi, int *fail, double (*wa)[LDT],
-                                                    double (*wx)[LDT], FactorScratch &fs) {
+                                                    double (*wx)[LDT], WaveScratch16 &ws) {
   load_tile(wa, src, lds, 0, 0);
   __syncthreads();
-  tile_factor_inverse_lds(dstL, ldl, dstI, ldi, fail, wa, wx, fs);
+  tile_factor_inverse_lds(dstL, ldl, dstI, ldi, fail, wa, nullptr, wx, ws);
 }
 
 // 64 x 64 x 64 products on the matrix cores (v_mfma_f64_16x16x4_f64).  Wave
@@ -385,27 +466,6 @@ __device__ __forceinline__ void tile_mul_nt_tril_lds(double (*out)[LDT], const d
   __syncthreads();
 }
 
-// c (LDS) -= a a^T on the ten lower 16 x 16 blocks (i >= j) only, dealt
-// 3 / 3 / 2 / 2 to the waves; the upper blocks of c are left as they are.
-// c must not alias a.
-__device__ __forceinline__ void tile_sub_aat_lower_lds(double (*c)[LDT], const double (*a)[LDT]) {
-  const int w = threadIdx.x >> 6;
-  // wave 0: (0,0) (1,0) (3,3); wave 1: (1,1) (2,0) (3,2); wave 2: (2,1) (3,0); wave 3: (2,2) (3,1)
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    int bi, bj;
-    if (q == 0) { bi = w == 0 ? 0 : (w == 1 ? 1 : 2); bj = w == 0 ? 0 : (w == 3 ? 2 : 1); }
-    else if (q == 1) { bi = w == 0 ? 1 : (w == 1 ? 2 : 3); bj = w == 3 ? 1 : 0; }
-    else { bi = w < 2 ? 3 : -1; bj = w == 0 ? 3 : 2; }
-    if (bi >= 0) {
-      dbl4 acc = dbl4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-      for (int kb = 0; kb < 4; ++kb) acc = blk_nt(a, PB * bi, PB * kb, a, PB * bj, PB * kb, acc);
-      blk_sub(c, PB * bi, PB * bj, acc);
-    }
-  }
-}
-
 // y = M v (transposed = false) or M^T v for a 64 x 64 global tile M with row
 // pitch ld and a 64-vector v in LDS; result to LDS out (256 threads).
 __device__ __forceinline__ void tile_gemv(const double *M, int64_t ld, const double *v, double *out, bool transposed,
@@ -441,7 +501,7 @@ __global__ void __launch_bounds__(256) k_chol_first(const WinInfo *wis, WinState
                                                     const double *S, double *Lf, double *Dinv) {
   __shared__ double wa[NB][LDT];
   __shared__ double wx[NB][LDT];
-  __shared__ FactorScratch fs;
+  __shared__ WaveScratch16 fs;
   __shared__ int fail;
   const int w = large_list[blockIdx.y];
   const WinInfo wi = wis[w];
@@ -464,7 +524,7 @@ __global__ void __launch_bounds__(256) k_chol_step(const WinInfo *wis, WinState 
   __shared__ double pb[NB][LDT];
   __shared__ double vsh[3][NB];
   __shared__ double red[4][NB];
-  __shared__ FactorScratch fs;
+  __shared__ WaveScratch16 fs;
   __shared__ int fail;
   const int w = large_list[blockIdx.y];
   const WinInfo wi = wis[w];
@@ -515,11 +575,12 @@ __global__ void __launch_bounds__(256) k_chol_step(const WinInfo *wis, WinState 
   if (jb != ib) tile_mul_nt_tril_lds(pb, pb, li);       // L_jk
   if (jb == k + 1) store_tile(pa, L, ld, ib * NB, k * NB);
   if (look) {
-    tile_sub_aat_lower_lds(pb, pa);
+    // T = A_{k+1,k+1} - L_{k+1,k} L_{k+1,k}^T is fused into the factorization;
+    // li (read only by the product above) becomes its scratch
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
     tile_factor_inverse_lds(nullptr, ld, Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB,
-                            &fail, pb, pa, fs);
+                            &fail, pb, pa, li, fs);
     __syncthreads();
     if (threadIdx.x == 0 && fail) s.chol_fail = 1;
   } else {
@@ -650,7 +711,7 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
   __shared__ double c[NB][LDT];
   __shared__ double vsh[3][NB];
   __shared__ double red[4][NB];
-  __shared__ FactorScratch fs;
+  __shared__ WaveScratch16 fs;
   __shared__ int fail;
   const int w = win_list[blockIdx.x];
   const WinInfo wi = wis[w];

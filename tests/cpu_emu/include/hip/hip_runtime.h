@@ -74,6 +74,7 @@ using std::sqrt;
 inline int min(int a, int b) { return a < b ? a : b; }
 inline int atomicAdd(int *p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
 inline int max(int a, int b) { return a > b ? a : b; }
+inline double rsqrt(double x) { return 1.0 / std::sqrt(x); }
 inline double __builtin_amdgcn_rcp(double x) { return 1.0 / x; }
 inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
 // v_mfma_f64_16x16x4_f64: A[i][k] from lane i + 16 k, B[k][j] from lane
