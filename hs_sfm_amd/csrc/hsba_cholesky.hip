@@ -125,23 +125,25 @@ __device__ __forceinline__ void blk_sub(double (*out)[LDT], int r, int c, dbl4 a
 }
 
 // One wavefront factors the 16 x 16 diagonal block of wa at (c0, c0) (lower
-// triangle, SPD) in place into its Cholesky factor L_p (zero upper) and
-// writes L_p^-1 (zero upper) to the same block of wx.  Elimination of [A | I]
-// by 4 x 4 pivot blocks J: the block's four columns of A and four rows of X
-// are published through LDS; every lane forms P = L_P D_P L_P^T (unit lower
-// L_P, four reciprocals, the same in every lane) and by forward substitution
-//   y_i = A_iJ L_P^-T,  w_c = L_P^-1 X_J[:, c],  z_i = y_i D_P^-1,
-// then the trailing update A_ic -= z_i . y_c and X_ic -= z_i . w_c (rows
-// below the block) is one v_mfma_f64_16x16x4_f64 each, with the 4-wide k
-// dimension the pivot block's columns; the block's rows of X become w.  This
-// is four steps of LDL^T at once.  A and X live in the matrix-core result
-// layout: lane jj + 16 q, element e holds row q + 4 e, column jj.  At the
-// end L = Lt D^1/2 (written column by column as the blocks finish) and
-// L^-1 = D^-1/2 (Lt^-1).  A non-positive (or NaN) pivot sets bad and
-// continues with pivot 1.
+// triangle, SPD) and writes L_p^-1 (zero upper) to the same block of wx; the
+// block of wa is only read (nothing reads the diagonal factor blocks L_p).
+// A (symmetric, both triangles) and X (starting at I) live in matrix-core
+// result layout: lane jj + 16 q, element e holds row q + 4 e, column jj.
+// Elimination by 4 x 4 pivot blocks J (rows / columns p .. p+3): the block's
+// four columns of A are published through LDS, and every lane forms
+// P = L_P D_P L_P^T (unit lower L_P, four reciprocals, the same in every
+// lane), L_P^-1 and, for its row i, y_i = A_iJ L_P^-T, z_i = y_i D_P^-1 and
+// G_i = z_i L_P^-1.  Then one v_mfma_f64_16x16x4_f64 per matrix applies
+//   A_i -= G_i A_J           (rows i below the block: A_ic -= z_i . y_c)
+//   X_i -= G_i X_J           (rows below the block)
+//   X_J += (L_P^-1 - I) X_J  (the block's rows: X_J <- L_P^-1 X_J)
+// with the product's k = 4 dimension the pivot block's rows A_J, X_J as they
+// stand, which every lane holds as its own accumulator element J.  This is
+// four steps of LDL^T at once; columns left of the block are not kept.  At
+// the end X = Lt^-1 (A = Lt D Lt^T, Lt unit lower) and L^-1 = D^-1/2 X.  A
+// non-positive (or NaN) pivot sets bad and continues with pivot 1.
 struct WaveScratch16 {
   double pan[PB][4];  // A[i][p .. p+3]: the pivot block's columns
-  double xr[4][PB];   // X[p .. p+3][c]: the pivot block's rows of X
 };
 // a, b, c or d by q = 0..3 as selects; the operands are pinned to registers
 // first so that the compiler does not turn the choice into branches.
@@ -171,17 +173,13 @@ __device__ __forceinline__ void diag16_wave(double (*wa)[LDT], double (*wx)[LDT]
 #pragma unroll
       for (int e = 0; e < 4; ++e) sc.pan[q + 4 * e][jj - p] = accA[e];
     }
-    sc.xr[q][jj] = accX[J];
     wave_sync();
     const double p00 = sc.pan[p][0], p10 = sc.pan[p + 1][0], p11 = sc.pan[p + 1][1];
     const double p20 = sc.pan[p + 2][0], p21 = sc.pan[p + 2][1], p22 = sc.pan[p + 2][2];
     const double p30 = sc.pan[p + 3][0], p31 = sc.pan[p + 3][1], p32 = sc.pan[p + 3][2], p33 = sc.pan[p + 3][3];
-    double ai[4], xc[4];
+    double ai[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      ai[r] = sc.pan[jj][r];
-      xc[r] = sc.xr[r][jj];
-    }
+    for (int r = 0; r < 4; ++r) ai[r] = sc.pan[jj][r];
     wave_sync();  // the next block's writes come after every lane's reads
     // P = L_P D_P L_P^T (t_rs = l_rs d_s)
     const double d0 = p00;
@@ -202,27 +200,38 @@ __device__ __forceinline__ void diag16_wave(double (*wa)[LDT], double (*wx)[LDT]
     dv[p + 1] = d1;
     dv[p + 2] = d2;
     dv[p + 3] = d3;
-    double y[4], w[4];
+    // M = L_P^-1 (unit lower)
+    const double m10 = -l10, m21 = -l21, m32 = -l32;
+    const double m20 = -(l20 + l21 * m10), m31 = -(l31 + l32 * m21);
+    const double m30 = -(l30 + l31 * m10 + l32 * m20);
+    // row jj: y = a L_P^-T, z = y D_P^-1, G = z M
+    double y[4];
     y[0] = ai[0];
     y[1] = ai[1] - l10 * y[0];
     y[2] = ai[2] - l20 * y[0] - l21 * y[1];
     y[3] = ai[3] - l30 * y[0] - l31 * y[1] - l32 * y[2];
-    w[0] = xc[0];
-    w[1] = xc[1] - l10 * w[0];
-    w[2] = xc[2] - l20 * w[0] - l21 * w[1];
-    w[3] = xc[3] - l30 * w[0] - l31 * w[1] - l32 * w[2];
-    // this lane's k = q entries: operand A[i = jj][k] = -z_jj[q], B[k][c = jj] = y_jj[q]
-    const double yk = sel4(q, y[0], y[1], y[2], y[3]);
-    const double wk = sel4(q, w[0], w[1], w[2], w[3]);
-    const double rk = sel4(q, r0, r1, r2, r3);
-    const double dk = sel4(q, d0, d1, d2, d3);
+    const double z0 = y[0] * r0, z1 = y[1] * r1, z2 = y[2] * r2, z3 = y[3] * r3;
+    const double g3 = z3;
+    const double g2 = z2 + z3 * m32;
+    const double g1 = z1 + z2 * m21 + z3 * m31;
+    const double g0 = z0 + z1 * m10 + z2 * m20 + z3 * m30;
+    const double gk = sel4(q, g0, g1, g2, g3);  // this lane's k = q entry of G
     const bool below = jj >= p + 4;
-    const double zop = below ? -(yk * rk) : 0.0, yop = below ? yk : 0.0;
-    accA = __builtin_amdgcn_mfma_f64_16x16x4f64(zop, yop, accA, 0, 0, 0);
-    accX[J] = wk;  // rows p .. p+3 of X (element J of lane (c, q) is row p + q)
-    accX = __builtin_amdgcn_mfma_f64_16x16x4f64(zop, wk, accX, 0, 0, 0);
-    // column p + q of L: y / sqrt(d) on and below the diagonal
-    wa[c0 + jj][c0 + p + q] = (jj >= p + q) ? yk * rsqrt(dk > 0.0 ? dk : 1.0) : 0.0;
+    // the block's rows: (M - I) at (jj - p, q)
+    const int rr = jj - p;
+    double hb = 0.0;
+    hb = (rr == 1 && q == 0) ? m10 : hb;
+    hb = (rr == 2 && q == 0) ? m20 : hb;
+    hb = (rr == 2 && q == 1) ? m21 : hb;
+    hb = (rr == 3 && q == 0) ? m30 : hb;
+    hb = (rr == 3 && q == 1) ? m31 : hb;
+    hb = (rr == 3 && q == 2) ? m32 : hb;
+    // operand A[i = jj][k = q]; operand B[k = q][c = jj] = row p + q of A or X, element J of this lane
+    const double opA = below ? -gk : 0.0;
+    const double opX = below ? -gk : hb;
+    const double bA = accA[J], bX = accX[J];
+    accA = __builtin_amdgcn_mfma_f64_16x16x4f64(opA, bA, accA, 0, 0, 0);
+    accX = __builtin_amdgcn_mfma_f64_16x16x4f64(opX, bX, accX, 0, 0, 0);
   }
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -234,12 +243,12 @@ __device__ __forceinline__ void diag16_wave(double (*wa)[LDT], double (*wx)[LDT]
 
 // Factor a 64 x 64 SPD tile and invert the factor, in LDS: wa holds the tile
 // (lower 16 x 16 blocks valid, synchronised); with pa != null the tile is
-// first updated, T = wa - pa pa^T, fused into the schedule below.  Writes L
-// (zero upper) to dstL unless it is null and L^-1 (zero upper) to dstI; a
-// non-positive pivot sets *fail.  The factorizations pass a null dstL: the
-// solves and the inverse use L_kk^-1 for diagonal tiles and never read L_kk.
-// wa ends with L in its lower blocks, wx (scratch, must not alias wa or pa)
-// with L^-1 in its lower blocks.  256 threads.
+// first updated, T = wa - pa pa^T, fused into the schedule below.  Writes
+// L^-1 (zero upper) to dst; a non-positive pivot sets *fail.  Only L^-1 is
+// kept: the solves and the inverse use L_kk^-1 for diagonal tiles and never
+// read L_kk.  wa ends with the off-diagonal blocks of L (its diagonal
+// blocks hold the updated T_pp), wx (scratch, must not alias wa or pa) with
+// L^-1 in its lower blocks.  256 threads.
 //
 // Right-looking blocked Cholesky with 16-wide panels p = 0..3 and blocked
 // inversion of the factor:
@@ -262,9 +271,9 @@ __device__ __forceinline__ void diag16_wave(double (*wa)[LDT], double (*wx)[LDT]
 //   Sb3  w0: T_33 -= L_32 L_32^T, diag(3)   w1-3: S_32, S_31, S_30
 //   end  w1-3: X_32, X_31, X_30
 // The S blocks are staged in the unused upper blocks (c, r) of wx.
-__device__ __forceinline__ void tile_factor_inverse_lds(double *dstL, int64_t ldl, double *dstI, int64_t ldi,
-                                                        int *fail, double (*wa)[LDT], const double (*pa)[LDT],
-                                                        double (*wx)[LDT], WaveScratch16 &ws) {
+__device__ __forceinline__ void tile_factor_inverse_lds(double *dst, int64_t ld, int *fail, double (*wa)[LDT],
+                                                        const double (*pa)[LDT], double (*wx)[LDT],
+                                                        WaveScratch16 &ws) {
   const int tid = threadIdx.x, w = tid >> 6;
   const dbl4 zero = dbl4{0.0, 0.0, 0.0, 0.0};
   bool bad = false;
@@ -355,17 +364,16 @@ __device__ __forceinline__ void tile_factor_inverse_lds(double *dstL, int64_t ld
 #pragma unroll 4
   for (int q = 0; q < 16; ++q) {
     const int e = tid + 256 * q, i = e >> 6, j = e & 63;
-    if (dstL != nullptr) dstL[(int64_t)i * ldl + j] = (j <= i) ? wa[i][j] : 0.0;
-    dstI[(int64_t)i * ldi + j] = (j <= i) ? wx[i][j] : 0.0;
+    dst[(int64_t)i * ld + j] = (j <= i) ? wx[i][j] : 0.0;
   }
 }
 
-__device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dstL, int64_t ldl,
-                                                    double *dstI, int64_t ldi, int *fail, double (*wa)[LDT],
-                                                    double (*wx)[LDT], WaveScratch16 &ws) {
+__device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dst, int64_t ld,
+                                                    int *fail, double (*wa)[LDT], double (*wx)[LDT],
+                                                    WaveScratch16 &ws) {
   load_tile(wa, src, lds, 0, 0);
   __syncthreads();
-  tile_factor_inverse_lds(dstL, ldl, dstI, ldi, fail, wa, nullptr, wx, ws);
+  tile_factor_inverse_lds(dst, ld, fail, wa, nullptr, wx, ws);
 }
 
 // 64 x 64 x 64 products on the matrix cores (v_mfma_f64_16x16x4_f64).  Wave
@@ -509,7 +517,7 @@ __global__ void __launch_bounds__(256) k_chol_first(const WinInfo *wis, WinState
   if (!chol_active(s)) return;
   if (threadIdx.x == 0) fail = 0;
   __syncthreads();
-  tile_factor_inverse(S + wi.s_off, wi.ld, nullptr, wi.ld, Dinv + wi.dinv_off, NB, &fail, wa, wx, fs);
+  tile_factor_inverse(S + wi.s_off, wi.ld, Dinv + wi.dinv_off, NB, &fail, wa, wx, fs);
   __syncthreads();
   if (threadIdx.x == 0 && fail) s.chol_fail = 1;
 }
@@ -579,8 +587,7 @@ __global__ void __launch_bounds__(256) k_chol_step(const WinInfo *wis, WinState 
     // li (read only by the product above) becomes its scratch
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
-    tile_factor_inverse_lds(nullptr, ld, Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB,
-                            &fail, pb, pa, li, fs);
+    tile_factor_inverse_lds(Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB, &fail, pb, pa, li, fs);
     __syncthreads();
     if (threadIdx.x == 0 && fail) s.chol_fail = 1;
   } else {
@@ -724,8 +731,7 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
   for (int kb = 0; kb < wi.nb; ++kb) {
     __syncthreads();
     __threadfence_block();
-    tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, nullptr, ld,
-                        Dw + (int64_t)kb * NB * NB, NB, &fail, a, c, fs);
+    tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, Dw + (int64_t)kb * NB * NB, NB, &fail, a, c, fs);
     __threadfence_block();
     __syncthreads();
     load_tile(c, Dw + (int64_t)kb * NB * NB, NB, 0, 0);
