@@ -499,34 +499,35 @@ __device__ __forceinline__ bool chol_active(const WinState &s) { return s.active
 
 // ---------------------------------------------------------------- large path
 // Every large-path launch covers all large windows of the batch: the window
-// of a workgroup is large_list[blockIdx.y], and each window runs exactly the
-// work (and order of work) it runs alone, so its results do not depend on
-// the batch.  Windows with fewer tiles finish their steps earlier; their
+// of a workgroup is large window blockIdx.y (its CholWin record), and each
+// window runs exactly the work (and order of work) it runs alone, so its
+// results do not depend on the batch.  Windows with fewer tiles finish their steps earlier; their
 // workgroups of later launches return at once.
 
 // Diagonal tile 0 (before the first step).
-__global__ void __launch_bounds__(256) k_chol_first(const WinInfo *wis, WinState *ws, const int *large_list,
-                                                    const double *S, double *Lf, double *Dinv) {
+__global__ void __launch_bounds__(256) k_chol_first(const CholWin *cws, WinState *ws, const double *S, double *Dinv) {
   __shared__ double wa[NB][LDT];
   __shared__ double wx[NB][LDT];
   __shared__ WaveScratch16 fs;
   __shared__ int fail;
-  const int w = large_list[blockIdx.y];
-  const WinInfo wi = wis[w];
-  WinState &s = ws[w];
+  const CholWin cw = cws[blockIdx.y];
+  WinState &s = ws[cw.win];
+  double v[16];
+  fetch_tile(v, S + cw.s_off, cw.ld, 0, 0);  // in flight while the window's state is read
   if (!chol_active(s)) return;
   if (threadIdx.x == 0) fail = 0;
+  put_tile(wa, v);
   __syncthreads();
-  tile_factor_inverse(S + wi.s_off, wi.ld, Dinv + wi.dinv_off, NB, &fail, wa, wx, fs);
+  tile_factor_inverse_lds(Dinv + cw.dinv_off, NB, &fail, wa, nullptr, wx, fs);
   __syncthreads();
   if (threadIdx.x == 0 && fail) s.chol_fail = 1;
 }
 
-// Step k of window large_list[blockIdx.y]: trailing tiles (i, j),
+// Step k of large window blockIdx.y: trailing tiles (i, j),
 // k < j <= i < nb, in row order, then the right-hand-side workgroups
 // j = k+1 .. nb-1.
-__global__ void __launch_bounds__(256) k_chol_step(const WinInfo *wis, WinState *ws, const int *large_list,
-                                                   double *S, double *Lf, double *Dinv, double *b, double *y, int k) {
+__global__ void __launch_bounds__(256) k_chol_step(const CholWin *cws, WinState *ws, double *S, double *Lf,
+                                                   double *Dinv, double *b, double *y, int k) {
   __shared__ double li[NB][LDT];
   __shared__ double pa[NB][LDT];
   __shared__ double pb[NB][LDT];
@@ -534,18 +535,17 @@ __global__ void __launch_bounds__(256) k_chol_step(const WinInfo *wis, WinState 
   __shared__ double red[4][NB];
   __shared__ WaveScratch16 fs;
   __shared__ int fail;
-  const int w = large_list[blockIdx.y];
-  const WinInfo wi = wis[w];
+  const CholWin wi = cws[blockIdx.y];
   if (k + 1 >= wi.nb) return;  // this window's factorization is complete
   const int rest = wi.nb - k - 1, ntiles = rest * (rest + 1) / 2;
   if ((int)blockIdx.x >= ntiles + rest) return;
-  WinState &s = ws[w];
-  if (!chol_active(s)) return;
+  WinState &s = ws[wi.win];
   const int64_t ld = wi.ld;
   double *A = S + wi.s_off;
   double *L = Lf + wi.s_off;
   const double *Li = Dinv + wi.dinv_off + (int64_t)k * NB * NB;
   if (blockIdx.x >= ntiles) {
+    if (!chol_active(s)) return;
     // right-hand side: y_k = Linv_k b_k, w = Linv_k^T y_k, b_j -= A_jk w
     const int jb = k + 1 + (blockIdx.x - ntiles);
     for (int e = threadIdx.x; e < NB; e += blockDim.x) vsh[0][e] = b[wi.r_off + k * NB + e];
@@ -567,13 +567,15 @@ __global__ void __launch_bounds__(256) k_chol_step(const WinInfo *wis, WinState 
   // and factored in LDS by its workgroup
   const bool look = ib == k + 1 && jb == k + 1;
   {
-    // all three tiles' loads in flight together (the look-ahead workgroup's
-    // third tile is the diagonal tile it updates)
+    // all three tiles' loads in flight together, and with the read of the
+    // window's state (the look-ahead workgroup's third tile is the diagonal
+    // tile it updates)
     double v0[16], v1[16], v2[16];
     fetch_tile(v0, Li, NB, 0, 0);
     fetch_tile(v1, A, ld, ib * NB, k * NB);
     if (jb != ib) fetch_tile(v2, A, ld, jb * NB, k * NB);
     else if (look) fetch_tile(v2, A, ld, ib * NB, jb * NB);
+    if (!chol_active(s)) return;
     put_tile(li, v0);
     put_tile(pa, v1);
     if (jb != ib || look) put_tile(pb, v2);
@@ -632,21 +634,18 @@ __device__ __forceinline__ double reduce16(double part, double (*red)[NB]) {
 // with every tile of the chain fetched into registers before the first
 // product (the chain is then bound by its reductions, not by memory latency);
 // workgroup ib < k_lo then applies y_ib -= sum_{m = k_lo..k_hi} L_m,ib^T x_m.
-__global__ void __launch_bounds__(BT) k_chol_back(const WinInfo *wis, WinState *ws, const int *large_list,
-                                                  const double *Lf, const double *Dinv, const double *b, double *y,
-                                                  double *x, int q) {
+__global__ void __launch_bounds__(BT) k_chol_back(const CholWin *cws, WinState *ws, const double *Lf,
+                                                  const double *Dinv, const double *b, double *y, double *x, int q) {
   __shared__ double xs[4][NB];
   __shared__ double vs[NB];
   __shared__ double red[16][NB];
-  const int w = large_list[blockIdx.y];
-  const WinInfo wi = wis[w];
+  const CholWin wi = cws[blockIdx.y];
   // launch q solves block rows nb - 1 - 4 q down to (at most) nb - 4 - 4 q
   const int k_hi = wi.nb - 1 - 4 * q;
   if (k_hi < 0) return;
   const int k_lo = k_hi >= 3 ? k_hi - 3 : 0;
   if ((int)blockIdx.x >= (k_lo > 0 ? k_lo : 1)) return;
-  const WinState &s = ws[w];
-  if (!chol_active(s)) return;
+  const WinState &s = ws[wi.win];
   const int64_t ld = wi.ld;
   const double *L = Lf + wi.s_off;
   const double *Dw = Dinv + wi.dinv_off;
@@ -668,6 +667,7 @@ __global__ void __launch_bounds__(BT) k_chol_back(const WinInfo *wis, WinState *
   for (int a = 0; a < 4; ++a)
     yv[a] = (t < NB && a < nk) ? ((last && a == 0) ? b[wi.r_off + k_hi * NB + t] : y[wi.r_off + (k_hi - a) * NB + t])
                                : 0.0;
+  if (!chol_active(s)) return;  // read while the tiles above are in flight
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     if (a < nk) {
@@ -923,10 +923,9 @@ __global__ void k_pad_identity(const WinInfo *wis, int nwin, double *S) {
 }
 
 // b = rhs with zero padding (large windows: the step kernels update b in place).
-__global__ void k_rhs_copy(const WinInfo *wis, WinState *ws, const int *large_list, const double *rhs, double *b) {
-  const int w = large_list[blockIdx.y];
-  const WinInfo wi = wis[w];
-  const WinState &s = ws[w];
+__global__ void k_rhs_copy(const CholWin *cws, WinState *ws, const double *rhs, double *b) {
+  const CholWin wi = cws[blockIdx.y];
+  const WinState &s = ws[wi.win];
   if (!chol_active(s)) return;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wi.ld; i += gridDim.x * blockDim.x)
     b[wi.r_off + i] = i < wi.n ? rhs[wi.r_off + i] : 0.0;
@@ -948,18 +947,17 @@ void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int
     int max_nb = 0;
     for (int q = 0; q < num_large; ++q) max_nb = std::max(max_nb, plan.wi_host[large_host[q]].nb);
     const dim3 win_y(1, num_large);
-    hipLaunchKernelGGL(k_rhs_copy, dim3(8, num_large), dim3(256), 0, st, plan.wi, plan.ws, plan.large_list,
-                       plan.rhs, plan.bwork);
-    hipLaunchKernelGGL(k_chol_first, win_y, dim3(256), 0, st, plan.wi, plan.ws, plan.large_list, plan.S, plan.Lf,
-                       plan.Dinv);
+    hipLaunchKernelGGL(k_rhs_copy, dim3(8, num_large), dim3(256), 0, st, plan.large_wins, plan.ws, plan.rhs,
+                       plan.bwork);
+    hipLaunchKernelGGL(k_chol_first, win_y, dim3(256), 0, st, plan.large_wins, plan.ws, plan.S, plan.Dinv);
     for (int k = 0; k + 1 < max_nb; ++k) {
       int slots = 0;
       for (int q = 0; q < num_large; ++q) {
         const int rest = plan.wi_host[large_host[q]].nb - k - 1;
         if (rest > 0) slots = std::max(slots, rest * (rest + 1) / 2 + rest);
       }
-      hipLaunchKernelGGL(k_chol_step, dim3(slots, num_large), dim3(256), 0, st, plan.wi, plan.ws, plan.large_list,
-                         plan.S, plan.Lf, plan.Dinv, plan.bwork, plan.ywork, k);
+      hipLaunchKernelGGL(k_chol_step, dim3(slots, num_large), dim3(256), 0, st, plan.large_wins, plan.ws, plan.S,
+                         plan.Lf, plan.Dinv, plan.bwork, plan.ywork, k);
     }
     for (int q = 0; 4 * q < max_nb; ++q) {
       int blocks = 1;
@@ -967,8 +965,8 @@ void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int
         const int k_hi = plan.wi_host[large_host[u]].nb - 1 - 4 * q;
         if (k_hi >= 4) blocks = std::max(blocks, k_hi - 3);
       }
-      hipLaunchKernelGGL(k_chol_back, dim3(blocks, num_large), dim3(BT), 0, st, plan.wi, plan.ws, plan.large_list,
-                         plan.Lf, plan.Dinv, plan.bwork, plan.ywork, plan.dred, q);
+      hipLaunchKernelGGL(k_chol_back, dim3(blocks, num_large), dim3(BT), 0, st, plan.large_wins, plan.ws, plan.Lf,
+                         plan.Dinv, plan.bwork, plan.ywork, plan.dred, q);
     }
   }
 }

@@ -83,7 +83,8 @@ struct Engine final : EngineBase {
   int graph_iters = 0;
   static constexpr int kMaxGraphIterations = 64;
   int *small_list_dev = nullptr;
-  int *large_list_dev = nullptr;
+  CholWin *large_wins_dev = nullptr;
+  std::vector<CholWin> large_wins_host;  // source of the upload, kept while it is in flight
   CholPlan plan;
   std::vector<WinState> ws_host;
   bool built = false;  // a linearization is resident
@@ -208,7 +209,12 @@ struct Engine final : EngineBase {
       d.ich_img = ich_img; d.ich_begin = ich_begin; d.ich_end = ich_end; d.img_ch_ptr = img_ch_ptr;
     }
     A(small_list_dev, (int64_t)L.small_list.size());
-    A(large_list_dev, (int64_t)L.large_list.size());
+    A(large_wins_dev, (int64_t)L.large_list.size());
+    large_wins_host.clear();
+    for (int w : L.large_list) {
+      const WinInfo &q = L.wi[w];
+      large_wins_host.push_back(CholWin{q.s_off, q.dinv_off, q.ld, q.nb, q.n, q.r_off, w, 0});
+    }
 #undef A
     if ((rc = upload(key_img, L.key_img, st)) || (rc = upload(key_pt, L.key_pt, st)) ||
         (rc = upload(key_orig, L.key_orig, st)) || (rc = upload(pt_ptr, L.pt_ptr, st)) ||
@@ -224,7 +230,7 @@ struct Engine final : EngineBase {
         (rc = upload(cons, L.cons, st)) || (rc = upload(ck, L.chunk_kind, st)) ||
         (rc = upload(cbg, L.chunk_begin, st)) || (rc = upload(cen, L.chunk_end, st)) ||
         (rc = upload(cw, L.chunk_win, st)) || (rc = upload(wi, L.wi, st)) ||
-        (rc = upload(small_list_dev, L.small_list, st)) || (rc = upload(large_list_dev, L.large_list, st)))
+        (rc = upload(small_list_dev, L.small_list, st)) || (rc = upload(large_wins_dev, large_wins_host, st)))
       return rc;
     d.key_img = key_img; d.key_pt = key_pt; d.key_orig = key_orig; d.key_uv = key_uv; d.key_cov = key_cov;
     d.pt_ptr = pt_ptr; d.pt_win = pt_win;
@@ -247,7 +253,7 @@ struct Engine final : EngineBase {
       if ((rc = upload(d.cam_full, full, st)) || (rc = upload(d.cam_full_new, full, st))) return rc;
     }
     plan.nwin = L.nwin; plan.wi = wi; plan.ws = d.ws; plan.S = d.S; plan.rhs = d.rhs; plan.dred = d.dred;
-    plan.Dinv = d.Dinv; plan.wi_host = L.wi.data(); plan.large_list = large_list_dev;
+    plan.Dinv = d.Dinv; plan.wi_host = L.wi.data(); plan.large_wins = large_wins_dev;
     plan.Lf = nullptr; plan.bwork = nullptr; plan.ywork = nullptr;
     {
       int64_t large_s = 0;

@@ -55,6 +55,14 @@ struct KernelSet {
 template <typename T>
 KernelSet<T> *make_kernel_set(int img, int dcam);
 
+// What the large-path Cholesky launches need of one large window, indexed
+// by blockIdx.y: one 40-byte record instead of the window id and then its
+// WinInfo (two dependent loads at the head of every launch).
+struct CholWin {
+  int64_t s_off, dinv_off;
+  int32_t ld, nb, n, r_off, win, pad_;
+};
+
 // Dense FP64 Cholesky of every window's reduced system (hsba_cholesky.hip).
 struct CholPlan {
   int nwin;
@@ -64,7 +72,7 @@ struct CholPlan {
   double *Lf;                // factor of large windows (same layout as S)
   double *bwork, *ywork;     // right-hand side being reduced / forward solution
   const WinInfo *wi_host;    // host copy (large-window launch loop)
-  const int *large_list;     // device copy of the large windows' ids
+  const CholWin *large_wins; // device records of the large windows, in large-list order
 };
 void chol_init_padding(const CholPlan &plan, hipStream_t st);
 void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int num_small,
