@@ -271,9 +271,15 @@ __device__ __forceinline__ void diag16_wave(double (*wa)[LDT], double (*wx)[LDT]
 //   Sb3  w0: T_33 -= L_32 L_32^T, diag(3)   w1-3: S_32, S_31, S_30
 //   end  w1-3: X_32, X_31, X_30
 // The S blocks are staged in the unused upper blocks (c, r) of wx.
+//
+// nreal = the tile's rows that belong to the system (the rest is padding,
+// identity on the diagonal and zero elsewhere, and stays so under every
+// update).  A tile with at most 16 real rows (the last tile of a system whose
+// size leaves up to 16 rows in it) has L^-1 = [M_0 0; 0 I]: only block 0 is
+// updated and factored.
 __device__ __forceinline__ void tile_factor_inverse_lds(double *dst, int64_t ld, int *fail, double (*wa)[LDT],
                                                         const double (*pa)[LDT], double (*wx)[LDT],
-                                                        WaveScratch16 &ws) {
+                                                        WaveScratch16 &ws, int nreal) {
   const int tid = threadIdx.x, w = tid >> 6;
   const dbl4 zero = dbl4{0.0, 0.0, 0.0, 0.0};
   bool bad = false;
@@ -283,6 +289,23 @@ __device__ __forceinline__ void tile_factor_inverse_lds(double *dst, int64_t ld,
     for (int kb = 0; kb < 4; ++kb) acc = blk_nt(pa, PB * r, PB * kb, pa, PB * c, PB * kb, acc);
     blk_sub(wa, PB * r, PB * c, acc);
   };
+  if (nreal <= PB) {
+    if (w == 0) {
+      if (pa) {
+        sub_pp(0, 0);
+        wave_sync();
+      }
+      diag16_wave(wa, wx, 0, ws, bad);
+    }
+    __syncthreads();
+    if (bad && tid == 0) *fail = 1;
+#pragma unroll 4
+    for (int q = 0; q < 16; ++q) {
+      const int e = tid + 256 * q, i = e >> 6, j = e & 63;
+      dst[(int64_t)i * ld + j] = (i < PB) ? ((j <= i) ? wx[i][j] : 0.0) : (i == j ? 1.0 : 0.0);
+    }
+    return;
+  }
   auto lpanel = [&](int r, int c) {  // L_rc = T_rc M_c^T, in place (reads precede the writes)
     blk_put(wa, PB * r, PB * c, blk_nt(wa, PB * r, PB * c, wx, PB * c, PB * c, zero), 1.0);
   };
@@ -370,10 +393,10 @@ __device__ __forceinline__ void tile_factor_inverse_lds(double *dst, int64_t ld,
 
 __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t lds, double *dst, int64_t ld,
                                                     int *fail, double (*wa)[LDT], double (*wx)[LDT],
-                                                    WaveScratch16 &ws) {
+                                                    WaveScratch16 &ws, int nreal) {
   load_tile(wa, src, lds, 0, 0);
   __syncthreads();
-  tile_factor_inverse_lds(dst, ld, fail, wa, nullptr, wx, ws);
+  tile_factor_inverse_lds(dst, ld, fail, wa, nullptr, wx, ws, nreal);
 }
 
 // 64 x 64 x 64 products on the matrix cores (v_mfma_f64_16x16x4_f64).  Wave
@@ -518,7 +541,7 @@ __global__ void __launch_bounds__(256) k_chol_first(const CholWin *cws, WinState
   if (threadIdx.x == 0) fail = 0;
   put_tile(wa, v);
   __syncthreads();
-  tile_factor_inverse_lds(Dinv + cw.dinv_off, NB, &fail, wa, nullptr, wx, fs);
+  tile_factor_inverse_lds(Dinv + cw.dinv_off, NB, &fail, wa, nullptr, wx, fs, cw.n);
   __syncthreads();
   if (threadIdx.x == 0 && fail) s.chol_fail = 1;
 }
@@ -589,7 +612,7 @@ __global__ void __launch_bounds__(256) k_chol_step(const CholWin *cws, WinState 
     // li (read only by the product above) becomes its scratch
     if (threadIdx.x == 0) fail = 0;
     __syncthreads();
-    tile_factor_inverse_lds(Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB, &fail, pb, pa, li, fs);
+    tile_factor_inverse_lds(Dinv + wi.dinv_off + (int64_t)ib * NB * NB, NB, &fail, pb, pa, li, fs, wi.n - ib * NB);
     __syncthreads();
     if (threadIdx.x == 0 && fail) s.chol_fail = 1;
   } else {
@@ -731,7 +754,8 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
   for (int kb = 0; kb < wi.nb; ++kb) {
     __syncthreads();
     __threadfence_block();
-    tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, Dw + (int64_t)kb * NB * NB, NB, &fail, a, c, fs);
+    tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, Dw + (int64_t)kb * NB * NB, NB, &fail, a, c, fs,
+                        wi.n - kb * NB);
     __threadfence_block();
     __syncthreads();
     load_tile(c, Dw + (int64_t)kb * NB * NB, NB, 0, 0);

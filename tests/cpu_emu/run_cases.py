@@ -56,26 +56,32 @@ def sparse_ring(num_cameras, num_points, seed=5, kmax=4):
 
 
 def case_tiled_cholesky():
-    """n = 6 x 50 = 300 reduced rows (five 64-row tiles): FP64 step vs the
-    oracle and the device's own S solved in numpy."""
-    s = sparse_ring(50, 60)
-    o, p = Oracle(s), Problem(s, precision=abi.PRECISION_FP64)
-    assert p.n_reduced == 300
-    o.build_normal(s.x0)
-    p.build_normal(s.x0)
-    mu = 1e-3 * o.max_diagonal()
-    rc, S, rhs = p.reduced_system(mu)
-    assert rc == 0
-    rc_o, So, ro = o.reduced_system(mu)
-    assert rel(S, So) < 1e-10 and rel(rhs, ro) < 1e-10
-    rc, dg = p.solve(mu)
-    assert rc == 0
-    rc_o, do = o.solve(mu)
-    assert rel(dg, do) < 1e-8, rel(dg, do)
-    dr = dg[-300:]
-    back = np.abs(S @ dr - rhs).max() / (np.abs(S).max() * np.abs(dr).max())
-    assert back < 1e-12, back
-    return f"tiled factor n=300: step rel err {rel(dg, do):.1e}, backward err {back:.1e}"
+    """n = 6 x 50 = 300 reduced rows (five 64-row tiles, 44 real rows in the
+    last) and n = 6 x 55 = 330 (six tiles, 10 real rows in the last: its
+    factor takes the one-block path): FP64 step vs the oracle and the
+    device's own S solved in numpy."""
+    out = []
+    for cams in (50, 55):
+        n = 6 * cams
+        s = sparse_ring(cams, 60)
+        o, p = Oracle(s), Problem(s, precision=abi.PRECISION_FP64)
+        assert p.n_reduced == n
+        o.build_normal(s.x0)
+        p.build_normal(s.x0)
+        mu = 1e-3 * o.max_diagonal()
+        rc, S, rhs = p.reduced_system(mu)
+        assert rc == 0
+        rc_o, So, ro = o.reduced_system(mu)
+        assert rel(S, So) < 1e-10 and rel(rhs, ro) < 1e-10
+        rc, dg = p.solve(mu)
+        assert rc == 0
+        rc_o, do = o.solve(mu)
+        assert rel(dg, do) < 1e-8, rel(dg, do)
+        dr = dg[-n:]
+        back = np.abs(S @ dr - rhs).max() / (np.abs(S).max() * np.abs(dr).max())
+        assert back < 1e-12, back
+        out.append(f"n={n}: step rel err {rel(dg, do):.1e}, backward err {back:.1e}")
+    return "tiled factor " + "; ".join(out)
 
 
 def case_small_mma():
