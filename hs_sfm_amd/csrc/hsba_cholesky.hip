@@ -399,6 +399,16 @@ __device__ __forceinline__ void tile_factor_inverse(const double *src, int64_t l
   tile_factor_inverse_lds(dst, ld, fail, wa, nullptr, wx, ws, nreal);
 }
 
+// The same, not inlined: inside k_chol_small's loop over tiles the inlined
+// factor's LDS addresses are hoisted out of the loop and the kernel needs
+// more than 256 registers per lane (one wave per SIMD: one workgroup per CU
+// instead of two); called, it needs 243.
+__device__ __noinline__ void tile_factor_inverse_call(const double *src, int64_t lds, double *dst, int64_t ld,
+                                                      int *fail, double (*wa)[LDT], double (*wx)[LDT],
+                                                      WaveScratch16 &ws, int nreal) {
+  tile_factor_inverse(src, lds, dst, ld, fail, wa, wx, ws, nreal);
+}
+
 // 64 x 64 x 64 products on the matrix cores (v_mfma_f64_16x16x4_f64).  Wave
 // w owns the 32 x 32 quadrant (rows 32 (w / 2), columns 32 (w % 2)) as 2 x 2
 // blocks of 16 x 16.  Operand lanes: A[i][k] from lane i + 16 k, B[k][j] from
@@ -754,13 +764,13 @@ __global__ void __launch_bounds__(256) k_chol_small(const WinInfo *wis, WinState
   for (int kb = 0; kb < wi.nb; ++kb) {
     __syncthreads();
     __threadfence_block();
-    tile_factor_inverse(A + (int64_t)kb * NB * ld + kb * NB, ld, Dw + (int64_t)kb * NB * NB, NB, &fail, a, c, fs,
-                        wi.n - kb * NB);
+    tile_factor_inverse_call(A + (int64_t)kb * NB * ld + kb * NB, ld, Dw + (int64_t)kb * NB * NB, NB, &fail, a, c,
+                             fs, wi.n - kb * NB);
     __threadfence_block();
     __syncthreads();
-    load_tile(c, Dw + (int64_t)kb * NB * NB, NB, 0, 0);
-    __syncthreads();
-    // panel: L_ib = A_ib Linv^T (c holds Linv)
+    // panel: L_ib = A_ib Linv^T.  c holds Linv in its lower blocks (the
+    // triangular product reads no other block of c); a tile with at most 16
+    // real rows is the window's last, so no panel follows its short factor.
     for (int ib = kb + 1; ib < wi.nb; ++ib) {
       load_tile(a, A, ld, ib * NB, kb * NB);
       __syncthreads();
