@@ -100,6 +100,7 @@ struct Engine final : EngineBase {
       if (pg_graph[i]) (void)hipGraphDestroy(pg_graph[i]);
     }
     if (st) (void)hipStreamSynchronize(st);
+    if (pcg_host) (void)hipHostFree(pcg_host);
     for (void *p : owned) {
       if (arena) arena->release(p);
       else (void)hipFree(p);
@@ -531,7 +532,7 @@ struct Engine final : EngineBase {
   static constexpr int kPcgChunk = 8;
   hipGraphExec_t pg_exec[3] = {};
   hipGraph_t pg_graph[3] = {};
-  std::vector<PcgState> pcg_host;
+  PcgState *pcg_host = nullptr;  // pinned: the convergence flags read back between chunks
 
   void enqueue_pcg_part(int part) {
     if (part == 0) {
@@ -566,7 +567,10 @@ struct Engine final : EngineBase {
       if (e1 != hipSuccess) return HSBA_ERR_DEVICE;
       HCHK(hipGraphInstantiate(&pg_exec[part], pg_graph[part], nullptr, nullptr, 0));
     }
-    pcg_host.resize(L.nwin);
+    if (!pcg_host && hipHostMalloc((void **)&pcg_host, sizeof(PcgState) * std::max(L.nwin, 1)) != hipSuccess) {
+      pcg_host = nullptr;
+      return HSBA_ERR_OOM;
+    }
     return HSBA_OK;
   }
 
@@ -576,10 +580,10 @@ struct Engine final : EngineBase {
     const int chunks = (pcg_max_iterations + kPcgChunk - 1) / kPcgChunk;
     for (int c = 0; c < chunks; ++c) {
       HCHK(hipGraphLaunch(pg_exec[1], st));
-      HCHK(hipMemcpyAsync(pcg_host.data(), d.pcg, sizeof(PcgState) * L.nwin, hipMemcpyDeviceToHost, st));
+      HCHK(hipMemcpyAsync(pcg_host, d.pcg, sizeof(PcgState) * L.nwin, hipMemcpyDeviceToHost, st));
       HCHK(hipStreamSynchronize(st));
       bool all = true;
-      for (const PcgState &g : pcg_host) all = all && g.done;
+      for (int w = 0; w < L.nwin; ++w) all = all && pcg_host[w].done;
       if (all) break;
     }
     return HSBA_OK;

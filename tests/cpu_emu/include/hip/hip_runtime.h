@@ -134,6 +134,11 @@ inline hipError_t hipMalloc(void **p, size_t bytes) {
   return hipSuccess;
 }
 inline hipError_t hipFree(void *p) { std::free(p); return hipSuccess; }
+inline hipError_t hipHostMalloc(void **p, size_t bytes, unsigned = 0) {
+  *p = std::malloc(bytes ? bytes : 1);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipHostFree(void *p) { std::free(p); return hipSuccess; }
 inline hipError_t hipMemcpy(void *d, const void *s, size_t n, hipMemcpyKind) { std::memcpy(d, s, n); return hipSuccess; }
 inline hipError_t hipMemcpyAsync(void *d, const void *s, size_t n, hipMemcpyKind k, hipStream_t) {
   if (hipemu::capture) { std::fprintf(stderr, "hipemu: memcpy during capture\n"); std::abort(); }
