@@ -538,13 +538,22 @@ __device__ __forceinline__ bool chol_active(const WinState &s) { return s.active
 // workgroups of later launches return at once.
 
 // Diagonal tile 0 (before the first step).
-__global__ void __launch_bounds__(256) k_chol_first(const CholWin *cws, WinState *ws, const double *S, double *Dinv) {
+// Workgroup 0: diagonal tile 0.  Workgroups 1..: b = rhs with zero padding
+// (the step launches then update b in place).
+__global__ void __launch_bounds__(256) k_chol_first(const CholWin *cws, WinState *ws, const double *S, double *Dinv,
+                                                    const double *rhs, double *b) {
   __shared__ double wa[NB][LDT];
   __shared__ double wx[NB][LDT];
   __shared__ WaveScratch16 fs;
   __shared__ int fail;
   const CholWin cw = cws[blockIdx.y];
   WinState &s = ws[cw.win];
+  if (blockIdx.x > 0) {
+    if (!chol_active(s)) return;
+    for (int i = (blockIdx.x - 1) * blockDim.x + threadIdx.x; i < cw.ld; i += (gridDim.x - 1) * blockDim.x)
+      b[cw.r_off + i] = i < cw.n ? rhs[cw.r_off + i] : 0.0;
+    return;
+  }
   double v[16];
   fetch_tile(v, S + cw.s_off, cw.ld, 0, 0);  // in flight while the window's state is read
   if (!chol_active(s)) return;
@@ -956,15 +965,6 @@ __global__ void k_pad_identity(const WinInfo *wis, int nwin, double *S) {
   }
 }
 
-// b = rhs with zero padding (large windows: the step kernels update b in place).
-__global__ void k_rhs_copy(const CholWin *cws, WinState *ws, const double *rhs, double *b) {
-  const CholWin wi = cws[blockIdx.y];
-  const WinState &s = ws[wi.win];
-  if (!chol_active(s)) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < wi.ld; i += gridDim.x * blockDim.x)
-    b[wi.r_off + i] = i < wi.n ? rhs[wi.r_off + i] : 0.0;
-}
-
 void chol_init_padding(const CholPlan &plan, hipStream_t st) {
   if (plan.nwin == 0) return;
   hipLaunchKernelGGL(k_pad_identity, dim3(64, plan.nwin), dim3(256), 0, st, plan.wi, plan.nwin, plan.S);
@@ -980,10 +980,8 @@ void chol_factor_solve_impl(const CholPlan &plan, const int *small_list_dev, int
     // for the window with the most work in that launch
     int max_nb = 0;
     for (int q = 0; q < num_large; ++q) max_nb = std::max(max_nb, plan.wi_host[large_host[q]].nb);
-    const dim3 win_y(1, num_large);
-    hipLaunchKernelGGL(k_rhs_copy, dim3(8, num_large), dim3(256), 0, st, plan.large_wins, plan.ws, plan.rhs,
-                       plan.bwork);
-    hipLaunchKernelGGL(k_chol_first, win_y, dim3(256), 0, st, plan.large_wins, plan.ws, plan.S, plan.Dinv);
+    hipLaunchKernelGGL(k_chol_first, dim3(1 + 8, num_large), dim3(256), 0, st, plan.large_wins, plan.ws, plan.S,
+                       plan.Dinv, plan.rhs, plan.bwork);
     for (int k = 0; k + 1 < max_nb; ++k) {
       int slots = 0;
       for (int q = 0; q < num_large; ++q) {
