@@ -56,12 +56,12 @@ def sparse_ring(num_cameras, num_points, seed=5, kmax=4):
 
 
 def case_tiled_cholesky():
-    """n = 6 x 50 = 300 reduced rows (five 64-row tiles, 44 real rows in the
-    last) and n = 6 x 55 = 330 (six tiles, 10 real rows in the last: its
-    factor takes the one-block path): FP64 step vs the oracle and the
-    device's own S solved in numpy."""
+    """n = 6 x 55 = 330 reduced rows (six 64-row tiles, 10 real rows in the
+    last: its factor takes the one-block path; HSBA_EMU_TILED="50,55" adds
+    n = 300, five tiles with 44 real rows in the last): FP64 step vs the
+    oracle and the device's own S solved in numpy."""
     out = []
-    for cams in (50, 55):
+    for cams in [int(c) for c in os.environ.get("HSBA_EMU_TILED", "55").split(",")]:
         n = 6 * cams
         s = sparse_ring(cams, 60)
         o, p = Oracle(s), Problem(s, precision=abi.PRECISION_FP64)
