@@ -100,6 +100,10 @@ struct Engine final : EngineBase {
       if (pg_graph[i]) (void)hipGraphDestroy(pg_graph[i]);
     }
     if (st) (void)hipStreamSynchronize(st);
+    for (int i = 0; i < kPairVariants; ++i) {
+      if (pair_st[i]) (void)hipStreamSynchronize(pair_st[i]), (void)hipStreamDestroy(pair_st[i]);
+      if (pair_ev[i]) (void)hipEventDestroy(pair_ev[i]);
+    }
     if (pcg_host) (void)hipHostFree(pcg_host);
     for (void *p : owned) {
       if (arena) arena->release(p);
@@ -278,9 +282,33 @@ struct Engine final : EngineBase {
   int plist_n[kPairVariants] = {0, 0, 0};
   bool plist_all[kPairVariants] = {false, false, false};
 
+  // Pair kernels of the variants in use.  Each writes its own pairs' blocks,
+  // so with more than one variant (a batch whose window classes took
+  // different kernels) they run as concurrent branches: the others on helper
+  // streams forked from and joined back to st by events (branches of the
+  // captured graph), the one with the most pairs on st.
+  hipStream_t pair_st[kPairVariants] = {};
+  hipEvent_t pair_ev[kPairVariants] = {};
+  bool pair_fork = false;
   void enqueue_pairs() {
+    int order[kPairVariants], na = 0;
     for (int v = 0; v < kPairVariants; ++v)
-      if (plist_n[v]) ks->pairs(d, st, v, plist_all[v] ? nullptr : plist_dev[v], plist_n[v]);
+      if (plist_n[v]) order[na++] = v;
+    std::sort(order, order + na, [&](int a, int b) { return plist_n[a] > plist_n[b]; });
+    if (na <= 1 || !pair_fork) {
+      for (int i = 0; i < na; ++i) ks->pairs(d, st, order[i], plist_all[order[i]] ? nullptr : plist_dev[order[i]],
+                                              plist_n[order[i]]);
+      return;
+    }
+    (void)hipEventRecord(pair_ev[0], st);
+    for (int i = 1; i < na; ++i) {
+      const int v = order[i];
+      (void)hipStreamWaitEvent(pair_st[i], pair_ev[0], 0);
+      ks->pairs(d, pair_st[i], v, plist_all[v] ? nullptr : plist_dev[v], plist_n[v]);
+      (void)hipEventRecord(pair_ev[i], pair_st[i]);
+    }
+    ks->pairs(d, st, order[0], plist_all[order[0]] ? nullptr : plist_dev[order[0]], plist_n[order[0]]);
+    for (int i = 1; i < na; ++i) (void)hipStreamWaitEvent(st, pair_ev[i], 0);
   }
 
   // Average device time (ms) of `reps` launches of pair kernel `variant` over
@@ -370,6 +398,15 @@ struct Engine final : EngineBase {
         if (rc) return rc;
         if ((rc = upload(plist_dev[v], lists[v], st))) return rc;
       }
+    }
+    int na = 0;
+    for (int v = 0; v < kPairVariants; ++v) na += plist_n[v] > 0;
+    if (na > 1 && !pair_fork) {
+      for (int i = 0; i < kPairVariants; ++i) {
+        HCHK(hipEventCreateWithFlags(&pair_ev[i], hipEventDisableTiming));
+        if (i > 0) HCHK(hipStreamCreateWithFlags(&pair_st[i], hipStreamNonBlocking));
+      }
+      pair_fork = true;
     }
     HCHK(hipStreamSynchronize(st));
     return HSBA_OK;

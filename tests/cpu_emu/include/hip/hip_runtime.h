@@ -181,6 +181,10 @@ inline double hipemu_now() {
   return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 inline hipError_t hipEventCreate(hipEvent_t *e) { *e = new hipemuEvent{0.0}; return hipSuccess; }
+#define hipEventDisableTiming 2
+inline hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned) { return hipEventCreate(e); }
+// every stream is the one in-order queue of the emulation: waits are no-ops
+inline hipError_t hipStreamWaitEvent(hipStream_t, hipEvent_t, unsigned) { return hipSuccess; }
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t) {
   if (hipemu::capture) hipemu::capture->push_back([e]() { e->t = hipemu_now(); });
