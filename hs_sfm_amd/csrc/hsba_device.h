@@ -73,6 +73,11 @@ __host__ __device__ constexpr int z_stride(int e, int tbytes) {
   return ((3 * e + 3) * tbytes + 127) / 128 * (128 / tbytes);
 }
 
+// Per-image quantities of the angle-axis rotation (rot_quantities).
+struct RotQ {
+  double k0, k1, k2, ct, st, s, c, ds, dc, mst, theta, theta2;
+};
+
 // Keys per image chunk of the image-block reduction.
 constexpr int kImgChunk = 512;
 
@@ -119,6 +124,7 @@ struct Dev {
   int32_t nich, pad_ich_;
   T *ipart;               // [chunks][E(E+1)/2 + E] partial image blocks
   double *imgs, *imgs_new;
+  RotQ *imgq, *imgq_new;   // [I] rot_quantities of imgs / imgs_new
   T *Dimg, *gr, *rp;
   // cameras
   const int32_t *cam_ptr, *cam_imgs, *cam_win, *cam_row;
@@ -166,30 +172,62 @@ __device__ __forceinline__ double sel10(const double a[kFull], int j) {
   return v;
 }
 
+// Per-image quantities of the angle-axis rotation r (theta = |r|): the unit
+// axis k (zero for theta = 0), cos / sin of theta and the Rodrigues
+// derivative coefficients s = sin(theta)/theta, c = (1 - cos(theta))/theta^2,
+// ds, dc, mst = -sin(theta)/theta, with a Taylor branch below theta = 1e-4
+// (the reference divides by theta).  Every key of the image uses the same
+// values, so they are formed once per image (k_img_rot, k_step) and read by
+// the per-key kernels (struct RotQ).
+__device__ __forceinline__ RotQ rot_quantities(const double r[3]) {
+  RotQ q;
+  const double r0 = r[0], r1 = r[1], r2 = r[2];
+  q.theta2 = r0 * r0 + r1 * r1 + r2 * r2;
+  q.theta = sqrt(q.theta2);
+  sincos(q.theta, &q.st, &q.ct);
+  if (q.theta == 0.0) {
+    q.k0 = q.k1 = q.k2 = 0.0;
+  } else {
+    q.k0 = r0 / q.theta; q.k1 = r1 / q.theta; q.k2 = r2 / q.theta;
+  }
+  const double theta = q.theta, theta2 = q.theta2, ct = q.ct, st = q.st;
+  if (theta < 1e-4) {
+    q.s = 1.0 - theta2 / 6.0;
+    q.c = 0.5 - theta2 / 24.0;
+    q.ds = -1.0 / 3.0 + theta2 / 30.0;
+    q.dc = -1.0 / 12.0 + theta2 / 180.0;
+    q.mst = -q.s;
+  } else {
+    q.s = st / theta;
+    q.c = (1.0 - ct) / theta2;
+    q.ds = (ct * theta - st) / theta2 / theta;
+    q.dc = (st * theta - 2.0 * (1.0 - ct)) / (theta2 * theta2);
+    q.mst = -st / theta;
+  }
+  return q;
+}
+
 // Residual and analytic Jacobian of one key, FP64.
-//   X point, r/t image rotation & translation, cam full intrinsics, uv observation
+//   X point, r/t image rotation & translation (q: rot_quantities of r), cam
+//   full intrinsics, uv observation
 // Outputs: res (f(x) - y), Jp [2][3], Ji [2][6], Jc [2][D] (free parameters).
 // Rodrigues terms f = cos(theta) X, s v with s = sin(theta)/theta, v = r x X,
 // c u with c = (1 - cos(theta))/theta^2, u = (r.X) r as in
-// camera_shared_analytical_jacobian_matrix_calculator.hpp:334-429, with a
-// Taylor branch below theta = 1e-4 (the reference divides by theta).
+// camera_shared_analytical_jacobian_matrix_calculator.hpp:334-429.
 template <int D, bool WANT_J>
-__device__ __forceinline__ void key_linearize(const double X[3], const double img[6],
-                                              const double cam[kFull], const int fidx[kFull],
-                                              double u_obs, double v_obs, double res[2],
-                                              double Jp[2][3], double Ji[2][6],
-                                              double Jc[2][D > 0 ? D : 1]) {
+__device__ __forceinline__ void key_linearize_q(const double X[3], const double img[6], const RotQ &q,
+                                                const double cam[kFull], const int fidx[kFull],
+                                                double u_obs, double v_obs, double res[2],
+                                                double Jp[2][3], double Ji[2][6],
+                                                double Jc[2][D > 0 ? D : 1]) {
   const double r0 = img[0], r1 = img[1], r2 = img[2];
-  const double theta2 = r0 * r0 + r1 * r1 + r2 * r2;
-  const double theta = sqrt(theta2);
+  const double ct = q.ct, st = q.st;
   double y[3];
-  double ct, st;
-  sincos(theta, &st, &ct);
   // projection exactly as camera_shared_vector_function.hpp:688-710
-  if (theta == 0.0) {
+  if (q.theta == 0.0) {
     y[0] = X[0] + img[3]; y[1] = X[1] + img[4]; y[2] = X[2] + img[5];
   } else {
-    const double k0 = r0 / theta, k1 = r1 / theta, k2 = r2 / theta;
+    const double k0 = q.k0, k1 = q.k1, k2 = q.k2;
     const double kx0 = k1 * X[2] - k2 * X[1], kx1 = k2 * X[0] - k0 * X[2], kx2 = k0 * X[1] - k1 * X[0];
     const double kd = k0 * X[0] + k1 * X[1] + k2 * X[2];
     const double omc = 1.0 - ct;
@@ -211,20 +249,7 @@ __device__ __forceinline__ void key_linearize(const double X[3], const double im
 
   // derivative of the un-normalised camera point y w.r.t. r and X
   const double dot = r0 * X[0] + r1 * X[1] + r2 * X[2];
-  double s, c, ds, dc, mst;
-  if (theta < 1e-4) {
-    s = 1.0 - theta2 / 6.0;
-    c = 0.5 - theta2 / 24.0;
-    ds = -1.0 / 3.0 + theta2 / 30.0;
-    dc = -1.0 / 12.0 + theta2 / 180.0;
-    mst = -s;
-  } else {
-    s = st / theta;
-    c = (1.0 - ct) / theta2;
-    ds = (ct * theta - st) / theta2 / theta;
-    dc = (st * theta - 2.0 * (1.0 - ct)) / (theta2 * theta2);
-    mst = -st / theta;
-  }
+  const double s = q.s, c = q.c, ds = q.ds, dc = q.dc, mst = q.mst;
   const double rv[3] = {r0, r1, r2};
   const double v[3] = {r1 * X[2] - r2 * X[1], r2 * X[0] - r0 * X[2], r0 * X[1] - r1 * X[0]};
   const double dvdr[3][3] = {{0.0, X[2], -X[1]}, {-X[2], 0.0, X[0]}, {X[1], -X[0], 0.0}};
@@ -294,6 +319,17 @@ __device__ __forceinline__ void key_linearize(const double X[3], const double im
       Jc[1][q2] = c1;
     }
   }
+}
+
+// The same with the rotation quantities formed here.
+template <int D, bool WANT_J>
+__device__ __forceinline__ void key_linearize(const double X[3], const double img[6],
+                                              const double cam[kFull], const int fidx[kFull],
+                                              double u_obs, double v_obs, double res[2],
+                                              double Jp[2][3], double Ji[2][6],
+                                              double Jc[2][D > 0 ? D : 1]) {
+  const RotQ q = rot_quantities(img);
+  key_linearize_q<D, WANT_J>(X, img, q, cam, fidx, u_obs, v_obs, res, Jp, Ji, Jc);
 }
 
 // Whitening by the 2x2 inverse covariance Lambda = L L^T: J^ = L^T J, r^ = L^T r,

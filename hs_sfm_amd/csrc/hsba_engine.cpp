@@ -183,7 +183,7 @@ struct Engine final : EngineBase {
     double *key_cov; A(key_cov, 3 * L.N);
     A(d.pts, 3 * L.P); A(d.pts_new, 3 * L.P); A(d.dpt, 3 * L.P);
     A(d.V, 6 * L.P); A(d.gp, 3 * L.P); A(d.Linv, 6 * L.P); A(d.q, 3 * L.P); A(d.tpt, 3 * L.P);
-    A(d.imgs, 6 * L.I); A(d.imgs_new, 6 * L.I);
+    A(d.imgs, 6 * L.I); A(d.imgs_new, 6 * L.I); A(d.imgq, L.I); A(d.imgq_new, L.I);
     A(d.Dimg, (int64_t)E * E * L.I); A(d.gr, (int64_t)E * L.I); A(d.rp, (int64_t)E * L.I);
     A(d.cam_free, (int64_t)std::max(D, 1) * L.C); A(d.cam_free_new, (int64_t)std::max(D, 1) * L.C);
     A(d.cam_full, kFull * L.C); A(d.cam_full_new, kFull * L.C);
@@ -249,6 +249,7 @@ struct Engine final : EngineBase {
     if (!L.pts_free && (rc = upload(d.pts, L.fix_pts, st))) return rc;
     if (!L.pts_free && (rc = upload(d.pts_new, L.fix_pts, st))) return rc;
     if (!L.imgs_free && (rc = upload(d.imgs, L.fix_imgs, st))) return rc;
+    if (!L.imgs_free) ks->image_rotations(d, st);
     if (!L.imgs_free && (rc = upload(d.imgs_new, L.fix_imgs, st))) return rc;
     if (!L.cams_free) {
       std::vector<double> full = L.cam_base;
@@ -437,6 +438,7 @@ struct Engine final : EngineBase {
     int rc = 0;
     if (L.pts_free && ((rc = upload(d.pts, pts, st)) || (rc = upload(d.pts_new, pts, st)))) return rc;
     if (L.imgs_free && ((rc = upload(d.imgs, imgs, st)) || (rc = upload(d.imgs_new, imgs, st)))) return rc;
+    if (L.imgs_free) ks->image_rotations(d, st);
     if (L.cams_free && ((rc = upload(d.cam_free, camf, st)) || (rc = upload(d.cam_free_new, camf, st)) ||
                         (rc = upload(d.cam_full, full, st)) || (rc = upload(d.cam_full_new, full, st))))
       return rc;

@@ -32,6 +32,8 @@ struct KernelSet {
   virtual void grad_stats(const Dev<T> &d, hipStream_t st) = 0;
   virtual void lm_control(const Dev<T> &d, hipStream_t st, int phase) = 0;
   virtual void commit(const Dev<T> &d, hipStream_t st) = 0;
+  // rotation quantities of the current image rotations (after x is set)
+  virtual void image_rotations(const Dev<T> &d, hipStream_t st) = 0;
   virtual void export_keys(const Dev<T> &d, hipStream_t st, double *y, double *jp, double *ji,
                            double *jc, int want_j) = 0;
   // implicit-Schur PCG

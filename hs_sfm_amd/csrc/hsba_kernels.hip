@@ -52,9 +52,10 @@ __global__ void __launch_bounds__(256) k_linearize(Dev<T> d, int glog2) {
       double img[6], cam[kFull];
 #pragma unroll
       for (int j = 0; j < 6; ++j) img[j] = d.imgs[6 * im + j];
+      const RotQ rq = d.imgq[im];
       load_cam(d, d.img_cam[im], false, cam);
       double res[2], Jp[2][3], Ji[2][6], Jc[2][D > 0 ? D : 1];
-      key_linearize<D, true>(X, img, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
+      key_linearize_q<D, true>(X, img, rq, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
       if (d.has_cov) {
         const double *lt = d.key_cov + 3 * k;
         whiten2(lt, res[0], res[1]);
@@ -137,6 +138,7 @@ __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
   double img[6], cam[kFull];
 #pragma unroll
   for (int j = 0; j < 6; ++j) img[j] = d.imgs[6 * im + j];
+  const RotQ rq = d.imgq[im];
   load_cam(d, d.img_cam[im], false, cam);
   const int kb = d.ich_begin[ch], ke = d.ich_end[ch];
   for (int t = kb + threadIdx.x; t < ke; t += blockDim.x) {
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
     const int p = d.key_pt[k];
     const double X[3] = {d.pts[3 * p], d.pts[3 * p + 1], d.pts[3 * p + 2]};
     double res[2], Jp[2][3], Ji[2][6], Jc[2][D > 0 ? D : 1];
-    key_linearize<D, true>(X, img, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
+    key_linearize_q<D, true>(X, img, rq, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
     if (d.has_cov) {
       const double *lt = d.key_cov + 3 * k;
       whiten2(lt, res[0], res[1]);
@@ -857,6 +859,12 @@ __global__ void __launch_bounds__(256) k_step(Dev<T> d) {
         const double g = d.gr[(int64_t)im * E + j];
         d.imgs_new[e] = x - dl;
         a0 += dl * dl; a1 += x * x; a2 += dl * (mu * dl + g);
+        if (j == 0) {  // the image's new rotation, formed as the three threads above form it
+          double rn[3];
+#pragma unroll
+          for (int a = 0; a < 3; ++a) rn[a] = d.imgs[e + a] - d.dred[d.img_row[im] + a];
+          d.imgq_new[im] = rot_quantities(rn);
+        }
       } else {
         const int c = e / (D > 0 ? D : 1), j = e - c * (D > 0 ? D : 1);
         const double dl = d.dred[d.cam_row[c] + j], x = d.cam_free[e];
@@ -898,6 +906,7 @@ __global__ void __launch_bounds__(256) k_cost(Dev<T> d, int use_new) {
   if (act && kind == 3) {
     const double *pts = use_new && d.pts_free ? d.pts_new : d.pts;
     const double *imgs = use_new && d.imgs_free ? d.imgs_new : d.imgs;
+    const RotQ *imgq = use_new && d.imgs_free ? d.imgq_new : d.imgq;
     const double *camf = use_new && d.cams_free ? d.cam_full_new : d.cam_full;
     for (int k = d.chunk_begin[ch] + threadIdx.x; k < d.chunk_end[ch]; k += blockDim.x) {
       const int p = d.key_pt[k], im = d.key_img[k];
@@ -909,7 +918,8 @@ __global__ void __launch_bounds__(256) k_cost(Dev<T> d, int use_new) {
 #pragma unroll
       for (int j = 0; j < kFull; ++j) cam[j] = camf[(int64_t)c * kFull + j];
       double res[2], Jp[2][3], Ji[2][6], Jc[2][1];
-      key_linearize<0, false>(X, img, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
+      key_linearize_q<0, false>(X, img, imgq[im], cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji,
+                                Jc);
       if (d.has_cov) whiten2(d.key_cov + 3 * k, res[0], res[1]);
       acc += robust_cost(d.loss, d.loss_scale, res[0] * res[0] + res[1] * res[1]);
     }
@@ -1118,6 +1128,15 @@ __global__ void k_constraint_cost(Dev<T> d, int use_new) {
   }
 }
 
+// Rotation quantities of every image's current rotation (after x is set
+// from the host); one thread per image.
+template <typename T>
+__global__ void k_img_rot(Dev<T> d) {
+  const int im = blockIdx.x * blockDim.x + threadIdx.x;
+  if (im >= d.I) return;
+  d.imgq[im] = rot_quantities(d.imgs + 6 * (int64_t)im);
+}
+
 // Commit x_new -> x for windows whose step was accepted.
 template <typename T, int IMG, int D>
 __global__ void k_commit(Dev<T> d) {
@@ -1129,7 +1148,10 @@ __global__ void k_commit(Dev<T> d) {
     if (d.ws[d.pt_win[e / 3]].step_accepted) d.pts[e] = d.pts_new[e];
   } else if (e < np + ni) {
     const int64_t j = e - np;
-    if (d.ws[d.img_win[j / 6]].step_accepted) d.imgs[j] = d.imgs_new[j];
+    if (d.ws[d.img_win[j / 6]].step_accepted) {
+      d.imgs[j] = d.imgs_new[j];
+      if (j % 6 == 0) d.imgq[j / 6] = d.imgq_new[j / 6];
+    }
   } else if (e < np + ni + nc) {
     const int64_t j = e - np - ni;
     const int c = (int)(j / kFull), q = (int)(j - (int64_t)c * kFull);
@@ -1703,6 +1725,9 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void lm_control(const Dev<T> &d, hipStream_t st, int phase) override {
     hipLaunchKernelGGL((k_lm_control<T>), dim3(d.nwin), dim3(256), 0, st, d, phase);
+  }
+  void image_rotations(const Dev<T> &d, hipStream_t st) override {
+    if (d.I > 0) hipLaunchKernelGGL((k_img_rot<T>), dim3((d.I + 255) / 256), dim3(256), 0, st, d);
   }
   void commit(const Dev<T> &d, hipStream_t st) override {
     const int64_t n = (d.pts_free ? 3LL * d.P : 0) + (d.imgs_free ? 6LL * d.I : 0) +
