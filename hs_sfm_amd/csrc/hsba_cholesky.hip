@@ -675,7 +675,10 @@ __device__ __forceinline__ double reduce16(double part, double (*red)[NB]) {
 //   x_k = Linv_k^T v,
 // with every tile of the chain fetched into registers before the first
 // product (the chain is then bound by its reductions, not by memory latency);
-// workgroup ib < k_lo then applies y_ib -= sum_{m = k_lo..k_hi} L_m,ib^T x_m.
+// workgroup ib < k_lo then applies y_ib -= sum_{m = k_lo..k_hi} L_m,ib^T x_m,
+// its tiles of that update fetched while the chain runs (each into the
+// registers of chain tiles already used, so the peak register count stays
+// that of the chain).
 __global__ void __launch_bounds__(BT) k_chol_back(const CholWin *cws, WinState *ws, const double *Lf,
                                                   const double *Dinv, const double *b, double *y, double *x, int q) {
   __shared__ double xs[4][NB];
@@ -710,6 +713,9 @@ __global__ void __launch_bounds__(BT) k_chol_back(const CholWin *cws, WinState *
     yv[a] = (t < NB && a < nk) ? ((last && a == 0) ? b[wi.r_off + k_hi * NB + t] : y[wi.r_off + (k_hi - a) * NB + t])
                                : 0.0;
   if (!chol_active(s)) return;  // read while the tiles above are in flight
+  const int ib = blockIdx.x;      // < k_lo when k_lo > 0
+  double mu[4][4];                // L_{k_hi - a, ib}^T for the update of y_ib
+  auto fetch_upd = [&](int a) { fetch4(mu[a], L + (int64_t)(k_hi - a) * NB * ld + ib * NB, ld, true); };
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     if (a < nk) {
@@ -731,22 +737,21 @@ __global__ void __launch_bounds__(BT) k_chol_back(const CholWin *cws, WinState *
       __syncthreads();
       const double xk = reduce16(part4(di[a], vs), red);
       if (t < NB) xs[a][t] = xk;
+      // step a has used its chain tiles: the update tiles take their place
+      if (k_lo > 0) {
+        fetch_upd(a);
+        if (a + 2 == nk) fetch_upd(a + 1);
+      }
       __syncthreads();
     }
   }
   if (blockIdx.x == 0 && t < NB)
     for (int a = 0; a < nk; ++a) x[wi.r_off + (k_hi - a) * NB + t] = xs[a][t];
   if (k_lo == 0) return;
-  const int ib = blockIdx.x;  // < k_lo
   double part = 0.0;
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    if (a < nk) {
-      double m[4];
-      fetch4(m, L + (int64_t)(k_hi - a) * NB * ld + ib * NB, ld, true);
-      part += part4(m, xs[a]);
-    }
-  }
+  for (int a = 0; a < 4; ++a)
+    if (a < nk) part += part4(mu[a], xs[a]);
   const double u = reduce16(part, red);
   if (t < NB) y[wi.r_off + ib * NB + t] -= u;
 }
