@@ -1004,11 +1004,23 @@ __global__ void __launch_bounds__(256) k_lm_control(Dev<T> d, int phase) {
     }
     // thread partial sums / maxima over the window's chunks; then a fixed
     // butterfly in each wave and the four waves combined in order
+    // (selects instead of branches on the chunk kind, so that the loads of
+    // several passes are issued together; a select's neutral value leaves
+    // the sum or maximum exactly as it was)
+#pragma unroll 4
     for (int c = cb + t; c < ce; c += 256) {
-      if (d.chunk_kind[c] == 3) F += d.partial[4 * c + 3];
-      else {
-        if (phase == 0 || phase == 4) { gm = fmax(gm, d.partial[4 * c]); dm = fmax(dm, d.partial[4 * c + 1]); }
-        if (phase == 2) { dn += d.partial[4 * c]; xn += d.partial[4 * c + 1]; gd += d.partial[4 * c + 2]; }
+      const bool key_chunk = d.chunk_kind[c] == 3;
+      const double p0 = d.partial[4 * c], p1 = d.partial[4 * c + 1], p2 = d.partial[4 * c + 2],
+                   p3 = d.partial[4 * c + 3];
+      F += key_chunk ? p3 : 0.0;
+      if (phase == 0 || phase == 4) {
+        gm = fmax(gm, key_chunk ? 0.0 : p0);
+        dm = fmax(dm, key_chunk ? -1e308 : p1);
+      }
+      if (phase == 2) {
+        dn += key_chunk ? 0.0 : p0;
+        xn += key_chunk ? 0.0 : p1;
+        gd += key_chunk ? 0.0 : p2;
       }
     }
     for (int m = 1; m < 64; m <<= 1) {
