@@ -332,16 +332,103 @@ __global__ void k_point_factor(Dev<T> d) {
   d.q[3 * p + 2] = T(i20 * g0 + i21 * g1 + i22 * g2);
 }
 
+// Lanes of one wave see each other's earlier LDS writes after this point (the
+// LDS unit executes one wave's instructions in issue order; the fences keep
+// the compiler from moving LDS accesses across the call).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Z rows staged through LDS when a row is one 128-byte line of FP32.
+template <typename T, int E>
+constexpr bool kKeyZStaged = sizeof(T) == 4 && z_stride(E, sizeof(T)) == 32;
+
 // Z_k = L_p^-1 W_k and q_p, one thread per key.  The thread's W loads go out before
 // the dependent point / window / factor lookups, so they overlap that chain;
 // the Z row (padding included) leaves as whole 16-byte stores, so every
 // 128-byte line of Z is written completely.
+//
+// FP32 rows of one line: each wave moves its 64 keys' W rows (contiguous in
+// memory) and Z rows through a wave-private LDS region, so that every load
+// and store instruction covers consecutive addresses (64 x 4 bytes of W,
+// 64 x 16 bytes of Z) instead of one row per lane: per wave that is 2 lines
+// per W load and 8 per Z store, against up to 64 for row-per-lane access.
+// Rows of keys whose window is not in a trial are not written, as before.
 template <typename T, int E>
 __global__ void __launch_bounds__(256) k_key_z(Dev<T> d) {
   constexpr int BLK = 3 * E;
   constexpr int ZS = z_stride(E, sizeof(T));
   constexpr int VW = 16 / sizeof(T);  // elements per 16-byte store
   typedef T vec __attribute__((ext_vector_type(VW)));
+  if constexpr (kKeyZStaged<T, E>) {
+    constexpr int WP = BLK | 1;           // odd pitch: row reads by 64 lanes hit distinct banks
+    constexpr int ZP = ZS + VW;           // 16-byte row writes of 16 lanes hit distinct banks
+    constexpr int VPR = ZS / VW;          // 16-byte vectors per Z row
+    constexpr int REG = 64 * (WP > ZP ? WP : ZP);
+    __shared__ __align__(16) T stage[4][REG];
+    __shared__ int actf[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t k0 = (int64_t)blockIdx.x * blockDim.x + 64 * wv;  // the wave's first key
+    if (k0 >= d.N) return;                                           // whole wave past the end
+    const int nk = (int)(d.N - k0 < 64 ? d.N - k0 : 64);
+    const int k = (int)k0 + lane;
+    T *sw = stage[wv];
+    // W rows of keys k0 .. k0 + nk - 1: nk * BLK consecutive elements
+    const T *wsrc = d.W + k0 * BLK;
+    const int ne = nk * BLK;
+    T wl[BLK];
+#pragma unroll
+    for (int i = 0; i < BLK; ++i) {
+      const int e = 64 * i + lane;
+      wl[i] = e < ne ? wsrc[e] : T(0);
+    }
+    const bool valid = lane < nk;
+    const int p = valid ? d.key_pt[k] : 0;
+    const bool act = valid && win_trial(d.ws[d.pt_win[p]]);
+    T l[6], qp[3];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) l[j] = act ? d.Linv[6 * p + j] : T(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) qp[j] = act ? d.q[3 * p + j] : T(0);
+#pragma unroll
+    for (int i = 0; i < BLK; ++i) {
+      const int e = 64 * i + lane, r = e / BLK;
+      sw[r * WP + (e - r * BLK)] = wl[i];
+    }
+    actf[wv][lane] = act ? 1 : 0;
+    wave_lds_sync();
+    T w[BLK];
+#pragma unroll
+    for (int e = 0; e < BLK; ++e) w[e] = sw[lane * WP + e];
+    wave_lds_sync();  // every W row read before the region takes the Z rows
+    vec out[VPR];
+#pragma unroll
+    for (int e = 0; e < ZS; ++e) {
+      T v = 0;
+      if (e < E) v = l[0] * w[e];
+      else if (e < 2 * E) v = l[1] * w[e - E] + l[2] * w[e];
+      else if (e < BLK) v = l[3] * w[e - 2 * E] + l[4] * w[e - E] + l[5] * w[e];
+      else if (e < BLK + 3) v = qp[e - BLK];
+      out[e / VW][e % VW] = v;
+    }
+#pragma unroll
+    for (int q = 0; q < VPR; ++q) *(vec *)(sw + lane * ZP + q * VW) = out[q];
+    wave_lds_sync();
+    vec *zdst = (vec *)(d.Z + k0 * ZS);
+#pragma unroll
+    for (int q = 0; q < VPR; ++q) {
+      const int v = 64 * q + lane, r = v / VPR, c = v - r * VPR;
+      if (r < nk && actf[wv][r]) zdst[v] = *(const vec *)(sw + r * ZP + c * VW);
+    }
+    if (d.Zi && act) {
+      vec *zi = (vec *)(d.Zi + (int64_t)d.key_ipos[k] * ZS);
+#pragma unroll
+      for (int q = 0; q < VPR; ++q) zi[q] = out[q];
+    }
+    return;
+  }
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= d.N) return;
   // the W row does not depend on the point: its loads are issued first and
@@ -475,14 +562,6 @@ __global__ void __launch_bounds__(256) k_pairs(Dev<T> d, const int32_t *plist, i
 // element i / 4 for the f64 form (both checked on the device).
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
-// Lanes of one wave see each other's earlier LDS writes after this point (the
-// LDS unit executes one wave's instructions in issue order; the fences keep
-// the compiler from moving LDS accesses across the call).
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <typename T>
 struct Mma;
