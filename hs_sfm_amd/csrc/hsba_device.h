@@ -110,6 +110,7 @@ struct Dev {
   int32_t loss;           // HSBA_LOSS_*: robust loss on each key's squared whitened residual
   int32_t fidx[kFull];    // free intrinsic indices (first D valid)
   double loss_scale;      // Huber a
+  int64_t lower_max;      // the most lower-triangle entries of one window's reduced matrix
   // keys, point-major order
   const int32_t *key_img, *key_pt, *key_orig;
   const double *key_uv;   // [N][2]

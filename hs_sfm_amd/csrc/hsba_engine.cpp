@@ -151,6 +151,8 @@ struct Engine final : EngineBase {
     d.P = (int32_t)L.P; d.I = (int32_t)L.I; d.C = (int32_t)L.C; d.N = (int32_t)L.N;
     d.nwin = L.nwin; d.npairs = (int32_t)L.npairs; d.ncons = (int32_t)L.cons.size();
     d.nchunks = (int32_t)L.chunk_kind.size();
+    d.lower_max = 0;
+    for (const WinInfo &q : L.wi) d.lower_max = std::max<int64_t>(d.lower_max, (int64_t)q.n * (q.n + 1) / 2);
     d.pts_free = L.pts_free; d.imgs_free = L.imgs_free; d.cams_free = L.cams_free; d.has_cov = 0;
     d.pcg_rows = L.pcg ? (int32_t)L.rows_total : 0;
     d.loss = loss;

@@ -786,21 +786,13 @@ __global__ void __launch_bounds__(256) k_pairs_mma_wave(Dev<T> d, const int32_t 
 // One thread per lower-triangle entry of a window's reduced matrix (FP64):
 // S = U + mu I (+ image-camera coupling) - sum of pair blocks, by the
 // per-block contribution lists built at problem creation.
+// Entry u (row-major lower-triangle order) of window w.
 template <typename T, int IMG, int D>
-__global__ void k_assemble(Dev<T> d, int64_t total) {
+__device__ __forceinline__ void assemble_entry(const Dev<T> &d, int w, int64_t u) {
   constexpr int E = IMG + D;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  // window by binary search over lower-entry offsets
-  int lo = 0, hi = d.nwin - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (d.wi[mid].lo_off <= t) lo = mid; else hi = mid - 1;
-  }
-  const WinInfo &wi = d.wi[lo];
-  const WinState &s = d.ws[lo];
+  const WinInfo &wi = d.wi[w];
+  const WinState &s = d.ws[w];
   if (!win_trial(s) || s.chol_fail) return;
-  const int64_t u = t - wi.lo_off;
   int64_t R = (int64_t)((sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
   while (R * (R + 1) / 2 > u) --R;
   while ((R + 1) * (R + 2) / 2 <= u) ++R;
@@ -825,6 +817,32 @@ __global__ void k_assemble(Dev<T> d, int64_t total) {
     }
   }
   d.S[wi.s_off + R * wi.ld + Cc] = v;
+}
+
+// One thread per lower-triangle entry of all windows, the window found by
+// binary search over the windows' entry offsets.
+template <typename T, int IMG, int D>
+__global__ void k_assemble(Dev<T> d, int64_t total) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int lo = 0, hi = d.nwin - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d.wi[mid].lo_off <= t) lo = mid; else hi = mid - 1;
+  }
+  assemble_entry<T, IMG, D>(d, lo, t - d.wi[lo].lo_off);
+}
+
+// Batches of windows: blockIdx.y is the window, so no thread walks the
+// dependent loads of a search (the grid covers the largest window; threads
+// past a smaller window's entries exit at once).
+template <typename T, int IMG, int D>
+__global__ void k_assemble_win(Dev<T> d) {
+  const int w = blockIdx.y;
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = d.wi[w].n;
+  if (u >= n * (n + 1) / 2) return;
+  assemble_entry<T, IMG, D>(d, w, u);
 }
 
 // Reduced right-hand side: g_r - sum of rp over the row's images.
@@ -1794,9 +1812,14 @@ struct KernelSetImpl final : KernelSet<T> {
   }
   void assemble(const Dev<T> &d, hipStream_t st, int64_t lower_total, int rows) override {
     if (E == 0 || rows == 0) return;
-    if (lower_total > 0)
-      hipLaunchKernelGGL((k_assemble<T, IMG, D>), dim3((lower_total + 255) / 256), dim3(256), 0, st, d,
-                         lower_total);
+    if (lower_total > 0) {
+      if (d.nwin > 1 && d.nwin <= 65535)
+        hipLaunchKernelGGL((k_assemble_win<T, IMG, D>), dim3((unsigned)((d.lower_max + 255) / 256), d.nwin), dim3(256),
+                           0, st, d);
+      else
+        hipLaunchKernelGGL((k_assemble<T, IMG, D>), dim3((lower_total + 255) / 256), dim3(256), 0, st, d,
+                           lower_total);
+    }
     hipLaunchKernelGGL((k_assemble_rhs<T, IMG, D>), dim3((rows + 255) / 256), dim3(256), 0, st, d, rows);
   }
   void backsub(const Dev<T> &d, hipStream_t st, int glog2) override {
