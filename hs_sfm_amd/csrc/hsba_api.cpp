@@ -106,6 +106,7 @@ int create_engine(const hsba_problem_desc *descs, int64_t count, const hsba_opti
     int g = 0;
     while ((1 << g) < o.point_group_lanes && g < 6) ++g;
     eng->glog2 = g;
+    eng->lin_by_track = false;
   }
   if (o.pair_kernel < HSBA_PAIRS_TUNED || o.pair_kernel > HSBA_PAIRS_MFMA_WAVE) return HSBA_ERR_INVALID;
   eng->pair_kernel = o.pair_kernel;

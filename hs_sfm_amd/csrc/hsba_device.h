@@ -101,6 +101,13 @@ __host__ __device__ inline double robust_row_scale(int loss, double a, double s)
   return 1.0;
 }
 
+// Thread segments of one K1 launch: segment s covers threads [t0[s], t0[s+1])
+// (multiples of 64, so a wave lies in one segment), 2^g[s] lanes per point,
+// points plist[base[s] .. base[s] + cnt[s]) (plist null: point = index).
+struct LinSeg {
+  int32_t t0[4], g[3], base[3], cnt[3];
+};
+
 // All device buffers, passed by value to kernels.
 template <typename T>
 struct Dev {

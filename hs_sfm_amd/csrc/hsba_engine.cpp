@@ -84,6 +84,13 @@ struct Engine final : EngineBase {
   static constexpr int kMaxGraphIterations = 64;
   int *small_list_dev = nullptr;
   CholWin *large_wins_dev = nullptr;
+  // K1 point classes by track length: <= 6 keys 2 lanes, <= 12 keys 4 lanes,
+  // longer 8 lanes (the lanes of a point then follow from its own track, so a
+  // window alone and in a batch sums each point in the same order)
+  static constexpr int kLinClasses = 3;
+  int32_t *lin_list = nullptr;
+  int lin_cnt[kLinClasses] = {0, 0, 0};
+  std::vector<int32_t> lin_host;
   std::vector<CholWin> large_wins_host;  // source of the upload, kept while it is in flight
   CholPlan plan;
   std::vector<WinState> ws_host;
@@ -216,6 +223,19 @@ struct Engine final : EngineBase {
       d.ich_img = ich_img; d.ich_begin = ich_begin; d.ich_end = ich_end; d.img_ch_ptr = img_ch_ptr;
     }
     A(small_list_dev, (int64_t)L.small_list.size());
+    lin_host.clear();
+    for (int c = 0; c < kLinClasses; ++c) lin_cnt[c] = 0;
+    if (lin_by_track) {
+      for (int c = 0; c < kLinClasses; ++c)
+        for (int64_t p = 0; p < L.P; ++p) {
+          const int len = L.pt_ptr[p + 1] - L.pt_ptr[p];
+          if ((len <= 6 ? 0 : len <= 12 ? 1 : 2) == c) {
+            lin_host.push_back((int32_t)p);
+            ++lin_cnt[c];
+          }
+        }
+      A(lin_list, (int64_t)lin_host.size());
+    }
     A(large_wins_dev, (int64_t)L.large_list.size());
     large_wins_host.clear();
     for (int w : L.large_list) {
@@ -237,7 +257,8 @@ struct Engine final : EngineBase {
         (rc = upload(cons, L.cons, st)) || (rc = upload(ck, L.chunk_kind, st)) ||
         (rc = upload(cbg, L.chunk_begin, st)) || (rc = upload(cen, L.chunk_end, st)) ||
         (rc = upload(cw, L.chunk_win, st)) || (rc = upload(wi, L.wi, st)) ||
-        (rc = upload(small_list_dev, L.small_list, st)) || (rc = upload(large_wins_dev, large_wins_host, st)))
+        (rc = upload(small_list_dev, L.small_list, st)) || (rc = upload(large_wins_dev, large_wins_host, st)) ||
+        (lin_by_track && (rc = upload(lin_list, lin_host, st))))
       return rc;
     d.key_img = key_img; d.key_pt = key_pt; d.key_orig = key_orig; d.key_uv = key_uv; d.key_cov = key_cov;
     d.pt_ptr = pt_ptr; d.pt_win = pt_win;
@@ -552,8 +573,29 @@ struct Engine final : EngineBase {
     return HSBA_OK;
   }
 
+  void enqueue_lin_kernel() {
+    LinSeg seg{};
+    auto whole_waves = [](int64_t threads) { return (int32_t)((threads + 63) / 64 * 64); };
+    if (!lin_by_track) {
+      // one segment: every point in order, 2^glog2 lanes
+      seg.g[0] = seg.g[1] = seg.g[2] = glog2;
+      seg.cnt[2] = (int32_t)L.P;
+      seg.t0[3] = whole_waves(L.P << glog2);
+      ks->linearize(d, st, seg, nullptr);
+      return;
+    }
+    int32_t base = 0;
+    for (int c = 0; c < kLinClasses; ++c) {
+      seg.g[c] = c + 1;
+      seg.base[c] = base;
+      seg.cnt[c] = lin_cnt[c];
+      seg.t0[c + 1] = seg.t0[c] + whole_waves((int64_t)lin_cnt[c] << (c + 1));
+      base += lin_cnt[c];
+    }
+    ks->linearize(d, st, seg, lin_list);
+  }
   void enqueue_linearize() {
-    ks->linearize(d, st, glog2);
+    enqueue_lin_kernel();
     ks->image_reduce(d, st);
     ks->camera_reduce(d, st);
     ks->constraints(d, st);
@@ -749,7 +791,7 @@ struct Engine final : EngineBase {
         ks->lm_control(d, st, 3);
         ks->commit(d, st);
         break;
-      case 6: ks->linearize(d, st, glog2); break;  // K1 + K2
+      case 6: enqueue_lin_kernel(); break;  // K1 + K2
       default:
         ks->image_reduce(d, st);
         ks->camera_reduce(d, st);

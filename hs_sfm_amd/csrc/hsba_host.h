@@ -125,6 +125,9 @@ struct EngineBase {
   int loss = HSBA_LOSS_TRIVIAL;
   double loss_scale = 1.0;
   int glog2 = 3;
+  // K1 lanes per point from each point's own track length (2 / 4 / 8); false when
+  // the caller fixed point_group_lanes (then glog2 for every point)
+  bool lin_by_track = true;
   int pair_kernel = 0;  // HSBA_PAIRS_*: 0 tuned, else the forced pair kernel
   virtual int init(hipStream_t stream) = 0;
   virtual int upload_x(const double *const *xs) = 0;

@@ -12,7 +12,7 @@ template <typename T>
 struct KernelSet {
   virtual ~KernelSet() {}
   virtual int E_() const = 0;
-  virtual void linearize(const Dev<T> &d, hipStream_t st, int glog2) = 0;
+  virtual void linearize(const Dev<T> &d, hipStream_t st, const LinSeg &seg, const int32_t *plist) = 0;
   virtual void image_reduce(const Dev<T> &d, hipStream_t st) = 0;
   virtual void camera_reduce(const Dev<T> &d, hipStream_t st) = 0;
   virtual void constraints(const Dev<T> &d, hipStream_t st) = 0;

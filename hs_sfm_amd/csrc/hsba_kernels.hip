@@ -25,18 +25,25 @@ __device__ __forceinline__ void load_cam(const Dev<T> &d, int c, bool new_x, dou
 
 // ------------------------------------------------------------ K1+K2 linearize
 // One group of G lanes per point (points are contiguous per window; keys of a
-// point contiguous).  Each lane linearizes keys lane, lane+G, ... of its point:
+// point contiguous); the launch is cut into up to three segments of whole
+// waves with their own G (the host groups points by track length, so a
+// point's lanes, and with them its order of summation, depend on the point
+// alone).  Each lane linearizes keys lane, lane+G, ... of its point:
 // residual and Jacobian in FP64, then W_k = Jp^T L Jr (3 x E) is stored and
 // V_p = sum Jp^T L Jp, g_p = sum Jp^T L r are reduced across the group with a
 // butterfly.  Replaces the per-key loops of
 // camera_shared_analytical_jacobian_matrix_calculator.hpp:94-321 and
 // camera_shared_normal_equation_builder.hpp:207-220, 249-287, 377-391.
 template <typename T, int IMG, int D>
-__global__ void __launch_bounds__(256) k_linearize(Dev<T> d, int glog2) {
+__global__ void __launch_bounds__(256) k_linearize(Dev<T> d, LinSeg seg, const int32_t *plist) {
   constexpr int E = IMG + D;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sg = t < seg.t0[1] ? 0 : (t < seg.t0[2] ? 1 : 2);  // uniform in a wave
+  const int glog2 = seg.g[sg];
   const int G = 1 << glog2;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int p = gid >> glog2;
+  const int gid = t - seg.t0[sg];
+  const int gi = gid >> glog2;
+  const int p = (t < seg.t0[3] && gi < seg.cnt[sg]) ? (plist ? plist[seg.base[sg] + gi] : gi) : d.P;  // d.P: none
   const int lane = gid & (G - 1);
   bool act = false;
   if (p < d.P) {
@@ -1765,10 +1772,9 @@ template <typename T, int IMG, int D>
 struct KernelSetImpl final : KernelSet<T> {
   static constexpr int E = IMG + D;
   int E_() const override { return E; }
-  void linearize(const Dev<T> &d, hipStream_t st, int glog2) override {
-    if (!d.pts_free || d.P == 0) return;
-    const int64_t threads = (int64_t)d.P << glog2;
-    hipLaunchKernelGGL((k_linearize<T, IMG, D>), dim3((threads + 255) / 256), dim3(256), 0, st, d, glog2);
+  void linearize(const Dev<T> &d, hipStream_t st, const LinSeg &seg, const int32_t *plist) override {
+    if (!d.pts_free || d.P == 0 || seg.t0[3] <= 0) return;
+    hipLaunchKernelGGL((k_linearize<T, IMG, D>), dim3((seg.t0[3] + 255) / 256), dim3(256), 0, st, d, seg, plist);
   }
   void image_reduce(const Dev<T> &d, hipStream_t st) override {
     if (E == 0 || d.I == 0) return;
