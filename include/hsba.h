@@ -149,7 +149,8 @@ typedef struct {
   int32_t solver;                   /* HSBA_SOLVER_*    */
   double  pcg_rtol;                 /* relative residual tolerance (default 1e-6) */
   int32_t pcg_max_iterations;       /* default 500 */
-  int32_t point_group_lanes;        /* lanes per point in point kernels: 0 = tuned */
+  int32_t point_group_lanes;        /* lanes per point in point kernels: 0 = by each point's track
+                                       length in the linearization (2 / 4 / 8), 8 elsewhere */
   int32_t loss;                     /* HSBA_LOSS_* (0 = trivial)             */
   int32_t pair_kernel;              /* HSBA_PAIRS_* (0 = chosen by the autotuner) */
   double  loss_scale;               /* a of the Huber loss (0 = 1.0)         */
