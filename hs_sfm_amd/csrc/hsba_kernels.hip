@@ -148,9 +148,26 @@ __global__ void __launch_bounds__(128) k_image_reduce(Dev<T> d) {
   const RotQ rq = d.imgq[im];
   load_cam(d, d.img_cam[im], false, cam);
   const int kb = d.ich_begin[ch], ke = d.ich_end[ch];
-  for (int t = kb + threadIdx.x; t < ke; t += blockDim.x) {
-    const int k = d.img_keys[t];
-    const int p = d.key_pt[k];
+  // the key and point indices of this thread's keys (at most kImgChunk / 128
+  // of them) are loaded up front, so only the first key waits out the two
+  // dependent index loads; keys are summed in the same order as one by one
+  constexpr int KPT = kImgChunk / 128;
+  int kk[KPT], pk[KPT];
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    const int t = kb + (int)threadIdx.x + 128 * i;
+    kk[i] = t < ke ? d.img_keys[t] : 0;
+  }
+#pragma unroll
+  for (int i = 0; i < KPT; ++i) {
+    const int t = kb + (int)threadIdx.x + 128 * i;
+    pk[i] = t < ke ? d.key_pt[kk[i]] : 0;
+  }
+#pragma unroll 1
+  for (int i = 0; i < KPT; ++i) {
+    if (kb + (int)threadIdx.x + 128 * i >= ke) break;
+    const int k = kk[i];
+    const int p = pk[i];
     const double X[3] = {d.pts[3 * p], d.pts[3 * p + 1], d.pts[3 * p + 2]};
     double res[2], Jp[2][3], Ji[2][6], Jc[2][D > 0 ? D : 1];
     key_linearize_q<D, true>(X, img, rq, cam, d.fidx, d.key_uv[2 * k], d.key_uv[2 * k + 1], res, Jp, Ji, Jc);
